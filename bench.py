@@ -1,0 +1,139 @@
+#!/usr/bin/env python3
+"""Throughput of the plain colour sweep on one MI355X.
+
+Workload: the large linear melt (512^3 periodic box, 16384 chains of N=512,
+8,388,608 monomers, volume fraction 0.5, 2 colours, bit-packed compact-Morton
+lattice), synthetic seeded initial state.  A step is one Monte-Carlo step
+(every monomer attempts one move).  Inputs are resident in HBM before the
+timed region.  Prints one JSON line.
+
+The cpu_baseline leg times the oracle's random-sequential updater (the
+LeMonADE-style CPU updater, one core) on a bounded number of MCS of the same
+system.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    "large_melt": dict(box=512, n_chains=16384, chain_len=512, seed=0x5EED000000000002),
+    "small_melt": dict(box=64, n_chains=512, chain_len=32, seed=0x5EED000000000001),
+}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(system: dict, seed: int, target_s: float) -> dict:
+    from oracle import oracle as orc
+    osys = orc.System(system, seed)
+    n = len(system["xyz"])
+    t = time.perf_counter()
+    xyz, _ = orc.random_sequential(osys, system["xyz"], 0, 1)
+    one = time.perf_counter() - t
+    m = max(1, int(target_s / max(one, 1e-6)) - 1)
+    t = time.perf_counter()
+    orc.random_sequential(osys, xyz, 1, m)
+    dt = time.perf_counter() - t
+    return {"value": m * n / dt, "unit": "attempts/s", "cores": 1, "kind": "port",
+            "sample": f"oracle random-sequential updater, {m} MCS x {n} monomers of the same system "
+                      f"({dt:.1f} s on 1 host core)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200, help="timed Monte-Carlo steps")
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large_melt")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this engine runs every system on a single device; "
+              "use --gpus 1", file=sys.stderr)
+        return 2
+
+    from pscbfm_amd import device_count, systems, updater_from_system
+    if device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+
+    w = WORKLOADS[args.workload]
+    t = time.perf_counter()
+    system = systems.linear_melt(w["box"], w["n_chains"], w["chain_len"], w["seed"])
+    n = len(system["xyz"])
+    log(f"generated {args.workload}: {n} monomers in {time.perf_counter() - t:.1f} s")
+    t = time.perf_counter()
+    u = updater_from_system(system, seed=w["seed"], gpu=0)
+    colors, n_colors = u.getColors(n)
+    log(f"initialized in {time.perf_counter() - t:.1f} s, {n_colors} colours")
+
+    u.setKernelTiming(True)
+    u.runResident(args.warmup)
+    s0 = u.getStats()  # synchronizes the updater's stream
+    t0 = time.perf_counter()
+    u.runResident(args.steps)
+    s1 = u.getStats()
+    t1 = time.perf_counter()
+    dt = t1 - t0
+    attempts = args.steps * n
+    value = attempts / dt
+    log(f"timed {args.steps} MCS in {dt:.3f} s -> {value:.4g} attempts/s")
+
+    launches = s1["timed_launches"] - s0["timed_launches"]
+    check_ms = (s1["check_ms"] - s0["check_ms"]) / max(launches, 1)
+    perform_ms = (s1["perform_ms"] - s0["perform_ms"]) / max(launches, 1)
+    accepted = s1["accepted"] - s0["accepted"]
+    proposals = s1["proposals"] - s0["proposals"]
+    p_acc = accepted / attempts
+    kbar = 2.0 * len(system["bonds"]) / n
+    per_launch = n / n_colors                       # attempts in one check launch
+    b_check = 12.5 + 16.0 * kbar                    # own xyz + partner ids/xyz + 4 front bits
+    achieved = per_launch * b_check / (check_ms * 1e-3) / 1e9
+    b_attempt = 12.5 + 16.0 * kbar + 13.0 * p_acc   # whole substep, per attempt
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_check", "kernel_avg_ms": round(check_ms, 4),
+                "perform_avg_ms": round(perform_ms, 4), "bytes_per_attempt_check": round(b_check, 3),
+                "path_frac": round(value * b_attempt / 1e9 / HBM_PEAK_GBS, 4)}
+
+    out = {
+        "metric": "monomer move attempts/s (plain colour sweep, bit-exact)",
+        "value": value,
+        "unit": "attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded random-walk melt)",
+        "config": {"workload": args.workload, "box": w["box"], "chains": w["n_chains"],
+                   "chain_len": w["chain_len"], "monomers": n, "colours": n_colors, "mcs": args.steps,
+                   "p_acc": round(p_acc, 4), "p_proposal": round(proposals / attempts, 4)},
+        "roofline": roofline,
+    }
+    u.cleanup()
+    if not args.no_cpu:
+        log("timing the CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(system, w["seed"], args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

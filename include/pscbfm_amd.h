@@ -1,0 +1,162 @@
+/*
+ * pscbfm_amd.h -- C ABI of the MI355X bond-fluctuation-model updater.
+ *
+ * Every entry point mirrors one public member of the reference updater
+ * UpdaterGPUScBFM_AB_Type (src/pscbfm/UpdaterGPUScBFM_AB_Type.h:453-514),
+ * which the LeMonADE adapter GPUScBFM_AB_Type drives
+ * (src/pscbfm/GPUScBFM_AB_Type.h:120-251).  Plain pointers and sizes only.
+ *
+ * Errors: no C++ exception and no exit() crosses this boundary.  Every call
+ * returns BFM_OK (0) or a negative status; bfm_last_error() then returns a
+ * thread-local message.  BFM_E_INVALID_ARGUMENT stands for the reference's
+ * std::invalid_argument, BFM_E_RUNTIME for its std::runtime_error, BFM_E_HIP
+ * for a failed HIP call (the reference would exit(EXIT_FAILURE),
+ * src/pscbfm/cudacommon.hpp:36-51).
+ *
+ * Threading: every handle is independent (no module-global tables, unlike the
+ * reference's __constant__ box/bond symbols, UpdaterGPUScBFM_AB_Type.cu:84-133).
+ * A handle may be used by one host thread at a time.
+ */
+#ifndef PSCBFM_AMD_H
+#define PSCBFM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    BFM_OK = 0,
+    BFM_E_INVALID_ARGUMENT = -1,
+    BFM_E_RUNTIME = -2,
+    BFM_E_HIP = -3,
+    BFM_E_NO_DEVICE = -4,
+    BFM_E_OUT_OF_MEMORY = -5,
+    BFM_E_STATE = -6
+};
+
+/* Random number generators; value 1 matches Rng::IntHash's slot of the
+ * reference enum (UpdaterGPUScBFM_AB_Type.h:413-420) but selects this
+ * engine's counter RNG, Philox4x32-10 keyed on (seed, MCS, monomer id). */
+enum { BFM_RNG_PHILOX = 1 };
+
+typedef struct bfm_updater bfm_updater;
+
+/* Thread-local description of the last failed call ("" if none). */
+const char *bfm_last_error(void);
+
+/* Constructor / destructor (UpdaterGPUScBFM_AB_Type.cu:1372-1490). */
+int bfm_create(bfm_updater **out);
+int bfm_destroy(bfm_updater *h);
+
+/* setGpu, UpdaterGPUScBFM_AB_Type.cu:1498-1514. */
+int bfm_set_gpu(bfm_updater *h, int gpu);
+
+/* copyBondSet, UpdaterGPUScBFM_AB_Type.cu:2540-2545; forbidden != 0 marks the
+ * vector as not allowed.  dx,dy,dz in [-4,3]. */
+int bfm_copy_bond_set(bfm_updater *h, int dx, int dy, int dz, int forbidden);
+
+/* setLatticeSize, UpdaterGPUScBFM_AB_Type.cu:2639-2684: powers of two. */
+int bfm_set_lattice_size(bfm_updater *h, uint64_t box_x, uint64_t box_y, uint64_t box_z);
+
+/* setNrOfAllMonomers, UpdaterGPUScBFM_AB_Type.cu:2548-2563 (may be called once). */
+int bfm_set_nr_of_all_monomers(bfm_updater *h, uint32_t n);
+
+/* setAttribute, UpdaterGPUScBFM_AB_Type.cu:2579 (stored, not used by the sweep). */
+int bfm_set_attribute(bfm_updater *h, uint32_t i, int32_t attribute);
+
+/* setMonomerCoordinates, UpdaterGPUScBFM_AB_Type.cu:2582-2610. */
+int bfm_set_monomer_coordinates(bfm_updater *h, uint32_t i, int32_t x, int32_t y, int32_t z);
+
+/* setConnectivity, UpdaterGPUScBFM_AB_Type.cu:2617-2635: directed link i->j,
+ * at most 8 per monomer.  The LeMonADE adapter adds both directions. */
+int bfm_set_connectivity(bfm_updater *h, uint32_t i, uint32_t j);
+
+/* setPeriodicity, UpdaterGPUScBFM_AB_Type.cu:2565-2576. */
+int bfm_set_periodicity(bfm_updater *h, int periodic_x, int periodic_y, int periodic_z);
+
+/* setAge / getAge, UpdaterGPUScBFM_AB_Type.h:504-505.  The age is the MCS
+ * counter that keys the RNG, so resuming at an age continues the trajectory. */
+int bfm_set_age(bfm_updater *h, int64_t age);
+int bfm_get_age(const bfm_updater *h, int64_t *age);
+
+/* setRng, UpdaterGPUScBFM_AB_Type.h:506 (only BFM_RNG_PHILOX). */
+int bfm_set_rng(bfm_updater *h, int rng);
+
+/* Seed of the counter RNG.  The reference seeds LeMonADE's r250 from a seed
+ * file (src/pscbfm/mainCUDASimulatorGPUScBFM_AB_Type.cpp:123-134); here the
+ * 64-bit seed is the Philox key. */
+int bfm_set_seed(bfm_updater *h, uint64_t seed);
+
+/* setStepsBetweenSortings, UpdaterGPUScBFM_AB_Type.h:510-513.  Accepted for
+ * API compatibility; re-sorting never changes a trajectory here because the
+ * RNG is keyed on the original monomer id. */
+int bfm_set_steps_between_sortings(bfm_updater *h, int64_t steps);
+
+/* The host half of initialize (no device work): the checks of checkSystem
+ * (UpdaterGPUScBFM_AB_Type.cu:2789-2900) and the bond-set count (:1526), then
+ * the colouring of initializeSpeciesSorting (:1561-1686).  Lets a caller
+ * validate a system and read its colours on a machine without a GPU. */
+int bfm_validate(bfm_updater *h);
+
+/* initialize, UpdaterGPUScBFM_AB_Type.cu:2393-2537: validates the system
+ * (bond set of 108 vectors, overlap-free cubes, bonds in the set), colours
+ * the bond graph, lays the system out on the device. */
+int bfm_initialize(bfm_updater *h);
+
+/* runSimulationOnGPU, UpdaterGPUScBFM_AB_Type.cu:2902-3400: n_mcs Monte-Carlo
+ * steps, then positions are copied back (doCopyBack, :3402-3484) and the age
+ * advances by n_mcs. */
+int bfm_run_simulation_on_gpu(bfm_updater *h, uint32_t n_mcs);
+
+/* getMonomerPositionIn{X,Y,Z}, UpdaterGPUScBFM_AB_Type.cu:2612-2614
+ * (unwrapped coordinates). */
+int bfm_get_monomer_position_in_x(const bfm_updater *h, uint32_t i, int32_t *x);
+int bfm_get_monomer_position_in_y(const bfm_updater *h, uint32_t i, int32_t *y);
+int bfm_get_monomer_position_in_z(const bfm_updater *h, uint32_t i, int32_t *z);
+
+/* cleanup, UpdaterGPUScBFM_AB_Type.cu:3493-3502: frees everything set up by
+ * setNrOfAllMonomers and initialize; the handle can be set up again. */
+int bfm_cleanup(bfm_updater *h);
+
+/* ---- bulk extensions (no one-call-per-monomer loop over the ABI) ---------- */
+
+/* xyz: 3*n int32, monomer-major.  Same checks as the single-monomer calls. */
+int bfm_set_monomer_coordinates_bulk(bfm_updater *h, const int32_t *xyz, uint32_t n);
+/* pairs: 2*n_links uint32 (i, j), each a directed link i->j. */
+int bfm_set_connectivity_bulk(bfm_updater *h, const uint32_t *pairs, uint64_t n_links);
+int bfm_get_monomer_positions(const bfm_updater *h, int32_t *xyz, uint32_t n);
+
+/* Colour of every monomer after bfm_initialize, and the colour count. */
+int bfm_get_colors(const bfm_updater *h, uint32_t *color, uint32_t n, uint32_t *n_colors);
+
+/* Counters accumulated since bfm_initialize. */
+typedef struct bfm_stats {
+    uint64_t attempts;       /* monomer move attempts (MCS x monomers)        */
+    uint64_t proposals;      /* attempts that passed walls, bonds, volume     */
+    uint64_t accepted;       /* moves applied                                  */
+    uint64_t substeps;       /* colour substeps run                            */
+    double   check_ms;       /* summed duration of check kernels (if timed)   */
+    double   perform_ms;     /* summed duration of perform kernels (if timed) */
+    uint64_t timed_launches; /* check launches covered by check_ms             */
+} bfm_stats;
+int bfm_get_stats(const bfm_updater *h, bfm_stats *out);
+
+/* When on, every check/perform launch is bracketed by HIP events on the
+ * handle's stream and the durations are summed into bfm_stats. */
+int bfm_set_kernel_timing(bfm_updater *h, int on);
+
+/* Runs n_mcs steps without the copy-back of runSimulationOnGPU (positions
+ * stay on the device; the age still advances).  For timing loops. */
+int bfm_run_resident(bfm_updater *h, uint32_t n_mcs);
+/* Copies the device state back to the host (the doCopyBack half). */
+int bfm_copy_back(bfm_updater *h);
+
+/* Number of HIP devices visible (0 on a machine without a GPU). */
+int bfm_device_count(int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,22 @@
+"""MI355X-native bond-fluctuation-model Monte Carlo engine.
+
+The hot path (graph-coloured species sweep) runs as hand-written HIP kernels
+for gfx950 in lib/libpscbfm_amd.so; this package is the host-side face of its
+C ABI (include/pscbfm_amd.h).
+"""
+from ._capi import BfmError, HipError, InvalidArgument, NoDeviceError, StateError, device_count, load
+from .updater import GPUScBFM_AB_Type, Rng_Philox, UpdaterGPUScBFM_AB_Type, updater_from_system
+
+__all__ = [
+    "BfmError",
+    "GPUScBFM_AB_Type",
+    "HipError",
+    "InvalidArgument",
+    "NoDeviceError",
+    "Rng_Philox",
+    "StateError",
+    "UpdaterGPUScBFM_AB_Type",
+    "device_count",
+    "load",
+    "updater_from_system",
+]

@@ -1,0 +1,138 @@
+"""ctypes binding of the C ABI in include/pscbfm_amd.h and pscbfm_amd_systems.h.
+
+The library is pscbfm_amd/lib/libpscbfm_amd.so, built by ``python -m
+pscbfm_amd.build``.  There is no fallback: without the library every call
+raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_int64, c_uint32, c_uint64, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libpscbfm_amd.so"
+
+BFM_OK = 0
+BFM_E_INVALID_ARGUMENT = -1
+BFM_E_RUNTIME = -2
+BFM_E_HIP = -3
+BFM_E_NO_DEVICE = -4
+BFM_E_OUT_OF_MEMORY = -5
+BFM_E_STATE = -6
+
+
+class BfmError(RuntimeError):
+    """A failed call; .code holds the BFM_E_* status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class InvalidArgument(BfmError, ValueError):
+    """The reference's std::invalid_argument."""
+
+
+class HipError(BfmError):
+    """A HIP runtime call failed."""
+
+
+class NoDeviceError(BfmError):
+    """No HIP device is visible."""
+
+
+class StateError(BfmError):
+    """Call out of order (e.g. run before initialize)."""
+
+
+_ERRORS = {
+    BFM_E_INVALID_ARGUMENT: InvalidArgument,
+    BFM_E_RUNTIME: BfmError,
+    BFM_E_HIP: HipError,
+    BFM_E_NO_DEVICE: NoDeviceError,
+    BFM_E_OUT_OF_MEMORY: BfmError,
+    BFM_E_STATE: StateError,
+}
+
+
+class BfmStats(ctypes.Structure):
+    _fields_ = [
+        ("attempts", c_uint64),
+        ("proposals", c_uint64),
+        ("accepted", c_uint64),
+        ("substeps", c_uint64),
+        ("check_ms", c_double),
+        ("perform_ms", c_double),
+        ("timed_launches", c_uint64),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/*.h
+_H = c_void_p
+SIGNATURES = {
+    "bfm_last_error": (c_char_p, []),
+    "bfm_create": (c_int, [POINTER(c_void_p)]),
+    "bfm_destroy": (c_int, [_H]),
+    "bfm_set_gpu": (c_int, [_H, c_int]),
+    "bfm_copy_bond_set": (c_int, [_H, c_int, c_int, c_int, c_int]),
+    "bfm_set_lattice_size": (c_int, [_H, c_uint64, c_uint64, c_uint64]),
+    "bfm_set_nr_of_all_monomers": (c_int, [_H, c_uint32]),
+    "bfm_set_attribute": (c_int, [_H, c_uint32, c_int32]),
+    "bfm_set_monomer_coordinates": (c_int, [_H, c_uint32, c_int32, c_int32, c_int32]),
+    "bfm_set_connectivity": (c_int, [_H, c_uint32, c_uint32]),
+    "bfm_set_periodicity": (c_int, [_H, c_int, c_int, c_int]),
+    "bfm_set_age": (c_int, [_H, c_int64]),
+    "bfm_get_age": (c_int, [_H, POINTER(c_int64)]),
+    "bfm_set_rng": (c_int, [_H, c_int]),
+    "bfm_set_seed": (c_int, [_H, c_uint64]),
+    "bfm_set_steps_between_sortings": (c_int, [_H, c_int64]),
+    "bfm_validate": (c_int, [_H]),
+    "bfm_initialize": (c_int, [_H]),
+    "bfm_run_simulation_on_gpu": (c_int, [_H, c_uint32]),
+    "bfm_get_monomer_position_in_x": (c_int, [_H, c_uint32, POINTER(c_int32)]),
+    "bfm_get_monomer_position_in_y": (c_int, [_H, c_uint32, POINTER(c_int32)]),
+    "bfm_get_monomer_position_in_z": (c_int, [_H, c_uint32, POINTER(c_int32)]),
+    "bfm_cleanup": (c_int, [_H]),
+    "bfm_set_monomer_coordinates_bulk": (c_int, [_H, c_void_p, c_uint32]),
+    "bfm_set_connectivity_bulk": (c_int, [_H, c_void_p, c_uint64]),
+    "bfm_get_monomer_positions": (c_int, [_H, c_void_p, c_uint32]),
+    "bfm_get_colors": (c_int, [_H, c_void_p, c_uint32, POINTER(c_uint32)]),
+    "bfm_get_stats": (c_int, [_H, POINTER(BfmStats)]),
+    "bfm_set_kernel_timing": (c_int, [_H, c_int]),
+    "bfm_run_resident": (c_int, [_H, c_uint32]),
+    "bfm_copy_back": (c_int, [_H]),
+    "bfm_device_count": (c_int, [POINTER(c_int)]),
+    "bfm_gen_linear_melt": (c_int, [c_uint32, c_uint32, c_uint32, c_uint64, c_void_p, c_void_p]),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads the HIP library once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m pscbfm_amd.build` "
+                "(there is no CPU fallback)")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != BFM_OK:
+        msg = load().bfm_last_error()
+        text = msg.decode() if msg else ""
+        raise _ERRORS.get(rc, BfmError)(rc, f"status {rc}: {text}")
+
+
+def device_count() -> int:
+    n = c_int(0)
+    check(load().bfm_device_count(ctypes.byref(n)))
+    return n.value

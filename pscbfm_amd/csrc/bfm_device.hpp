@@ -1,0 +1,153 @@
+// bfm_device.hpp -- data layout and device helpers of the MI355X BFM sweep.
+//
+// Layout of one system in HBM (all arrays are indexed by the *sorted* monomer
+// index: monomers grouped by colour, each colour segment padded to a multiple
+// of kSegmentAlign so that every colour starts on a 1 KiB boundary):
+//   int4     pos[nPad]         x, y, z (unwrapped int32), w = original id
+//   uint32_t nbr[8][nPad]      sorted index of the j-th bond partner (slot-major,
+//                              so lane i reads nbr[j*nPad+i] coalesced)
+//   uint8_t  nnbr[nPad]        number of bond partners
+//   uint32_t lattice[W]        occupancy bit lattice: the 8 sites of every
+//                              monomer cube, compact Morton order
+//   uint32_t claim[W]          sites entered by this substep's proposals
+//   uint32_t coll[2][W]        sites entered by more than one proposal
+//   uint32_t clist[2][cap]     words of coll[] that hold a bit (for clearing)
+//   uint32_t surv[maxColour]   compacted proposals: sorted index | dir << 29
+//
+// The reference keeps 1 lattice byte per monomer (its lower corner) and tests
+// a 9-site front (UpdaterGPUScBFM_AB_Type.cu:420-594); this engine stores the
+// full cube, 1 bit per site, so the same excluded-volume rule needs only the
+// 4 sites of the face the cube enters.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pscbfm {
+
+constexpr int kMaxConnectivity = 8;      // MAX_CONNECTIVITY, UpdaterGPUScBFM_AB_Type.h:85
+constexpr uint32_t kSegmentAlign = 64;   // monomers; 64 x 16 B = 1 KiB
+constexpr int kBlock = 256;              // threads per workgroup (4 wave64)
+constexpr uint32_t kSortedIndexBits = 29;
+constexpr uint32_t kSortedIndexMask = (1u << kSortedIndexBits) - 1u;
+constexpr uint32_t kMaxBox = 1024;       // 10 bits per axis, 30-bit site index
+
+// Geometry of one periodic (or walled) power-of-two box.
+struct Geometry {
+    uint32_t mask[3];    // L_a - 1
+    uint32_t dmask[3];   // index bits that belong to axis a (deposited L_a - 1)
+    int32_t wall_hi[3];  // L_a - 2: largest lower-corner coordinate between walls
+    uint32_t periodic;   // bit a set: axis a is periodic
+};
+
+// Small per-system tables; each workgroup copies them to LDS.
+struct Tables {
+    uint32_t dep[3][64];  // deposit of coordinate bits: [a][v & 31] | [a][32 + (v >> 5)]
+    uint32_t bond[16];    // 512-bit forbidden-bond mask, bit (dx&7)|(dy&7)<<3|(dz&7)<<6
+};
+constexpr int kTableWords = 3 * 64 + 16;
+
+// ---- Philox4x32-10 (Salmon et al., SC'11) -----------------------------------
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = uint64_t(0xD2511F53u) * c[0];
+        const uint64_t p1 = uint64_t(0xCD9E8D57u) * c[2];
+        const uint32_t n0 = uint32_t(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = uint32_t(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = uint32_t(p1);
+        c[3] = uint32_t(p0);
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// Word 0 of Philox(key = seed, ctr = (c0, age_lo, age_hi, tag)).
+__host__ __device__ inline uint32_t philox_word0(uint64_t seed, uint32_t c0, uint64_t age, uint32_t tag)
+{
+    uint32_t c[4] = {c0, uint32_t(age), uint32_t(age >> 32), tag};
+    philox4x32_10(c, uint32_t(seed), uint32_t(seed >> 32));
+    return c[0];
+}
+
+constexpr uint32_t kTagMove = 0;
+constexpr uint32_t kTagColour = 1;
+
+// Colour order of one MCS: Fisher-Yates, slot i swaps with word % (i+1).
+inline void colour_order(uint64_t seed, uint64_t age, uint32_t n, uint32_t *order)
+{
+    for (uint32_t i = 0; i < n; ++i)
+        order[i] = i;
+    for (uint32_t i = n; i-- > 1;) {
+        const uint32_t j = philox_word0(seed, i, age, kTagColour) % (i + 1u);
+        const uint32_t t = order[i];
+        order[i] = order[j];
+        order[j] = t;
+    }
+}
+
+// ---- dilated (Morton) integer arithmetic ------------------------------------
+__device__ inline uint32_t deposit(const uint32_t *dep, int a, uint32_t v)
+{
+    return dep[a * 64 + (v & 31u)] | dep[a * 64 + 32 + (v >> 5)];
+}
+__host__ __device__ inline uint32_t dilated_inc(uint32_t d, uint32_t m) { return ((d | ~m) + 1u) & m; }
+__host__ __device__ inline uint32_t dilated_dec(uint32_t d, uint32_t m) { return (d - 1u) & m; }
+
+__host__ __device__ inline int32_t comp(const int4 &r, int a) { return a == 0 ? r.x : (a == 1 ? r.y : r.z); }
+// v[a] for a per-lane axis without indexing memory (kernel arguments live in
+// SGPRs; a dynamic index would turn them into vector loads).
+__host__ __device__ inline uint32_t sel3(const uint32_t v[3], int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); }
+
+// Site indices of the 2x2 face a cube with lower corner r enters when it
+// moves in direction dir (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z, as the reference's
+// DXTable_d/DYTable_d/DZTable_d, UpdaterGPUScBFM_AB_Type.cu:1537-1541).
+// Also returns the dilated coordinate of the face plane along the move axis.
+__device__ inline void face_sites(const Geometry &g, const uint32_t *dep, const int4 &r, int dir,
+                                  uint32_t s[4], uint32_t &d_face)
+{
+    const int a = dir >> 1;
+    const int b = a == 0 ? 1 : 0;
+    const int c = a == 2 ? 1 : 2;
+    const int32_t fa = (dir & 1) ? comp(r, a) + 2 : comp(r, a) - 1;
+    d_face = deposit(dep, a, uint32_t(fa) & sel3(g.mask, a));
+    const uint32_t b0 = deposit(dep, b, uint32_t(comp(r, b)) & sel3(g.mask, b));
+    const uint32_t c0 = deposit(dep, c, uint32_t(comp(r, c)) & sel3(g.mask, c));
+    const uint32_t b1 = dilated_inc(b0, sel3(g.dmask, b));
+    const uint32_t c1 = dilated_inc(c0, sel3(g.dmask, c));
+    s[0] = d_face | b0 | c0;
+    s[1] = d_face | b1 | c0;
+    s[2] = d_face | b0 | c1;
+    s[3] = d_face | b1 | c1;
+}
+
+// Merges bit updates that fall into the same 32-bit word: on return every
+// word appears once, in an entry with a non-zero mask; other masks are zero.
+template <int N>
+__device__ inline void merge_words(uint32_t (&w)[N], uint32_t (&m)[N])
+{
+#pragma unroll
+    for (int k = 1; k < N; ++k)
+#pragma unroll
+        for (int j = 0; j < k; ++j)
+            if (m[j] != 0u && m[k] != 0u && w[j] == w[k]) {
+                m[j] |= m[k];
+                m[k] = 0u;
+            }
+}
+
+__device__ inline bool bit_test(const uint32_t *words, uint32_t site)
+{
+    return (words[site >> 5] >> (site & 31u)) & 1u;
+}
+
+__device__ inline bool bond_forbidden(const uint32_t *bond, int dx, int dy, int dz)
+{
+    const uint32_t idx = uint32_t(dx & 7) | (uint32_t(dy & 7) << 3) | (uint32_t(dz & 7) << 6);
+    return (bond[idx >> 5] >> (idx & 31u)) & 1u;
+}
+
+}  // namespace pscbfm

@@ -1,0 +1,280 @@
+// host_system.cpp -- validation, colouring, Morton tables and layout.
+#include "host_system.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <sstream>
+
+namespace pscbfm {
+
+HostSystem::HostSystem() { reset(); }
+
+void HostSystem::reset()
+{
+    box[0] = box[1] = box[2] = 0;
+    periodic[0] = periodic[1] = periodic[2] = true;
+    n = 0;
+    xyz.clear();
+    attribute.clear();
+    nnbr.clear();
+    nbr.clear();
+    // Until copyBondSet says otherwise every vector is forbidden; the
+    // LeMonADE adapter sends all 512 entries (GPUScBFM_AB_Type.h:168-175).
+    std::memset(forbidden, 1, sizeof forbidden);
+    geom = Geometry{};
+    tables = Tables{};
+    lattice_words = 0;
+    colour.clear();
+    n_colours = 0;
+    seg_begin.clear();
+    seg_count.clear();
+    n_pad = 0;
+    max_segment = 0;
+    to_sorted.clear();
+    pos_sorted.clear();
+    nbr_sorted.clear();
+    nnbr_sorted.clear();
+    lattice.clear();
+    mean_bonds = 0.0;
+}
+
+int count_allowed_bonds(const uint8_t forbidden[512])
+{
+    int k = 0;
+    for (int i = 0; i < 512; ++i)
+        k += forbidden[i] == 0;
+    return k;
+}
+
+void standard_bond_set(uint8_t forbidden[512])
+{
+    std::memset(forbidden, 1, 512);
+    for (int dx = -3; dx <= 3; ++dx)
+        for (int dy = -3; dy <= 3; ++dy)
+            for (int dz = -3; dz <= 3; ++dz) {
+                int a[3] = {std::abs(dx), std::abs(dy), std::abs(dz)};
+                std::sort(a, a + 3);
+                const int key = a[0] * 100 + a[1] * 10 + a[2];
+                // sorted magnitudes of the six generating vectors
+                if (key == 2 || key == 12 || key == 112 || key == 122 || key == 3 || key == 13)
+                    forbidden[bond_table_index(dx, dy, dz)] = 0;
+            }
+}
+
+static int log2_exact(uint64_t v)
+{
+    int k = 0;
+    while ((uint64_t(1) << k) < v)
+        ++k;
+    return (uint64_t(1) << k) == v ? k : -1;
+}
+
+void make_geometry(const uint64_t box[3], const bool periodic[3], Geometry &g, Tables &t,
+                   uint64_t &lattice_words)
+{
+    int k[3];
+    for (int a = 0; a < 3; ++a) {
+        k[a] = log2_exact(box[a]);
+        if (k[a] < 3 || box[a] > kMaxBox) {
+            std::ostringstream msg;
+            msg << "box edge " << box[a] << " must be a power of two in [8," << kMaxBox << "]";
+            throw std::runtime_error(msg.str());
+        }
+    }
+    int bitpos[3][10];
+    int next = 0;
+    for (int lvl = 0; lvl < 10; ++lvl)
+        for (int a = 0; a < 3; ++a)
+            if (lvl < k[a])
+                bitpos[a][lvl] = next++;
+    auto dep = [&](int a, uint32_t v) {
+        uint32_t d = 0;
+        for (int lvl = 0; lvl < k[a]; ++lvl)
+            if ((v >> lvl) & 1u)
+                d |= 1u << bitpos[a][lvl];
+        return d;
+    };
+    g.periodic = 0;
+    for (int a = 0; a < 3; ++a) {
+        g.mask[a] = uint32_t(box[a] - 1);
+        g.dmask[a] = dep(a, g.mask[a]);
+        g.wall_hi[a] = int32_t(box[a]) - 2;
+        if (periodic[a])
+            g.periodic |= 1u << a;
+        for (uint32_t j = 0; j < 32; ++j) {
+            t.dep[a][j] = dep(a, j & g.mask[a]);
+            t.dep[a][32 + j] = dep(a, (j << 5) & g.mask[a]);
+        }
+    }
+    const uint64_t sites = uint64_t(1) << next;
+    lattice_words = std::max<uint64_t>(1, sites / 32);
+}
+
+uint32_t host_site(const Geometry &g, const Tables &t, int32_t x, int32_t y, int32_t z)
+{
+    const int32_t v[3] = {x, y, z};
+    uint32_t s = 0;
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t u = uint32_t(v[a]) & g.mask[a];
+        s |= t.dep[a][u & 31u] | t.dep[a][32 + (u >> 5)];
+    }
+    return s;
+}
+
+uint32_t greedy_colouring(uint32_t n, const std::vector<uint8_t> &nnbr, const std::vector<uint32_t> &nbr,
+                          std::vector<uint32_t> &colour)
+{
+    // symmetrised adjacency in CSR form
+    std::vector<uint32_t> deg(n + 1, 0);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = 0; j < nnbr[i]; ++j) {
+            const uint32_t p = nbr[size_t(i) * kMaxConnectivity + j];
+            ++deg[i + 1];
+            ++deg[p + 1];
+        }
+    for (uint32_t i = 0; i < n; ++i)
+        deg[i + 1] += deg[i];
+    std::vector<uint32_t> adj(deg[n]);
+    std::vector<uint32_t> fill(deg.begin(), deg.end() - 1);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = 0; j < nnbr[i]; ++j) {
+            const uint32_t p = nbr[size_t(i) * kMaxConnectivity + j];
+            adj[fill[i]++] = p;
+            adj[fill[p]++] = i;
+        }
+    const uint32_t none = 0xFFFFFFFFu;
+    colour.assign(n, none);
+    uint32_t n_colours = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint64_t used[4] = {0, 0, 0, 0};
+        for (uint32_t k = deg[i]; k < deg[i + 1]; ++k) {
+            const uint32_t c = colour[adj[k]];
+            if (c != none)
+                used[c >> 6] |= uint64_t(1) << (c & 63);
+        }
+        uint32_t c = 0;
+        while (c < 256 && ((used[c >> 6] >> (c & 63)) & 1u))
+            ++c;
+        if (c >= 255)
+            throw std::runtime_error("graph colouring needs more than 255 colours");
+        colour[i] = c;
+        n_colours = std::max(n_colours, c + 1);
+    }
+    return n_colours;
+}
+
+void plan(HostSystem &s)
+{
+    if (s.n == 0)
+        throw std::runtime_error("setNrOfAllMonomers must be called with n > 0 before initialize");
+    if (s.box[0] == 0 || s.box[1] == 0 || s.box[2] == 0)
+        throw std::runtime_error("setLatticeSize must be called before initialize");
+    const int allowed = count_allowed_bonds(s.forbidden);
+    if (allowed != 108) {
+        std::ostringstream msg;
+        msg << "Wrong bond-set! Expected 108 allowed bonds, but got " << allowed;
+        throw std::runtime_error(msg.str());
+    }
+    make_geometry(s.box, s.periodic, s.geom, s.tables, s.lattice_words);
+    if (s.n > kSortedIndexMask / 2)
+        throw std::runtime_error("too many monomers for the 29-bit sorted index");
+    for (int i = 0; i < 512; ++i)
+        if (s.forbidden[i])
+            s.tables.bond[i >> 5] |= 1u << (i & 31);
+
+    // walls
+    for (uint32_t i = 0; i < s.n; ++i)
+        for (int a = 0; a < 3; ++a)
+            if (!s.periodic[a]) {
+                const int32_t v = s.xyz[3 * size_t(i) + a];
+                if (v < 0 || v > int32_t(s.box[a]) - 2) {
+                    std::ostringstream msg;
+                    msg << "monomer " << i << " lies outside the non-periodic walls on axis " << a;
+                    throw std::runtime_error(msg.str());
+                }
+            }
+
+    // occupancy lattice; overlap check (checkSystem, UpdaterGPUScBFM_AB_Type.cu:2808-2851)
+    s.lattice.assign(s.lattice_words, 0u);
+    for (uint32_t i = 0; i < s.n; ++i) {
+        const int32_t *r = &s.xyz[3 * size_t(i)];
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    const uint32_t site = host_site(s.geom, s.tables, r[0] + dx, r[1] + dy, r[2] + dz);
+                    uint32_t &w = s.lattice[site >> 5];
+                    const uint32_t bit = 1u << (site & 31u);
+                    if (w & bit) {
+                        std::ostringstream msg;
+                        msg << "Occupation count in lattice is wrong: monomer " << i << " at (" << r[0] << ","
+                            << r[1] << "," << r[2] << ") overlaps another monomer";
+                        throw std::runtime_error(msg.str());
+                    }
+                    w |= bit;
+                }
+    }
+
+    // bonds (UpdaterGPUScBFM_AB_Type.cu:2853-2899)
+    uint64_t n_links = 0;
+    for (uint32_t i = 0; i < s.n; ++i)
+        for (uint32_t j = 0; j < s.nnbr[i]; ++j) {
+            const uint32_t p = s.nbr[size_t(i) * kMaxConnectivity + j];
+            if (p >= s.n) {
+                std::ostringstream msg;
+                msg << "monomer " << i << " is linked to monomer " << p << " which does not exist";
+                throw std::invalid_argument(msg.str());
+            }
+            const int32_t d[3] = {s.xyz[3 * size_t(p)] - s.xyz[3 * size_t(i)],
+                                  s.xyz[3 * size_t(p) + 1] - s.xyz[3 * size_t(i) + 1],
+                                  s.xyz[3 * size_t(p) + 2] - s.xyz[3 * size_t(i) + 2]};
+            const bool too_long = std::abs(d[0]) > 3 || std::abs(d[1]) > 3 || std::abs(d[2]) > 3;
+            if (too_long || s.forbidden[bond_table_index(d[0], d[1], d[2])]) {
+                std::ostringstream msg;
+                msg << "Invalid bond (" << d[0] << "," << d[1] << "," << d[2] << ") between monomer " << i
+                    << " and monomer " << p;
+                throw std::runtime_error(msg.str());
+            }
+            ++n_links;
+        }
+    s.mean_bonds = double(n_links) / double(s.n);
+
+    // colouring and colour-sorted layout (initializeSpeciesSorting, :1561-1686)
+    s.n_colours = greedy_colouring(s.n, s.nnbr, s.nbr, s.colour);
+    s.seg_count.assign(s.n_colours, 0);
+    for (uint32_t i = 0; i < s.n; ++i)
+        ++s.seg_count[s.colour[i]];
+    s.seg_begin.assign(s.n_colours, 0);
+    uint64_t offset = 0;
+    s.max_segment = 0;
+    for (uint32_t c = 0; c < s.n_colours; ++c) {
+        s.seg_begin[c] = uint32_t(offset);
+        offset += (uint64_t(s.seg_count[c]) + kSegmentAlign - 1) / kSegmentAlign * kSegmentAlign;
+        s.max_segment = std::max(s.max_segment, s.seg_count[c]);
+    }
+    if (offset > kSortedIndexMask)
+        throw std::runtime_error("padded system exceeds the 29-bit sorted index");
+    s.n_pad = uint32_t(std::max<uint64_t>(offset, kSegmentAlign));
+    s.to_sorted.assign(s.n, 0);
+    {
+        std::vector<uint32_t> next(s.seg_begin);
+        for (uint32_t i = 0; i < s.n; ++i)
+            s.to_sorted[i] = next[s.colour[i]]++;
+    }
+    s.pos_sorted.assign(4 * size_t(s.n_pad), 0);
+    s.nbr_sorted.assign(size_t(kMaxConnectivity) * s.n_pad, 0);
+    s.nnbr_sorted.assign(s.n_pad, 0);
+    for (size_t k = 0; k < s.n_pad; ++k)
+        s.pos_sorted[4 * k + 3] = -1;
+    for (uint32_t i = 0; i < s.n; ++i) {
+        const uint32_t t = s.to_sorted[i];
+        s.pos_sorted[4 * size_t(t) + 0] = s.xyz[3 * size_t(i) + 0];
+        s.pos_sorted[4 * size_t(t) + 1] = s.xyz[3 * size_t(i) + 1];
+        s.pos_sorted[4 * size_t(t) + 2] = s.xyz[3 * size_t(i) + 2];
+        s.pos_sorted[4 * size_t(t) + 3] = int32_t(i);
+        s.nnbr_sorted[t] = s.nnbr[i];
+        for (uint32_t j = 0; j < s.nnbr[i]; ++j)
+            s.nbr_sorted[size_t(j) * s.n_pad + t] = s.to_sorted[s.nbr[size_t(i) * kMaxConnectivity + j]];
+    }
+}
+
+}  // namespace pscbfm

@@ -1,0 +1,74 @@
+// host_system.hpp -- host half of UpdaterGPUScBFM_AB_Type::initialize:
+// validation, graph colouring, compact Morton tables and the colour-sorted
+// device layout.  Pure C++ (no HIP calls), so every rule here is testable on a
+// machine without a GPU.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bfm_device.hpp"
+
+namespace pscbfm {
+
+struct HostSystem {
+    // ---- inputs (setters) ---------------------------------------------------
+    uint64_t box[3] = {0, 0, 0};
+    bool periodic[3] = {true, true, true};
+    uint32_t n = 0;
+    std::vector<int32_t> xyz;        // 3*n unwrapped coordinates
+    std::vector<int32_t> attribute;  // n (stored only)
+    std::vector<uint8_t> nnbr;       // n
+    std::vector<uint32_t> nbr;       // n * kMaxConnectivity, directed links as given
+    uint8_t forbidden[512];          // 1 = forbidden (copyBondSet)
+
+    // ---- derived by plan() --------------------------------------------------
+    Geometry geom{};
+    Tables tables{};
+    uint64_t lattice_words = 0;
+    std::vector<uint32_t> colour;    // n
+    uint32_t n_colours = 0;
+    std::vector<uint32_t> seg_begin; // per colour, sorted index of first monomer
+    std::vector<uint32_t> seg_count; // per colour
+    uint32_t n_pad = 0;              // padded sorted length
+    uint32_t max_segment = 0;
+    std::vector<uint32_t> to_sorted; // original id -> sorted index
+    std::vector<int32_t> pos_sorted; // 4 * n_pad: x, y, z, original id
+    std::vector<uint32_t> nbr_sorted;  // kMaxConnectivity * n_pad, slot-major
+    std::vector<uint8_t> nnbr_sorted;  // n_pad
+    std::vector<uint32_t> lattice;     // lattice_words occupancy bits
+    double mean_bonds = 0.0;           // mean number of listed partners per monomer
+
+    HostSystem();
+    void reset();
+};
+
+// Number of allowed entries of a 512-entry bond table.
+int count_allowed_bonds(const uint8_t forbidden[512]);
+// Standard 108-vector bond set (permutations and sign flips of (2,0,0),
+// (2,1,0), (2,1,1), (2,2,1), (3,0,0), (3,1,0)).
+void standard_bond_set(uint8_t forbidden[512]);
+inline int bond_table_index(int dx, int dy, int dz) { return (dx & 7) | ((dy & 7) << 3) | ((dz & 7) << 6); }
+
+// Compact Morton geometry of a power-of-two box (edges 8..1024): bits of x,
+// y, z interleave from bit 0 upward in the order x, y, z, skipping an axis
+// whose bits are used up.  A cubic box gives the classic Z-curve with x in
+// bit 0 (UpdaterGPUScBFM_AB_Type.cu:289-316).
+void make_geometry(const uint64_t box[3], const bool periodic[3], Geometry &g, Tables &t,
+                   uint64_t &lattice_words);
+uint32_t host_site(const Geometry &g, const Tables &t, int32_t x, int32_t y, int32_t z);
+
+// Greedy sequential colouring over the symmetrised bond graph: monomers in
+// increasing id take the smallest colour none of their already coloured
+// partners has.  Throws std::runtime_error above 255 colours.
+uint32_t greedy_colouring(uint32_t n, const std::vector<uint8_t> &nnbr, const std::vector<uint32_t> &nbr,
+                          std::vector<uint32_t> &colour);
+
+// Runs every check of initialize() and fills the derived fields.
+// Throws std::invalid_argument / std::runtime_error like the reference.
+void plan(HostSystem &s);
+
+}  // namespace pscbfm

@@ -1,0 +1,412 @@
+// updater.cpp -- host orchestration of the colour sweep on one device.
+#include "updater.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <sstream>
+
+namespace pscbfm {
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        std::ostringstream msg;
+        msg << what << ": " << hipGetErrorName(e) << " (" << hipGetErrorString(e) << ")";
+        if (e == hipErrorNoDevice || e == hipErrorInvalidDevice)
+            throw NoDeviceError(msg.str());
+        throw HipError(e, msg.str());
+    }
+}
+
+UpdaterGPUScBFM_AB_Type::UpdaterGPUScBFM_AB_Type() = default;
+
+UpdaterGPUScBFM_AB_Type::~UpdaterGPUScBFM_AB_Type()
+{
+    try {
+        freeDevice();
+    } catch (...) {
+    }
+}
+
+void UpdaterGPUScBFM_AB_Type::setGpu(int gpu)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e == hipErrorNoDevice || (e == hipSuccess && n == 0))
+        throw NoDeviceError("no HIP device is visible");
+    hip_check(e, "hipGetDeviceCount");
+    if (gpu < 0 || gpu >= n) {
+        std::ostringstream msg;
+        msg << "GPU with ID " << gpu << " not present. Only " << n << " GPUs are available.";
+        throw std::invalid_argument(msg.str());
+    }
+    hip_check(hipSetDevice(gpu), "hipSetDevice");
+    mGpu = gpu;
+}
+
+void UpdaterGPUScBFM_AB_Type::copyBondSet(int dx, int dy, int dz, bool forbidden)
+{
+    if (dx < -4 || dx > 3 || dy < -4 || dy > 3 || dz < -4 || dz > 3)
+        throw std::invalid_argument("copyBondSet: components must lie in [-4,3]");
+    mSys.forbidden[bond_table_index(dx, dy, dz)] = forbidden ? 1 : 0;
+    mValidated = false;
+}
+
+void UpdaterGPUScBFM_AB_Type::setLatticeSize(T_BoxSize x, T_BoxSize y, T_BoxSize z)
+{
+    const T_BoxSize v[3] = {x, y, z};
+    for (int a = 0; a < 3; ++a) {
+        if (v[a] == 0 || v[a] > T_BoxSize(INT32_MAX)) {
+            std::ostringstream msg;
+            msg << "The box size (" << x << "," << y << "," << z << ") is outside the int32 coordinate range";
+            throw std::invalid_argument(msg.str());
+        }
+        if ((v[a] & (v[a] - 1)) != 0) {
+            std::ostringstream msg;
+            msg << "Check whether the box size is a power of 2! (" << x << "," << y << "," << z << ")";
+            throw std::runtime_error(msg.str());
+        }
+    }
+    if (mInitialized)
+        throw StateError("setLatticeSize after initialize; call cleanup first");
+    mSys.box[0] = x;
+    mSys.box[1] = y;
+    mSys.box[2] = z;
+    mValidated = false;
+}
+
+void UpdaterGPUScBFM_AB_Type::setNrOfAllMonomers(T_Id n)
+{
+    if (mMonomersSet) {
+        std::ostringstream msg;
+        msg << "Number of Monomers already set to " << mSys.n << "!";
+        throw std::runtime_error(msg.str());
+    }
+    mSys.n = n;
+    mSys.xyz.assign(3 * size_t(n), 0);
+    mSys.attribute.assign(n, 0);
+    mSys.nnbr.assign(n, 0);
+    mSys.nbr.assign(size_t(n) * kMaxConnectivity, 0);
+    mMonomersSet = true;
+    mValidated = false;
+}
+
+static void check_index(uint32_t i, uint32_t n, const char *where)
+{
+    if (i >= n) {
+        std::ostringstream msg;
+        msg << where << ": monomer index " << i << " out of range (" << n << " monomers)";
+        throw std::invalid_argument(msg.str());
+    }
+}
+
+void UpdaterGPUScBFM_AB_Type::setAttribute(T_Id i, int32_t attribute)
+{
+    check_index(i, mSys.n, "setAttribute");
+    mSys.attribute[i] = attribute;
+}
+
+void UpdaterGPUScBFM_AB_Type::setMonomerCoordinates(T_Id i, T_Coordinate x, T_Coordinate y, T_Coordinate z)
+{
+    check_index(i, mSys.n, "setMonomerCoordinates");
+    if (mInitialized)
+        throw StateError("setMonomerCoordinates after initialize; call cleanup first");
+    mSys.xyz[3 * size_t(i) + 0] = x;
+    mSys.xyz[3 * size_t(i) + 1] = y;
+    mSys.xyz[3 * size_t(i) + 2] = z;
+    mValidated = false;
+}
+
+void UpdaterGPUScBFM_AB_Type::setConnectivity(T_Id i, T_Id j)
+{
+    check_index(i, mSys.n, "setConnectivity");
+    check_index(j, mSys.n, "setConnectivity");
+    if (mInitialized)
+        throw StateError("setConnectivity after initialize; call cleanup first");
+    if (mSys.nnbr[i] >= kMaxConnectivity) {
+        std::ostringstream msg;
+        msg << "The maximum amount of bonds per monomer (" << kMaxConnectivity << ") has been exceeded!";
+        throw std::invalid_argument(msg.str());
+    }
+    mSys.nbr[size_t(i) * kMaxConnectivity + mSys.nnbr[i]++] = j;
+    mValidated = false;
+}
+
+void UpdaterGPUScBFM_AB_Type::setPeriodicity(bool px, bool py, bool pz)
+{
+    if (mInitialized)
+        throw StateError("setPeriodicity after initialize; call cleanup first");
+    mSys.periodic[0] = px;
+    mSys.periodic[1] = py;
+    mSys.periodic[2] = pz;
+    mValidated = false;
+}
+
+void UpdaterGPUScBFM_AB_Type::setRng(int rng)
+{
+    if (rng != 1) {
+        std::ostringstream msg;
+        msg << "random number generator " << rng << " is not available; only 1 (Philox4x32-10) is";
+        throw std::invalid_argument(msg.str());
+    }
+}
+
+void UpdaterGPUScBFM_AB_Type::setStepsBetweenSortings(int64_t steps)
+{
+    if (steps <= 0)
+        throw std::invalid_argument("setStepsBetweenSortings: steps must be positive");
+}
+
+void UpdaterGPUScBFM_AB_Type::validate()
+{
+    if (mInitialized)
+        throw std::runtime_error("Initialize was already called and may not be called again until cleanup was called!");
+    plan(mSys);
+    mValidated = true;
+}
+
+void UpdaterGPUScBFM_AB_Type::initialize()
+{
+    validate();
+    if (mGpu < 0)
+        setGpu(0);
+    else
+        hip_check(hipSetDevice(mGpu), "hipSetDevice");
+
+    const HostSystem &s = mSys;
+    const size_t blocks = check_blocks(std::max<uint32_t>(s.max_segment, 1));
+    // one slab, every array 256-byte aligned
+    struct Part {
+        size_t bytes;
+        size_t offset;
+    };
+    auto round = [](size_t b) { return (b + 255) / 256 * 256; };
+    std::vector<Part> parts = {
+        {sizeof(int4) * s.n_pad, 0},                              // 0 pos
+        {sizeof(uint32_t) * kMaxConnectivity * s.n_pad, 0},       // 1 nbr
+        {size_t(s.n_pad), 0},                                     // 2 nnbr
+        {sizeof(uint32_t) * s.lattice_words, 0},                  // 3 lattice
+        {sizeof(uint32_t) * s.lattice_words, 0},                  // 4 claim
+        {sizeof(uint32_t) * s.lattice_words, 0},                  // 5 coll0
+        {sizeof(uint32_t) * s.lattice_words, 0},                  // 6 coll1
+        {sizeof(uint32_t) * kCollSlots * blocks, 0},              // 7 clist0
+        {sizeof(uint32_t) * kCollSlots * blocks, 0},              // 8 clist1
+        {sizeof(uint32_t) * blocks, 0},                           // 9 ccount0
+        {sizeof(uint32_t) * blocks, 0},                           // 10 ccount1
+        {sizeof(uint32_t) * kBlock * blocks, 0},                  // 11 surv
+        {sizeof(uint32_t) * blocks, 0},                           // 12 scount
+        {sizeof(unsigned long long) * kStatSlots, 0},             // 13 accepted
+        {sizeof(unsigned long long) * kStatSlots, 0},             // 14 proposals
+        {sizeof(Tables), 0},                                      // 15 tables
+    };
+    size_t total = 0;
+    for (auto &p : parts) {
+        p.offset = total;
+        total += round(p.bytes);
+    }
+    hipError_t e = hipMalloc(&mDevSlab, total);
+    if (e != hipSuccess) {
+        mDevSlab = nullptr;
+        hip_check(e, "hipMalloc (system slab)");
+    }
+    mDevBytes = total;
+    auto ptr = [&](int k) { return static_cast<char *>(mDevSlab) + parts[k].offset; };
+    try {
+        hip_check(hipStreamCreate(&mStream), "hipStreamCreate");
+        hip_check(hipMemsetAsync(mDevSlab, 0, total, mStream), "hipMemsetAsync");
+        hip_check(hipMemcpyAsync(ptr(0), s.pos_sorted.data(), parts[0].bytes, hipMemcpyHostToDevice, mStream),
+                  "upload pos");
+        hip_check(hipMemcpyAsync(ptr(1), s.nbr_sorted.data(), parts[1].bytes, hipMemcpyHostToDevice, mStream),
+                  "upload nbr");
+        hip_check(hipMemcpyAsync(ptr(2), s.nnbr_sorted.data(), parts[2].bytes, hipMemcpyHostToDevice, mStream),
+                  "upload nnbr");
+        hip_check(hipMemcpyAsync(ptr(3), s.lattice.data(), parts[3].bytes, hipMemcpyHostToDevice, mStream),
+                  "upload lattice");
+        hip_check(hipMemcpyAsync(ptr(15), &s.tables, sizeof(Tables), hipMemcpyHostToDevice, mStream),
+                  "upload tables");
+        hip_check(hipStreamSynchronize(mStream), "hipStreamSynchronize (initialize)");
+    } catch (...) {
+        freeDevice();
+        throw;
+    }
+    mBuf.pos = reinterpret_cast<int4 *>(ptr(0));
+    mBuf.nbr = reinterpret_cast<const uint32_t *>(ptr(1));
+    mBuf.nnbr = reinterpret_cast<const uint8_t *>(ptr(2));
+    mBuf.n_pad = s.n_pad;
+    mBuf.lattice = reinterpret_cast<uint32_t *>(ptr(3));
+    mBuf.claim = reinterpret_cast<uint32_t *>(ptr(4));
+    mBuf.coll[0] = reinterpret_cast<uint32_t *>(ptr(5));
+    mBuf.coll[1] = reinterpret_cast<uint32_t *>(ptr(6));
+    mBuf.clist[0] = reinterpret_cast<uint32_t *>(ptr(7));
+    mBuf.clist[1] = reinterpret_cast<uint32_t *>(ptr(8));
+    mBuf.ccount[0] = reinterpret_cast<uint32_t *>(ptr(9));
+    mBuf.ccount[1] = reinterpret_cast<uint32_t *>(ptr(10));
+    mBuf.surv = reinterpret_cast<uint32_t *>(ptr(11));
+    mBuf.scount = reinterpret_cast<uint32_t *>(ptr(12));
+    mBuf.accepted = reinterpret_cast<unsigned long long *>(ptr(13));
+    mBuf.proposals = reinterpret_cast<unsigned long long *>(ptr(14));
+    mBuf.tables = reinterpret_cast<const Tables *>(ptr(15));
+    mBuf.geom = s.geom;
+    mBuf.seed = mSeed;
+    mSubsteps = 0;
+    mPrevBlocks = 0;
+    mAttempts = 0;
+    mCheckMs = mPerformMs = 0.0;
+    mTimedLaunches = 0;
+    mInitialized = true;
+}
+
+void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs)
+{
+    if (!mInitialized)
+        throw StateError("runSimulationOnGPU before initialize");
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    mBuf.seed = mSeed;
+    const HostSystem &s = mSys;
+    std::vector<uint32_t> order(s.n_colours);
+    for (uint32_t m = 0; m < n_mcs; ++m) {
+        const uint64_t age = uint64_t(mAge);
+        colour_order(mSeed, age, s.n_colours, order.data());
+        for (uint32_t k = 0; k < s.n_colours; ++k) {
+            const uint32_t c = order[k];
+            const uint32_t parity = uint32_t(mSubsteps & 1u);
+            hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+            if (mTiming) {
+                for (auto &x : ev)
+                    x = takeEvent();
+                hip_check(hipEventRecord(ev[0], mStream), "hipEventRecord");
+            }
+            hip_check(launch_check(mBuf, s.seg_begin[c], s.seg_count[c], age, parity, mPrevBlocks, mStream),
+                      "check kernel");
+            mPrevBlocks = check_blocks(s.seg_count[c]);
+            if (mTiming)
+                hip_check(hipEventRecord(ev[1], mStream), "hipEventRecord");
+            hip_check(launch_perform(mBuf, s.seg_count[c], parity, mStream), "perform kernel");
+            if (mTiming) {
+                hip_check(hipEventRecord(ev[2], mStream), "hipEventRecord");
+                mEvents.insert(mEvents.end(), ev, ev + 3);
+            }
+            ++mSubsteps;
+        }
+        mAttempts += s.n;
+        ++mAge;
+        if (mEvents.size() > 3 * 4096)
+            collectTimings();
+    }
+}
+
+hipEvent_t UpdaterGPUScBFM_AB_Type::takeEvent()
+{
+    if (mEventPool.empty()) {
+        hipEvent_t e = nullptr;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        return e;
+    }
+    hipEvent_t e = mEventPool.back();
+    mEventPool.pop_back();
+    return e;
+}
+
+void UpdaterGPUScBFM_AB_Type::collectTimings() const
+{
+    if (mEvents.empty())
+        return;
+    hip_check(hipEventSynchronize(mEvents.back()), "hipEventSynchronize");
+    for (size_t k = 0; k + 2 < mEvents.size(); k += 3) {
+        float a = 0.f, b = 0.f;
+        hip_check(hipEventElapsedTime(&a, mEvents[k], mEvents[k + 1]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&b, mEvents[k + 1], mEvents[k + 2]), "hipEventElapsedTime");
+        mCheckMs += a;
+        mPerformMs += b;
+        ++mTimedLaunches;
+    }
+    auto &pool = const_cast<std::vector<hipEvent_t> &>(mEventPool);
+    pool.insert(pool.end(), mEvents.begin(), mEvents.end());
+    mEvents.clear();
+}
+
+void UpdaterGPUScBFM_AB_Type::copyBack()
+{
+    if (!mInitialized)
+        throw StateError("copy-back before initialize");
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    HostSystem &s = mSys;
+    hip_check(hipMemcpyAsync(s.pos_sorted.data(), mBuf.pos, sizeof(int4) * s.n_pad, hipMemcpyDeviceToHost, mStream),
+              "download pos");
+    hip_check(hipStreamSynchronize(mStream), "hipStreamSynchronize (copy-back)");
+    collectTimings();
+    for (uint32_t i = 0; i < s.n; ++i) {
+        const uint32_t t = s.to_sorted[i];
+        s.xyz[3 * size_t(i) + 0] = s.pos_sorted[4 * size_t(t) + 0];
+        s.xyz[3 * size_t(i) + 1] = s.pos_sorted[4 * size_t(t) + 1];
+        s.xyz[3 * size_t(i) + 2] = s.pos_sorted[4 * size_t(t) + 2];
+    }
+}
+
+void UpdaterGPUScBFM_AB_Type::runSimulationOnGPU(uint32_t n_mcs)
+{
+    runResident(n_mcs);
+    copyBack();
+}
+
+void UpdaterGPUScBFM_AB_Type::deviceCounters(uint64_t &accepted, uint64_t &proposals) const
+{
+    accepted = proposals = 0;
+    if (!mInitialized)
+        return;
+    unsigned long long slots[2 * kStatSlots];
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    hip_check(hipMemcpyAsync(slots, mBuf.accepted, sizeof(unsigned long long) * kStatSlots, hipMemcpyDeviceToHost,
+                             mStream),
+              "download counters");
+    hip_check(hipMemcpyAsync(slots + kStatSlots, mBuf.proposals, sizeof(unsigned long long) * kStatSlots,
+                             hipMemcpyDeviceToHost, mStream),
+              "download counters");
+    hip_check(hipStreamSynchronize(mStream), "hipStreamSynchronize (counters)");
+    collectTimings();
+    for (uint32_t k = 0; k < kStatSlots; ++k) {
+        accepted += slots[k];
+        proposals += slots[kStatSlots + k];
+    }
+}
+
+int32_t UpdaterGPUScBFM_AB_Type::at(T_Id i, int a) const
+{
+    check_index(i, mSys.n, "getMonomerPosition");
+    return mSys.xyz[3 * size_t(i) + a];
+}
+
+void UpdaterGPUScBFM_AB_Type::freeDevice()
+{
+    if (mGpu >= 0 && (mStream || mDevSlab))
+        hipSetDevice(mGpu);
+    for (auto e : mEvents)
+        hipEventDestroy(e);
+    mEvents.clear();
+    for (auto e : mEventPool)
+        hipEventDestroy(e);
+    mEventPool.clear();
+    if (mStream) {
+        hipStreamSynchronize(mStream);
+        hipStreamDestroy(mStream);
+        mStream = nullptr;
+    }
+    if (mDevSlab) {
+        hipFree(mDevSlab);
+        mDevSlab = nullptr;
+    }
+    mBuf = SweepBuffers{};
+    mInitialized = false;
+}
+
+void UpdaterGPUScBFM_AB_Type::cleanup()
+{
+    freeDevice();
+    mSys.reset();
+    mMonomersSet = false;
+    mValidated = false;
+    mSubsteps = 0;
+    mAttempts = 0;
+}
+
+}  // namespace pscbfm

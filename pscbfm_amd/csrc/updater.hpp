@@ -1,0 +1,113 @@
+// updater.hpp -- MI355X counterpart of UpdaterGPUScBFM_AB_Type
+// (src/pscbfm/UpdaterGPUScBFM_AB_Type.h:176-518).  Same setters, same call
+// order, same exceptions; one HIP stream and one device-resident system per
+// object.  Coordinates are int32 on the device (the reference picks uint8 or
+// uint16 plus periodic-image counters, GPUScBFM_AB_Type.h:253-280, and caps
+// the box at 256; int32 removes the overflow repair and the cap).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bfm_kernels.hpp"
+#include "host_system.hpp"
+
+namespace pscbfm {
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t c, const std::string &what) : std::runtime_error(what), code(c) {}
+};
+struct NoDeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct StateError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char *what);
+
+class UpdaterGPUScBFM_AB_Type {
+public:
+    using T_BoxSize = uint64_t;
+    using T_Coordinate = int32_t;
+    using T_Id = uint32_t;
+
+    UpdaterGPUScBFM_AB_Type();
+    ~UpdaterGPUScBFM_AB_Type();
+    UpdaterGPUScBFM_AB_Type(const UpdaterGPUScBFM_AB_Type &) = delete;
+    UpdaterGPUScBFM_AB_Type &operator=(const UpdaterGPUScBFM_AB_Type &) = delete;
+
+    void setGpu(int gpu);
+    void copyBondSet(int dx, int dy, int dz, bool forbidden);
+    void setLatticeSize(T_BoxSize x, T_BoxSize y, T_BoxSize z);
+    void setNrOfAllMonomers(T_Id n);
+    void setAttribute(T_Id i, int32_t attribute);
+    void setMonomerCoordinates(T_Id i, T_Coordinate x, T_Coordinate y, T_Coordinate z);
+    void setConnectivity(T_Id i, T_Id j);
+    void setPeriodicity(bool px, bool py, bool pz);
+    void setAge(int64_t age) { mAge = age; }
+    int64_t getAge() const { return mAge; }
+    void setRng(int rng);
+    void setSeed(uint64_t seed) { mSeed = seed; }
+    void setStepsBetweenSortings(int64_t steps);
+
+    // host half of initialize(): checks and colouring, no device work
+    void validate();
+    void initialize();
+    void runSimulationOnGPU(uint32_t n_mcs);
+    int32_t getMonomerPositionInX(T_Id i) const { return at(i, 0); }
+    int32_t getMonomerPositionInY(T_Id i) const { return at(i, 1); }
+    int32_t getMonomerPositionInZ(T_Id i) const { return at(i, 2); }
+    void cleanup();
+
+    // extensions
+    void runResident(uint32_t n_mcs);
+    void copyBack();
+    void setKernelTiming(bool on) { mTiming = on; }
+    const HostSystem &system() const { return mSys; }
+    bool initialized() const { return mInitialized; }
+    bool validated() const { return mValidated; }
+    uint64_t attempts() const { return mAttempts; }
+    uint64_t substeps() const { return mSubsteps; }
+    size_t deviceBytes() const { return mDevBytes; }
+    // syncs the stream; also folds pending kernel timings into checkMs/performMs
+    void deviceCounters(uint64_t &accepted, uint64_t &proposals) const;
+    double checkMs() const { return mCheckMs; }
+    double performMs() const { return mPerformMs; }
+    uint64_t timedLaunches() const { return mTimedLaunches; }
+
+private:
+    int32_t at(T_Id i, int a) const;
+    void freeDevice();
+    void collectTimings() const;
+
+    HostSystem mSys;
+    int mGpu = -1;
+    int64_t mAge = 0;
+    uint64_t mSeed = 0x5EED000000000001ull;
+    bool mValidated = false;
+    bool mInitialized = false;
+    bool mMonomersSet = false;
+
+    hipStream_t mStream = nullptr;
+    SweepBuffers mBuf{};
+    void *mDevSlab = nullptr;   // one allocation holds every device array
+    size_t mDevBytes = 0;
+    uint64_t mSubsteps = 0;     // substep counter, its parity selects buffers
+    uint32_t mPrevBlocks = 0;   // grid of the previous check launch
+    uint64_t mAttempts = 0;
+
+    bool mTiming = false;
+    std::vector<hipEvent_t> mEventPool;          // reusable events
+    mutable std::vector<hipEvent_t> mEvents;     // pending (start, mid, end) triples
+    mutable double mCheckMs = 0.0, mPerformMs = 0.0;
+    mutable uint64_t mTimedLaunches = 0;
+    hipEvent_t takeEvent();
+};
+
+}  // namespace pscbfm

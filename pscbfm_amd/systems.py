@@ -1,0 +1,39 @@
+"""Seeded synthetic systems (linear melts) as plain dicts:
+{"box": (L, L, L), "periodic": (1, 1, 1), "xyz": int32 [n, 3],
+ "bonds": uint32 [m, 2] undirected, "forbidden": uint8 [512]}.
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+from . import _capi
+
+
+def standard_forbidden() -> np.ndarray:
+    """512-entry table of the standard 108-vector BFM bond set (1 = forbidden),
+    index (dx&7)|(dy&7)<<3|(dz&7)<<6 as UpdaterGPUScBFM_AB_Type.cu:886-888."""
+    f = np.ones(512, dtype=np.uint8)
+    allowed = {(0, 0, 2), (0, 1, 2), (1, 1, 2), (1, 2, 2), (0, 0, 3), (0, 1, 3)}
+    for d in itertools.product(range(-3, 4), repeat=3):
+        if tuple(sorted(abs(v) for v in d)) in allowed:
+            f[(d[0] & 7) | ((d[1] & 7) << 3) | ((d[2] & 7) << 6)] = 0
+    return f
+
+
+def linear_melt(box: int, n_chains: int, chain_len: int, seed: int) -> dict:
+    """Linear melt grown by bfm_gen_linear_melt (include/pscbfm_amd_systems.h)."""
+    lib = _capi.load()
+    n = n_chains * chain_len
+    xyz = np.empty((n, 3), dtype=np.int32)
+    bonds = np.empty((n_chains * max(chain_len - 1, 0), 2), dtype=np.uint32)
+    _capi.check(lib.bfm_gen_linear_melt(box, n_chains, chain_len, seed & 0xFFFFFFFFFFFFFFFF,
+                                        xyz.ctypes.data, bonds.ctypes.data if bonds.size else None))
+    return {"box": (box, box, box), "periodic": (1, 1, 1), "xyz": xyz, "bonds": bonds,
+            "forbidden": standard_forbidden(), "chain_len": chain_len}
+
+
+def melt_volume_fraction(system: dict) -> float:
+    L = system["box"]
+    return 8.0 * len(system["xyz"]) / float(L[0] * L[1] * L[2])

@@ -1,0 +1,206 @@
+"""Python face of the updater: the reference's UpdaterGPUScBFM_AB_Type API
+(src/pscbfm/UpdaterGPUScBFM_AB_Type.h:453-514) and its LeMonADE adapter
+GPUScBFM_AB_Type (src/pscbfm/GPUScBFM_AB_Type.h:27-286), over the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_int32, c_int64, c_uint32, c_void_p
+
+import numpy as np
+
+from . import _capi
+
+Rng_Philox = 1
+
+
+class UpdaterGPUScBFM_AB_Type:
+    """Same setters, call order and errors as the reference updater.
+
+    Errors: InvalidArgument (a ValueError) where the reference throws
+    std::invalid_argument, BfmError where it throws std::runtime_error.
+    """
+
+    def __init__(self):
+        self._lib = _capi.load()
+        h = c_void_p()
+        _capi.check(self._lib.bfm_create(byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.bfm_destroy(h)
+            self._h = None
+
+    # ---- reference setters --------------------------------------------------
+    def setGpu(self, gpu: int) -> None:
+        _capi.check(self._lib.bfm_set_gpu(self._h, gpu))
+
+    def copyBondSet(self, dx: int, dy: int, dz: int, forbidden: bool) -> None:
+        _capi.check(self._lib.bfm_copy_bond_set(self._h, dx, dy, dz, int(bool(forbidden))))
+
+    def setLatticeSize(self, x: int, y: int, z: int) -> None:
+        _capi.check(self._lib.bfm_set_lattice_size(self._h, x, y, z))
+
+    def setNrOfAllMonomers(self, n: int) -> None:
+        _capi.check(self._lib.bfm_set_nr_of_all_monomers(self._h, n))
+
+    def setAttribute(self, i: int, attribute: int) -> None:
+        _capi.check(self._lib.bfm_set_attribute(self._h, i, attribute))
+
+    def setMonomerCoordinates(self, i: int, x: int, y: int, z: int) -> None:
+        _capi.check(self._lib.bfm_set_monomer_coordinates(self._h, i, x, y, z))
+
+    def setConnectivity(self, i: int, j: int) -> None:
+        _capi.check(self._lib.bfm_set_connectivity(self._h, i, j))
+
+    def setPeriodicity(self, px: bool, py: bool, pz: bool) -> None:
+        _capi.check(self._lib.bfm_set_periodicity(self._h, int(px), int(py), int(pz)))
+
+    def setAge(self, age: int) -> None:
+        _capi.check(self._lib.bfm_set_age(self._h, age))
+
+    def getAge(self) -> int:
+        v = c_int64()
+        _capi.check(self._lib.bfm_get_age(self._h, byref(v)))
+        return v.value
+
+    def setRng(self, rng: int) -> None:
+        _capi.check(self._lib.bfm_set_rng(self._h, rng))
+
+    def setSeed(self, seed: int) -> None:
+        _capi.check(self._lib.bfm_set_seed(self._h, seed & 0xFFFFFFFFFFFFFFFF))
+
+    def setStepsBetweenSortings(self, steps: int) -> None:
+        _capi.check(self._lib.bfm_set_steps_between_sortings(self._h, steps))
+
+    def initialize(self) -> None:
+        _capi.check(self._lib.bfm_initialize(self._h))
+
+    def runSimulationOnGPU(self, n_mcs: int) -> None:
+        _capi.check(self._lib.bfm_run_simulation_on_gpu(self._h, n_mcs))
+
+    def getMonomerPositionInX(self, i: int) -> int:
+        v = c_int32()
+        _capi.check(self._lib.bfm_get_monomer_position_in_x(self._h, i, byref(v)))
+        return v.value
+
+    def getMonomerPositionInY(self, i: int) -> int:
+        v = c_int32()
+        _capi.check(self._lib.bfm_get_monomer_position_in_y(self._h, i, byref(v)))
+        return v.value
+
+    def getMonomerPositionInZ(self, i: int) -> int:
+        v = c_int32()
+        _capi.check(self._lib.bfm_get_monomer_position_in_z(self._h, i, byref(v)))
+        return v.value
+
+    def cleanup(self) -> None:
+        _capi.check(self._lib.bfm_cleanup(self._h))
+
+    # ---- extensions ---------------------------------------------------------
+    def validate(self) -> None:
+        _capi.check(self._lib.bfm_validate(self._h))
+
+    def setMonomerCoordinatesBulk(self, xyz: np.ndarray) -> None:
+        a = np.ascontiguousarray(xyz, dtype=np.int32).reshape(-1, 3)
+        _capi.check(self._lib.bfm_set_monomer_coordinates_bulk(self._h, a.ctypes.data, a.shape[0]))
+
+    def setConnectivityBulk(self, pairs: np.ndarray) -> None:
+        a = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+        _capi.check(self._lib.bfm_set_connectivity_bulk(self._h, a.ctypes.data, a.shape[0]))
+
+    def getMonomerPositions(self, n: int) -> np.ndarray:
+        out = np.empty((n, 3), dtype=np.int32)
+        _capi.check(self._lib.bfm_get_monomer_positions(self._h, out.ctypes.data, n))
+        return out
+
+    def getColors(self, n: int) -> tuple[np.ndarray, int]:
+        out = np.empty(n, dtype=np.uint32)
+        nc = c_uint32()
+        _capi.check(self._lib.bfm_get_colors(self._h, out.ctypes.data, n, byref(nc)))
+        return out, nc.value
+
+    def getStats(self) -> dict:
+        st = _capi.BfmStats()
+        _capi.check(self._lib.bfm_get_stats(self._h, byref(st)))
+        return {name: getattr(st, name) for name, _ in _capi.BfmStats._fields_}
+
+    def setKernelTiming(self, on: bool) -> None:
+        _capi.check(self._lib.bfm_set_kernel_timing(self._h, int(bool(on))))
+
+    def runResident(self, n_mcs: int) -> None:
+        _capi.check(self._lib.bfm_run_resident(self._h, n_mcs))
+
+    def copyBack(self) -> None:
+        _capi.check(self._lib.bfm_copy_back(self._h))
+
+
+def updater_from_system(system: dict, seed: int, gpu: int | None = 0, age: int = 0,
+                        validate_only: bool = False) -> UpdaterGPUScBFM_AB_Type:
+    """Builds an updater the way the LeMonADE adapter does
+    (GPUScBFM_AB_Type.h:120-184): box, periodicity, age, positions, both
+    directions of every bond, the 512-entry bond table, then initialize."""
+    u = UpdaterGPUScBFM_AB_Type()
+    if gpu is not None and not validate_only:
+        u.setGpu(gpu)
+    u.setAge(age)
+    u.setSeed(seed)
+    px, py, pz = system.get("periodic", (1, 1, 1))
+    u.setPeriodicity(bool(px), bool(py), bool(pz))
+    u.setLatticeSize(*[int(v) for v in system["box"]])
+    xyz = np.asarray(system["xyz"], dtype=np.int32).reshape(-1, 3)
+    u.setNrOfAllMonomers(xyz.shape[0])
+    u.setMonomerCoordinatesBulk(xyz)
+    bonds = np.asarray(system["bonds"], dtype=np.uint32).reshape(-1, 2)
+    if bonds.size:
+        both = np.empty((2 * bonds.shape[0], 2), dtype=np.uint32)
+        both[0::2] = bonds
+        both[1::2] = bonds[:, ::-1]
+        u.setConnectivityBulk(both)
+    forbidden = np.asarray(system["forbidden"], dtype=np.uint8)
+    for dx in range(-4, 4):
+        for dy in range(-4, 4):
+            for dz in range(-4, 4):
+                idx = (dx & 7) | ((dy & 7) << 3) | ((dz & 7) << 6)
+                u.copyBondSet(dx, dy, dz, bool(forbidden[idx]))
+    if validate_only:
+        u.validate()
+    else:
+        u.initialize()
+    return u
+
+
+class GPUScBFM_AB_Type:
+    """LeMonADE-style updater adapter (GPUScBFM_AB_Type.h:27-286).
+
+    `ingredients` needs: box (3 ints), periodic (3 bools), age (int, updated
+    by execute), xyz (n x 3 int array, updated by execute), bonds (m x 2
+    undirected pairs), forbidden (512 uint8), seed (int).
+    """
+
+    def __init__(self, ingredients, n_steps: int, gpu: int = 0):
+        self.ingredients = ingredients
+        self.n_steps = int(n_steps)
+        self.gpu = gpu
+        self._u = None
+
+    def initialize(self) -> None:
+        ing = self.ingredients
+        system = {"box": ing.box, "periodic": ing.periodic, "xyz": ing.xyz, "bonds": ing.bonds,
+                  "forbidden": ing.forbidden}
+        self._u = updater_from_system(system, seed=ing.seed, gpu=self.gpu, age=ing.age)
+
+    def execute(self) -> bool:
+        ing = self.ingredients
+        self._u.setAge(ing.age)
+        self._u.runSimulationOnGPU(self.n_steps)
+        ing.xyz = self._u.getMonomerPositions(np.asarray(ing.xyz).reshape(-1, 3).shape[0])
+        ing.age = ing.age + self.n_steps
+        return True
+
+    def cleanup(self) -> None:
+        if self._u is not None:
+            self._u.cleanup()
+            self._u = None

@@ -1,0 +1,93 @@
+"""Bit-exact parity of the HIP colour sweep with the CPU oracle's serial replay.
+
+The reference's only correctness test is an end-to-end golden-file diff of a
+simulated system (src/pscbfm/testCorrectness.sh:219-237); these tests do the
+same against the oracle's replay of the same sweep with the same RNG stream:
+final coordinates and the accepted / proposed move counts must match exactly.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from pscbfm_amd import systems, updater_from_system
+
+pytestmark = pytest.mark.gpu
+
+
+def run_pair(system, seed, chunks, age0=0):
+    n = len(system["xyz"])
+    print(f"case: n={n} box={system['box']} periodic={system.get('periodic')} seed={seed:#x} "
+          f"chunks={chunks} age0={age0}", flush=True)
+    u = updater_from_system(system, seed=seed, gpu=0, age=age0)
+    for c in chunks:
+        u.runSimulationOnGPU(c)
+    got = u.getMonomerPositions(n)
+    colors, nc = u.getColors(n)
+    stats = u.getStats()
+    osys = orc.System(system, seed, colors)
+    want, acc, prop = orc.sweep(osys, system["xyz"], age0, sum(chunks), 0)
+    u.cleanup()
+    return got, want, stats, acc, prop, osys
+
+
+def assert_parity(system, seed, chunks, age0=0):
+    got, want, stats, acc, prop, osys = run_pair(system, seed, chunks, age0)
+    diff = np.nonzero((got != want).any(axis=1))[0]
+    assert diff.size == 0, f"{diff.size} monomers differ, first {diff[:5]}: gpu {got[diff[:3]]} oracle {want[diff[:3]]}"
+    assert stats["accepted"] == acc
+    assert stats["proposals"] == prop
+    assert stats["attempts"] == sum(chunks) * len(system["xyz"])
+    rc, msg = orc.check_system(osys, got)
+    assert rc == 0, msg
+    return stats
+
+
+def test_tiny_melt(gpu_ready):
+    s = systems.linear_melt(16, 16, 16, seed=11)
+    st = assert_parity(s, seed=0x1234, chunks=[200])
+    assert st["accepted"] > 0
+
+
+def test_small_melt_1000_mcs(gpu_ready):
+    # the plain small-melt workload (64^3, 512 x N=32, 2 colours) for 1000 MCS
+    s = systems.linear_melt(64, 512, 32, seed=0x5EED000000000001)
+    assert_parity(s, seed=0x5EED000000000001, chunks=[1000])
+
+
+def test_resume_is_continuous(gpu_ready):
+    # counter RNG keyed on the age: split runs equal one long run
+    s = systems.linear_melt(32, 64, 16, seed=5)
+    assert_parity(s, seed=99, chunks=[37, 63], age0=1000)
+
+
+def test_hard_walls(gpu_ready):
+    s = systems.linear_melt(32, 100, 16, seed=6)
+    s = dict(s, periodic=(0, 1, 0))
+    assert_parity(s, seed=7, chunks=[300])
+    s = dict(s, periodic=(0, 0, 0))
+    assert_parity(s, seed=8, chunks=[300])
+
+
+def test_low_density_and_long_chains(gpu_ready):
+    s = systems.linear_melt(64, 40, 128, seed=21)
+    assert_parity(s, seed=22, chunks=[400])
+
+
+def test_non_cubic_box(gpu_ready):
+    # compact Morton order for a 64 x 16 x 32 box
+    s = systems.linear_melt(16, 16, 16, seed=31)
+    xyz = s["xyz"].copy()
+    big = {"box": (64, 16, 32), "periodic": (1, 1, 1), "forbidden": s["forbidden"]}
+    parts, bonds = [], []
+    for k, off in enumerate([(0, 0, 0), (16, 0, 0), (32, 0, 16), (48, 0, 16)]):
+        parts.append(xyz + np.array(off, dtype=np.int32))
+        bonds.append(s["bonds"] + k * len(xyz))
+    big["xyz"] = np.concatenate(parts)
+    big["bonds"] = np.concatenate(bonds)
+    assert_parity(big, seed=32, chunks=[250])
+
+
+def test_large_melt_full_size_10_mcs(gpu_ready):
+    # the large-melt workload (512^3, 16384 x N=512) at full size, 10 MCS
+    s = systems.linear_melt(512, 16384, 512, seed=0x5EED000000000002)
+    assert_parity(s, seed=0x5EED000000000002, chunks=[10])

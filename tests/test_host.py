@@ -1,0 +1,148 @@
+"""Host logic of the product library, no GPU needed: the C ABI loads and
+exports every declared symbol, the setters and initialize's checks raise what
+the reference raises, colouring and the generators are right."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import pscbfm_amd
+from oracle import oracle as orc
+from pscbfm_amd import BfmError, InvalidArgument, systems, updater_from_system
+from pscbfm_amd import _capi
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(bfm_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = pscbfm_amd.load()
+    names = declared_functions()
+    assert len(names) >= 30
+    for n in sorted(names):
+        assert hasattr(lib, n), f"{n} declared in include/ but not exported"
+    assert names <= set(_capi.SIGNATURES), "binding table misses a declared function"
+
+
+def test_last_error_is_thread_local_message():
+    u = pscbfm_amd.UpdaterGPUScBFM_AB_Type()
+    with pytest.raises(InvalidArgument) as e:
+        u.setRng(5)
+    assert "not available" in str(e.value)
+
+
+def fresh():
+    return pscbfm_amd.UpdaterGPUScBFM_AB_Type()
+
+
+def test_setter_errors_mirror_reference():
+    u = fresh()
+    with pytest.raises(BfmError, match="power of 2"):
+        u.setLatticeSize(48, 64, 64)                 # runtime_error, .cu:2673
+    u.setNrOfAllMonomers(3)
+    with pytest.raises(BfmError, match="already set"):
+        u.setNrOfAllMonomers(3)                      # runtime_error, .cu:2550-2556
+    with pytest.raises(InvalidArgument):
+        u.setMonomerCoordinates(3, 0, 0, 0)          # index out of range
+    for _ in range(8):
+        u.setConnectivity(0, 1)
+    with pytest.raises(InvalidArgument, match="maximum amount of bonds"):
+        u.setConnectivity(0, 2)                      # invalid_argument, .cu:2625-2633
+    with pytest.raises(InvalidArgument):
+        u.copyBondSet(4, 0, 0, True)                 # outside [-4,3]
+    with pytest.raises(InvalidArgument):
+        u.setStepsBetweenSortings(0)
+
+
+def valid_melt():
+    return systems.linear_melt(16, 16, 16, seed=3)
+
+
+def test_validate_accepts_melt_and_colours_like_oracle():
+    s = systems.linear_melt(32, 64, 16, seed=4)
+    u = updater_from_system(s, seed=1, validate_only=True)
+    col, nc = u.getColors(len(s["xyz"]))
+    o = orc.System(s, 1)
+    assert nc == o.n_colors == 2
+    assert np.array_equal(col, o.colors)
+
+
+def test_validate_rejects_wrong_bond_set():
+    s = valid_melt()
+    f = s["forbidden"].copy()
+    f[(2 & 7) | ((2 & 7) << 3)] = 0                   # allow (2,2,0): 109 vectors
+    with pytest.raises(BfmError, match="Expected 108 allowed bonds, but got 109"):
+        updater_from_system(dict(s, forbidden=f), seed=1, validate_only=True)
+
+
+def test_validate_rejects_overlap_and_bad_bond():
+    s = valid_melt()
+    x = s["xyz"].copy()
+    x[7] = x[100] + np.array([0, 1, 1])
+    with pytest.raises(BfmError, match="overlaps"):
+        updater_from_system(dict(s, xyz=x), seed=1, validate_only=True)
+    x = s["xyz"].copy()
+    # move the chain end 0 so that its bond to 1 becomes (2,2,0)
+    x[0] = x[1] + np.array([2, 2, 0])
+    occupied = {tuple(v) for v in x[1:]}
+    if tuple(x[0]) not in occupied:
+        with pytest.raises(BfmError):
+            updater_from_system(dict(s, xyz=x), seed=1, validate_only=True)
+
+
+def test_validate_rejects_walls_and_box():
+    s = valid_melt()
+    x = s["xyz"].copy()
+    x[:, 2] += 1                                     # cubes reach z = L
+    with pytest.raises(BfmError, match="walls"):
+        updater_from_system(dict(s, xyz=x, periodic=(1, 1, 0)), seed=1, validate_only=True)
+    with pytest.raises(BfmError, match="power of two"):
+        tiny = {"box": (4, 4, 4), "xyz": np.zeros((1, 3), np.int32), "bonds": np.zeros((0, 2), np.uint32),
+                "forbidden": s["forbidden"]}
+        updater_from_system(tiny, seed=1, validate_only=True)
+
+
+def test_initialize_without_device_fails_loudly():
+    if pscbfm_amd.device_count() > 0:
+        pytest.skip("a HIP device is visible")
+    u = fresh()
+    with pytest.raises(pscbfm_amd.NoDeviceError):
+        u.setGpu(0)
+    s = valid_melt()
+    with pytest.raises(pscbfm_amd.NoDeviceError):
+        updater_from_system(s, seed=1, gpu=None)
+
+
+@pytest.mark.parametrize("box,chains,n", [(16, 16, 16), (32, 64, 32), (64, 100, 64), (64, 8, 1), (32, 30, 7)])
+def test_generator_produces_valid_seeded_melts(box, chains, n):
+    a = systems.linear_melt(box, chains, n, seed=10)
+    b = systems.linear_melt(box, chains, n, seed=10)
+    c = systems.linear_melt(box, chains, n, seed=11)
+    assert np.array_equal(a["xyz"], b["xyz"])
+    assert not np.array_equal(a["xyz"], c["xyz"]) or chains * n == 1
+    o = orc.System(a, 1)
+    rc, msg = orc.check_system(o, a["xyz"])
+    assert rc == 0, msg
+    assert len(a["bonds"]) == chains * (n - 1)
+
+
+def test_generator_rejects_overfull_box():
+    with pytest.raises(InvalidArgument):
+        systems.linear_melt(16, 17, 16, seed=1)      # 17 chains of 16 in 16^3 exceeds 0.5
+
+
+def test_large_melt_shape():
+    s = systems.linear_melt(128, 256, 512, seed=2)
+    assert systems.melt_volume_fraction(s) == 0.5
+    o = orc.System(s, 1)
+    assert orc.check_system(o, s["xyz"])[0] == 0

@@ -1,0 +1,216 @@
+"""Pins of the CPU oracle against facts that do not come from the oracle itself:
+published Philox4x32-10 known-answer vectors, the textbook BFM bond set, the
+reference's own front-cell enumeration, brute force and closed forms.
+"""
+import itertools
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+# ---- Philox4x32-10 ---------------------------------------------------------
+# Known-answer vectors published with Random123 (kat_vectors, philox4x32 R=10).
+KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,want", KAT)
+def test_philox_known_answers(ctr, key, want):
+    assert orc.philox(ctr, key) == want
+
+
+def test_move_word_layout():
+    # word 0 of Philox(key=(seed_lo, seed_hi), ctr=(id, age_lo, age_hi, 0))
+    seed, age, mid = 0x0123456789ABCDEF, 0x00000002_80000001, 4242
+    want = orc.philox([mid, age & 0xFFFFFFFF, age >> 32, 0], [seed & 0xFFFFFFFF, seed >> 32])[0]
+    assert orc.move_word(seed, age, mid) == want
+
+
+def chi2_uniform(counts):
+    counts = np.asarray(counts, dtype=float)
+    e = counts.sum() / counts.size
+    return ((counts - e) ** 2 / e).sum()
+
+
+def test_directions_uniform():
+    # 60000 draws into 6 directions: chi^2 with 5 dof, 99.9% quantile 20.5
+    counts = np.zeros(6)
+    for mid in range(20000):
+        for age in range(3):
+            counts[orc.move_word(99, age, mid) % 6] += 1
+    assert chi2_uniform(counts) < 20.5
+
+
+def test_colour_order_is_uniform_permutation():
+    assert orc.color_order(1, 5, 1) == [0]
+    seen = {}
+    for age in range(6000):
+        p = tuple(orc.color_order(12345, age, 3))
+        assert sorted(p) == [0, 1, 2]
+        seen[p] = seen.get(p, 0) + 1
+    assert len(seen) == 6
+    assert chi2_uniform(list(seen.values())) < 20.5     # 5 dof
+    for n in (2, 5, 17, 255):
+        p = orc.color_order(7, 3, n)
+        assert sorted(p) == list(range(n))
+
+
+# ---- bond set ----------------------------------------------------------------
+def test_standard_bondset_is_textbook_set():
+    # BFM bond lengths are sqrt(4), sqrt(5), sqrt(6), 3, sqrt(10): every vector
+    # with |v|^2 in {4,5,6,9,10}; exactly 108 of them (UpdaterGPUScBFM_AB_Type.cu:1526)
+    f = orc.standard_bondset()
+    allowed = set()
+    for d in itertools.product(range(-4, 4), repeat=3):
+        idx = (d[0] & 7) | ((d[1] & 7) << 3) | ((d[2] & 7) << 6)
+        if d[0] * d[0] + d[1] * d[1] + d[2] * d[2] in (4, 5, 6, 9, 10):
+            allowed.add(idx)
+    assert len(allowed) == 108
+    assert {i for i in range(512) if f[i] == 0} == allowed
+
+
+def test_product_bondset_table_matches():
+    from pscbfm_amd import systems
+    assert np.array_equal(systems.standard_forbidden(), orc.standard_bondset())
+
+
+# ---- the reference's front-cell enumeration ---------------------------------
+def test_front_cells_match_reference_study():
+    """src/findBestZCurveFrontOrder.cpp enumerates the 54 cells that a move can
+    newly collide with (one axis at distance 2, the others within 1) -- the
+    cells the oracle's reference form (form 1) tests."""
+    exe = ROOT / "oracle" / "_ref" / "findBestZCurveFrontOrder"
+    if not exe.exists():
+        if not Path("/root/reference/src/findBestZCurveFrontOrder.cpp").exists() or not shutil.which("g++"):
+            pytest.skip("reference study program not built (reference sources absent)")
+        subprocess.run(["make", "-s", "-f", str(ROOT / "oracle" / "ref.mk")], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    first = [ln for ln in out.splitlines() if ln.startswith("Best whole ordering")][0]
+    cells = [tuple(int(v) for v in c.split(",")) for c in first.split(": ", 1)[1].strip(" ,").replace("), (", ")|(").strip("()").split(")|(")]
+    want = {d for d in itertools.product(range(-2, 3), repeat=3)
+            if sum(abs(v) == 2 for v in d) == 1 and all(abs(v) <= 1 or abs(v) == 2 for v in d)}
+    assert len(cells) == 54
+    assert set(cells) == want
+
+    # the printed order is Morton order with x in bit 0 (part1by2 closed form)
+    def part1by2(v):
+        r = 0
+        for b in range(10):
+            r |= ((v >> b) & 1) << (3 * b)
+        return r
+    c0 = (2, 2, 2)
+    key = [part1by2(c0[0] + d[0]) | part1by2(c0[1] + d[1]) << 1 | part1by2(c0[2] + d[2]) << 2 for d in cells]
+    assert key == sorted(key)
+
+
+# ---- sweep -------------------------------------------------------------------
+def melt(box, n_chains, n, seed):
+    from pscbfm_amd import systems
+    return systems.linear_melt(box, n_chains, n, seed)
+
+
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 1, 0), (0, 0, 0)])
+@pytest.mark.parametrize("case", [(16, 16, 16, 3), (16, 8, 32, 4), (32, 64, 16, 5)])
+def test_three_forms_agree(case, periodic):
+    """Face-claim counting (form 0), the reference's corner lattice with the
+    9-site front of UpdaterGPUScBFM_AB_Type.cu:546-594 (form 1) and pairwise
+    brute force (form 2) give the same trajectory."""
+    box, chains, n, seed = case
+    s = dict(melt(box, chains, n, seed), periodic=periodic)
+    sys_ = orc.System(s, seed * 7 + 1)
+    outs = [orc.sweep(sys_, s["xyz"], 10, 60, form) for form in (0, 1, 2)]
+    for x, acc, prop in outs[1:]:
+        assert np.array_equal(x, outs[0][0])
+        assert (acc, prop) == outs[0][1:]
+    assert outs[0][1] > 0
+    assert orc.check_system(sys_, outs[0][0])[0] == 0
+
+
+def test_single_monomer_walk_closed_form():
+    """One free monomer: every proposal is accepted, so the trajectory is the
+    sum of the direction table entries (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z,
+    UpdaterGPUScBFM_AB_Type.cu:1537-1541) over the move words."""
+    s = {"box": (16, 16, 16), "periodic": (1, 1, 1), "xyz": np.array([[3, 4, 5]], dtype=np.int32),
+         "bonds": np.zeros((0, 2), dtype=np.uint32), "forbidden": orc.standard_bondset()}
+    sys_ = orc.System(s, 2024)
+    x, acc, prop = orc.sweep(sys_, s["xyz"], 7, 200, 0)
+    table = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
+    want = np.array([3, 4, 5])
+    for age in range(7, 207):
+        want += table[orc.move_word(2024, age, 0) % 6]
+    assert acc == prop == 200
+    assert np.array_equal(x[0], want)
+
+
+def test_wall_blocks_outward_moves():
+    # a free monomer against a hard wall never crosses it
+    s = {"box": (8, 8, 8), "periodic": (0, 0, 0), "xyz": np.array([[0, 0, 0]], dtype=np.int32),
+         "bonds": np.zeros((0, 2), dtype=np.uint32), "forbidden": orc.standard_bondset()}
+    sys_ = orc.System(s, 5)
+    x = s["xyz"]
+    for k in range(50):
+        x, _, _ = orc.sweep(sys_, x, k, 1, 0)
+        assert (x >= 0).all() and (x <= 6).all()
+
+
+def test_dimer_moves_keep_bond_in_set():
+    # two bonded monomers: brute-force check that every accepted state has a legal bond
+    f = orc.standard_bondset()
+    s = {"box": (16, 16, 16), "periodic": (1, 1, 1), "xyz": np.array([[2, 2, 2], [4, 2, 2]], dtype=np.int32),
+         "bonds": np.array([[0, 1]], dtype=np.uint32), "forbidden": f}
+    sys_ = orc.System(s, 9)
+    x = s["xyz"]
+    seen = set()
+    for k in range(400):
+        x, _, _ = orc.sweep(sys_, x, k, 1, 0)
+        d = tuple(int(v) for v in x[1] - x[0])
+        assert f[(d[0] & 7) | ((d[1] & 7) << 3) | ((d[2] & 7) << 6)] == 0
+        seen.add(d)
+    assert len(seen) > 10
+
+
+def test_random_sequential_keeps_invariants():
+    s = melt(32, 64, 16, 12)
+    sys_ = orc.System(s, 3)
+    x, acc = orc.random_sequential(sys_, s["xyz"], 0, 100)
+    assert acc > 0
+    assert orc.check_system(sys_, x)[0] == 0
+
+
+# ---- invariant checker catches planted mistakes ---------------------------
+def test_check_system_catches_planted_errors():
+    s = melt(16, 16, 16, 3)
+    sys_ = orc.System(s, 1)
+    x = s["xyz"].copy()
+    assert orc.check_system(sys_, x)[0] == 0
+    bad = x.copy()
+    bad[5] = bad[40] + np.array([1, 0, 0])        # overlap
+    assert orc.check_system(sys_, bad)[0] != 0
+    bad = x.copy()
+    bad[1] = bad[0] + np.array([2, 2, 0])         # |v|^2 = 8 is not a BFM bond
+    rc, msg = orc.check_system(sys_, bad)
+    assert rc != 0
+    walls = orc.System(dict(s, periodic=(0, 0, 0)), 1)
+    bad = x.copy()
+    bad[:, 0] -= 1                                # everything shifted through the x = 0 wall
+    assert orc.check_system(walls, bad)[0] != 0
+    same = orc.System(s, 1, colors=np.zeros(len(x), dtype=np.uint32))
+    assert orc.check_system(same, x)[0] != 0     # bonded monomers sharing a colour
+
+
+def test_greedy_colouring_alternates_chains():
+    s = melt(16, 16, 16, 3)
+    sys_ = orc.System(s, 1)
+    assert sys_.n_colors == 2
+    assert np.array_equal(sys_.colors, np.tile([0, 1], len(s["xyz"]) // 2))
