@@ -223,8 +223,14 @@ int orc_check_system(const orc_system *s, const int32_t *pos, char *msg, int msg
                 break;
             }
             int32_t d[3];
-            for (int a = 0; a < 3; ++a)
+            for (int a = 0; a < 3; ++a) {
                 d[a] = pos[3 * (size_t)j + a] - pos[3 * (size_t)i + a];
+                if (s->periodic[a]) {
+                    /* minimum image on periodic axes */
+                    int32_t m = d[a] & g.M[a];
+                    d[a] = m > g.L[a] / 2 ? m - g.L[a] : m;
+                }
+            }
             if (d[0] < -3 || d[0] > 3 || d[1] < -3 || d[1] > 3 || d[2] < -3 || d[2] > 3) {
                 if (msg) snprintf(msg, msglen, "bond %u-%u too long (%d,%d,%d)", i, j, d[0], d[1], d[2]);
                 rc = 5;

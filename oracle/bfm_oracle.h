@@ -67,7 +67,8 @@ void orc_standard_bondset(uint8_t forbidden[512]);
 
 /* Invariants of UpdaterGPUScBFM_AB_Type.cu:2789-2900 on an 8-site cube
  * occupancy lattice: no two cubes share a site, every bond |d_i|<=3 and
- * allowed, non-periodic axes keep the cube inside [0, L-1].
+ * allowed (difference taken as the minimum image on periodic axes),
+ * non-periodic axes keep the cube inside [0, L-1].
  * Returns 0 if valid; otherwise a positive code and a message. */
 int orc_check_system(const orc_system *sys, const int32_t *pos, char *msg, int msglen);
 

@@ -103,6 +103,10 @@ SIGNATURES = {
     "bfm_copy_back": (c_int, [_H]),
     "bfm_device_count": (c_int, [POINTER(c_int)]),
     "bfm_gen_linear_melt": (c_int, [c_uint32, c_uint32, c_uint32, c_uint64, c_void_p, c_void_p]),
+    "bfm_gen_diamond_network_size": (c_int, [c_uint32, c_uint32, c_uint32, c_uint32, c_uint32,
+                                             POINTER(c_uint32), POINTER(c_uint32)]),
+    "bfm_gen_diamond_network": (c_int, [c_uint32, c_uint32, c_uint32, c_uint32, c_uint32, c_uint64,
+                                        c_void_p, c_void_p]),
 }
 
 _lib = None

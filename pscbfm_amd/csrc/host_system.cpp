@@ -224,9 +224,20 @@ void plan(HostSystem &s)
                 msg << "monomer " << i << " is linked to monomer " << p << " which does not exist";
                 throw std::invalid_argument(msg.str());
             }
-            const int32_t d[3] = {s.xyz[3 * size_t(p)] - s.xyz[3 * size_t(i)],
-                                  s.xyz[3 * size_t(p) + 1] - s.xyz[3 * size_t(i) + 1],
-                                  s.xyz[3 * size_t(p) + 2] - s.xyz[3 * size_t(i) + 2]};
+            int32_t d[3];
+            for (int a = 0; a < 3; ++a) {
+                d[a] = s.xyz[3 * size_t(p) + a] - s.xyz[3 * size_t(i) + a];
+                if (s.periodic[a]) {
+                    // minimum image: a periodic network cannot be unwrapped, so a
+                    // bond is its difference modulo the box (the kernels' (d & 7)
+                    // index is the same modulo any box that is a multiple of 8)
+                    const int64_t L = int64_t(s.box[a]);
+                    int64_t m = ((int64_t(d[a]) % L) + L) % L;
+                    if (m > L / 2)
+                        m -= L;
+                    d[a] = int32_t(m);
+                }
+            }
             const bool too_long = std::abs(d[0]) > 3 || std::abs(d[1]) > 3 || std::abs(d[2]) > 3;
             if (too_long || s.forbidden[bond_table_index(d[0], d[1], d[2])]) {
                 std::ostringstream msg;

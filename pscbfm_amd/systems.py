@@ -34,6 +34,23 @@ def linear_melt(box: int, n_chains: int, chain_len: int, seed: int) -> dict:
             "forbidden": standard_forbidden(), "chain_len": chain_len}
 
 
+def diamond_network(box: int, cells: tuple, strand_len: int, seed: int) -> dict:
+    """End-linked diamond network grown by bfm_gen_diamond_network
+    (include/pscbfm_amd_systems.h)."""
+    import ctypes
+    lib = _capi.load()
+    nm, nb = ctypes.c_uint32(), ctypes.c_uint32()
+    cx, cy, cz = cells
+    _capi.check(lib.bfm_gen_diamond_network_size(box, cx, cy, cz, strand_len, ctypes.byref(nm), ctypes.byref(nb)))
+    xyz = np.empty((nm.value, 3), dtype=np.int32)
+    bonds = np.empty((nb.value, 2), dtype=np.uint32)
+    _capi.check(lib.bfm_gen_diamond_network(box, cx, cy, cz, strand_len, seed & 0xFFFFFFFFFFFFFFFF,
+                                            xyz.ctypes.data, bonds.ctypes.data))
+    return {"box": (box, box, box), "periodic": (1, 1, 1), "xyz": xyz, "bonds": bonds,
+            "forbidden": standard_forbidden(), "strand_len": strand_len,
+            "crosslinkers": 8 * cx * cy * cz}
+
+
 def melt_volume_fraction(system: dict) -> float:
     L = system["box"]
     return 8.0 * len(system["xyz"]) / float(L[0] * L[1] * L[2])

@@ -91,3 +91,16 @@ def test_large_melt_full_size_10_mcs(gpu_ready):
     # the large-melt workload (512^3, 16384 x N=512) at full size, 10 MCS
     s = systems.linear_melt(512, 16384, 512, seed=0x5EED000000000002)
     assert_parity(s, seed=0x5EED000000000002, chunks=[10])
+
+
+def test_network_three_colours(gpu_ready):
+    s = systems.diamond_network(64, (2, 2, 8), 32, seed=41)
+    st = assert_parity(s, seed=42, chunks=[150, 150])
+    assert st["accepted"] > 0
+
+
+def test_network_full_size_20_mcs(gpu_ready):
+    # the end-linked network workload (256^3, 16384 crosslinkers, 32768 strands of N=32)
+    s = systems.diamond_network(256, (8, 8, 32), 32, seed=0x5EED000000000003)
+    assert len(s["xyz"]) == 1064960
+    assert_parity(s, seed=0x5EED000000000003, chunks=[20])

@@ -146,3 +146,27 @@ def test_large_melt_shape():
     assert systems.melt_volume_fraction(s) == 0.5
     o = orc.System(s, 1)
     assert orc.check_system(o, s["xyz"])[0] == 0
+
+
+@pytest.mark.parametrize("box,cells,n", [(64, (2, 2, 8), 32), (64, (4, 4, 4), 13), (32, (2, 2, 4), 8)])
+def test_network_generator_valid(box, cells, n):
+    a = systems.diamond_network(box, cells, n, seed=5)
+    b = systems.diamond_network(box, cells, n, seed=5)
+    assert np.array_equal(a["xyz"], b["xyz"])
+    k = 8 * cells[0] * cells[1] * cells[2]
+    assert len(a["xyz"]) == k + 2 * k * n
+    assert len(a["bonds"]) == 2 * k * (n + 1)
+    deg = np.bincount(a["bonds"].ravel(), minlength=len(a["xyz"]))
+    assert (deg[:k] == 4).all() and (deg[k:] == 2).all()     # 4-functional crosslinkers, linear strands
+    o = orc.System(a, 1)
+    rc, msg = orc.check_system(o, a["xyz"])
+    assert rc == 0, msg
+    u = updater_from_system(a, seed=1, validate_only=True)
+    col, nc = u.getColors(len(a["xyz"]))
+    assert nc == o.n_colors >= 2
+    assert np.array_equal(col, o.colors)
+
+
+def test_network_generator_rejects_parity():
+    with pytest.raises(InvalidArgument, match="parity"):
+        systems.diamond_network(64, (2, 2, 8), 31, seed=1)   # 32 steps vs spacing 4+4+1

@@ -214,3 +214,15 @@ def test_greedy_colouring_alternates_chains():
     sys_ = orc.System(s, 1)
     assert sys_.n_colors == 2
     assert np.array_equal(sys_.colors, np.tile([0, 1], len(s["xyz"]) // 2))
+
+
+def test_three_forms_agree_on_network():
+    from pscbfm_amd import systems
+    s = systems.diamond_network(64, (2, 2, 8), 32, seed=3)
+    sys_ = orc.System(s, 17)
+    assert sys_.n_colors == 3
+    outs = [orc.sweep(sys_, s["xyz"], 0, 30, form) for form in (0, 1, 2)]
+    for x, acc, prop in outs[1:]:
+        assert np.array_equal(x, outs[0][0])
+        assert (acc, prop) == outs[0][1:]
+    assert orc.check_system(sys_, outs[0][0])[0] == 0
