@@ -35,6 +35,16 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per k_check launch from the committed rocprofv3 PMC summary."""
+    import re
+    f = ROOT / "profiles" / "k_check_large_melt_hbm_traffic.md"
+    if workload != "large_melt" or not f.exists():
+        return None
+    m = re.search(r"```json\n(.*?)\n```", f.read_text(), flags=re.S)
+    return json.loads(m.group(1))["traffic_bytes"] if m else None
+
+
 def cpu_baseline(system: dict, seed: int, target_s: float) -> dict:
     from oracle import oracle as orc
     osys = orc.System(system, seed)
@@ -59,6 +69,8 @@ def main() -> int:
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large_melt")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=1024,
+                    help="size of the plain copy kernel that calibrates HBM counters (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this engine runs every system on a single device; "
@@ -69,6 +81,14 @@ def main() -> int:
     if device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
+
+    copy_gbs = None
+    if args.copy_mib > 0:
+        from pscbfm_amd import _capi
+        nbytes = args.copy_mib << 20
+        ms = _capi.calibration_copy(0, nbytes, 10)
+        copy_gbs = 2.0 * nbytes / (ms * 1e-3) / 1e9
+        log(f"plain copy of {args.copy_mib} MiB: {ms:.3f} ms -> {copy_gbs:.0f} GB/s (read + write)")
 
     w = WORKLOADS[args.workload]
     t = time.perf_counter()
@@ -104,10 +124,11 @@ def main() -> int:
     achieved = per_launch * b_check / (check_ms * 1e-3) / 1e9
     b_attempt = 12.5 + 16.0 * kbar + 13.0 * p_acc   # whole substep, per attempt
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
                 "kernel": "k_check", "kernel_avg_ms": round(check_ms, 4),
                 "perform_avg_ms": round(perform_ms, 4), "bytes_per_attempt_check": round(b_check, 3),
-                "path_frac": round(value * b_attempt / 1e9 / HBM_PEAK_GBS, 4)}
+                "path_frac": round(value * b_attempt / 1e9 / HBM_PEAK_GBS, 4),
+                "copy_kernel_gbs": None if copy_gbs is None else round(copy_gbs, 1)}
 
     out = {
         "metric": "monomer move attempts/s (plain colour sweep, bit-exact)",

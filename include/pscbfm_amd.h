@@ -153,6 +153,69 @@ int bfm_run_resident(bfm_updater *h, uint32_t n_mcs);
 /* Copies the device state back to the host (the doCopyBack half). */
 int bfm_copy_back(bfm_updater *h);
 
+/* ---- batched form: many independent systems per launch ----------------------
+ * No counterpart in the reference, whose box and bond tables are module-global
+ * __constant__ symbols (UpdaterGPUScBFM_AB_Type.cu:84-133), so one process
+ * can run only one system size at a time.  A batch sweeps every member in the
+ * same launches (one check + one perform launch per colour slot for all
+ * members), captured as a hipGraph per chunk of MCS.  Each member's trajectory
+ * is bit-identical to running it alone with bfm_run_simulation_on_gpu. */
+typedef struct bfm_batch bfm_batch;
+int bfm_batch_create(bfm_batch **out, int gpu);
+int bfm_batch_destroy(bfm_batch *b);
+/* Adds the system a configured updater describes (setters done; it need not
+ * be initialized; it keeps its own copy).  Uses its seed and age. */
+int bfm_batch_add(bfm_batch *b, bfm_updater *h, uint32_t *member);
+/* Device bytes the updater's system takes as a batch member. */
+int bfm_member_bytes(bfm_updater *h, uint64_t *bytes);
+/* n_mcs MCS on every member, in chunks of `chunk` MCS (one graph per chunk). */
+int bfm_batch_run(bfm_batch *b, uint32_t n_mcs, uint32_t chunk);
+/* The same for the listed members only. */
+int bfm_batch_run_members(bfm_batch *b, const uint32_t *members, uint32_t n_members, uint32_t n_mcs,
+                          uint32_t chunk);
+int bfm_batch_get_positions(bfm_batch *b, uint32_t member, int32_t *xyz, uint32_t n);
+int bfm_batch_get_age(bfm_batch *b, uint32_t member, int64_t *age);
+/* attempts = 0 here (the batch does not track them); accepted and proposals
+ * are the member's device counters. */
+int bfm_batch_get_stats(bfm_batch *b, uint32_t member, bfm_stats *out);
+int bfm_batch_remove(bfm_batch *b, uint32_t member);
+int bfm_batch_set_graphs(bfm_batch *b, int on);
+int bfm_batch_synchronize(bfm_batch *b);
+
+/* ---- served form: continuous batching of simulation jobs --------------------
+ * No counterpart in the reference.  Jobs queue in arrival order; every
+ * bfm_server_step admits queued jobs while their device bytes fit the budget,
+ * runs one chunk of MCS on all admitted jobs together and retires finished
+ * ones (positions copied back, device memory freed).  States: */
+enum { BFM_JOB_QUEUED = 0, BFM_JOB_RUNNING = 1, BFM_JOB_DONE = 2, BFM_JOB_FAILED = 3, BFM_JOB_CANCELLED = 4 };
+typedef struct bfm_server bfm_server;
+/* hbm_budget = 0: 80% of the free device memory at creation. */
+int bfm_server_create(bfm_server **out, int gpu, uint64_t hbm_budget, uint32_t chunk_mcs);
+int bfm_server_destroy(bfm_server *s);
+/* Queues the configured updater's system (validated here) for n_mcs MCS.
+ * A job larger than the whole budget is accepted as BFM_JOB_FAILED. */
+int bfm_server_submit(bfm_server *s, bfm_updater *h, uint32_t n_mcs, uint64_t *job);
+/* One scheduling round; *pending = jobs still queued or running. */
+int bfm_server_step(bfm_server *s, uint32_t *pending);
+int bfm_server_job_state(bfm_server *s, uint64_t job, int *state, uint32_t *mcs_done);
+/* Seconds since server creation at submission, admission and completion (-1 if not yet). */
+int bfm_server_job_times(bfm_server *s, uint64_t job, double *submitted, double *admitted, double *finished);
+int bfm_server_job_result(bfm_server *s, uint64_t job, int32_t *xyz, uint32_t n);
+/* Thread-local message of a failed job. */
+int bfm_server_job_error(bfm_server *s, uint64_t job, char *buf, uint32_t len);
+int bfm_server_cancel(bfm_server *s, uint64_t job);
+int bfm_server_release(bfm_server *s, uint64_t job);
+/* Queued / running jobs, device bytes in use and budget, attempts done. */
+int bfm_server_info(bfm_server *s, uint32_t *queued, uint32_t *running, uint64_t *bytes_in_use,
+                    uint64_t *budget, uint64_t *attempts);
+
+/* Measurement helper, no counterpart in the reference: times `repeats`
+ * launches of a plain 16-byte-per-lane copy kernel over `bytes` on device
+ * `gpu` (after one warm-up launch).  A copy of B bytes reads B and writes B
+ * bytes of HBM, which calibrates the FETCH_SIZE / WRITE_SIZE counters and
+ * gives the achievable streaming bandwidth 2*B / avg_ms. */
+int bfm_calibration_copy(int gpu, uint64_t bytes, int repeats, double *avg_ms);
+
 /* Number of HIP devices visible (0 on a machine without a GPU). */
 int bfm_device_count(int *count);
 

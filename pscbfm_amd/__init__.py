@@ -6,9 +6,13 @@ C ABI (include/pscbfm_amd.h).
 """
 from ._capi import BfmError, HipError, InvalidArgument, NoDeviceError, StateError, device_count, load
 from .updater import GPUScBFM_AB_Type, Rng_Philox, UpdaterGPUScBFM_AB_Type, updater_from_system
+from .batch import Batch, Server, member_bytes
 
 __all__ = [
+    "Batch",
     "BfmError",
+    "Server",
+    "member_bytes",
     "GPUScBFM_AB_Type",
     "HipError",
     "InvalidArgument",

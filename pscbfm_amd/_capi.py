@@ -101,7 +101,32 @@ SIGNATURES = {
     "bfm_set_kernel_timing": (c_int, [_H, c_int]),
     "bfm_run_resident": (c_int, [_H, c_uint32]),
     "bfm_copy_back": (c_int, [_H]),
+    "bfm_batch_create": (c_int, [POINTER(c_void_p), c_int]),
+    "bfm_batch_destroy": (c_int, [_H]),
+    "bfm_batch_add": (c_int, [_H, _H, POINTER(c_uint32)]),
+    "bfm_member_bytes": (c_int, [_H, POINTER(c_uint64)]),
+    "bfm_batch_run": (c_int, [_H, c_uint32, c_uint32]),
+    "bfm_batch_run_members": (c_int, [_H, c_void_p, c_uint32, c_uint32, c_uint32]),
+    "bfm_batch_get_positions": (c_int, [_H, c_uint32, c_void_p, c_uint32]),
+    "bfm_batch_get_age": (c_int, [_H, c_uint32, POINTER(c_int64)]),
+    "bfm_batch_get_stats": (c_int, [_H, c_uint32, POINTER(BfmStats)]),
+    "bfm_batch_remove": (c_int, [_H, c_uint32]),
+    "bfm_batch_set_graphs": (c_int, [_H, c_int]),
+    "bfm_batch_synchronize": (c_int, [_H]),
+    "bfm_server_create": (c_int, [POINTER(c_void_p), c_int, c_uint64, c_uint32]),
+    "bfm_server_destroy": (c_int, [_H]),
+    "bfm_server_submit": (c_int, [_H, _H, c_uint32, POINTER(c_uint64)]),
+    "bfm_server_step": (c_int, [_H, POINTER(c_uint32)]),
+    "bfm_server_job_state": (c_int, [_H, c_uint64, POINTER(c_int), POINTER(c_uint32)]),
+    "bfm_server_job_times": (c_int, [_H, c_uint64, POINTER(c_double), POINTER(c_double), POINTER(c_double)]),
+    "bfm_server_job_result": (c_int, [_H, c_uint64, c_void_p, c_uint32]),
+    "bfm_server_job_error": (c_int, [_H, c_uint64, c_char_p, c_uint32]),
+    "bfm_server_cancel": (c_int, [_H, c_uint64]),
+    "bfm_server_release": (c_int, [_H, c_uint64]),
+    "bfm_server_info": (c_int, [_H, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint64),
+                                POINTER(c_uint64), POINTER(c_uint64)]),
     "bfm_device_count": (c_int, [POINTER(c_int)]),
+    "bfm_calibration_copy": (c_int, [c_int, c_uint64, c_int, POINTER(c_double)]),
     "bfm_gen_linear_melt": (c_int, [c_uint32, c_uint32, c_uint32, c_uint64, c_void_p, c_void_p]),
     "bfm_gen_diamond_network_size": (c_int, [c_uint32, c_uint32, c_uint32, c_uint32, c_uint32,
                                              POINTER(c_uint32), POINTER(c_uint32)]),
@@ -134,6 +159,13 @@ def check(rc: int) -> None:
         msg = load().bfm_last_error()
         text = msg.decode() if msg else ""
         raise _ERRORS.get(rc, BfmError)(rc, f"status {rc}: {text}")
+
+
+def calibration_copy(gpu: int, nbytes: int, repeats: int) -> float:
+    """Average milliseconds of one plain copy-kernel launch over nbytes."""
+    ms = c_double(0.0)
+    check(load().bfm_calibration_copy(gpu, nbytes, repeats, ctypes.byref(ms)))
+    return ms.value
 
 
 def device_count() -> int:

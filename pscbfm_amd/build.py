@@ -20,7 +20,10 @@ ARCH = os.environ.get("PSCBFM_OFFLOAD_ARCH", "gfx950")
 SOURCES = [
     "bfm_kernels.hip",
     "host_system.cpp",
+    "device_slab.cpp",
     "updater.cpp",
+    "batch.cpp",
+    "server.cpp",
     "capi.cpp",
     "generators.cpp",
 ]

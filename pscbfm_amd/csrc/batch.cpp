@@ -1,0 +1,319 @@
+// batch.cpp -- many systems swept together on one device.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+
+#include "device_slab.hpp"
+#include "updater.hpp"
+
+namespace pscbfm {
+
+BatchEngine::BatchEngine(int gpu) : mGpu(gpu)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e == hipErrorNoDevice || (e == hipSuccess && n == 0))
+        throw NoDeviceError("no HIP device is visible");
+    hip_check(e, "hipGetDeviceCount");
+    if (gpu < 0 || gpu >= n)
+        throw std::invalid_argument("batch: GPU id out of range");
+    hip_check(hipSetDevice(gpu), "hipSetDevice");
+    hip_check(hipStreamCreate(&mStream), "hipStreamCreate");
+}
+
+BatchEngine::~BatchEngine()
+{
+    hipSetDevice(mGpu);
+    if (mStream)
+        hipStreamSynchronize(mStream);
+    for (int b = 0; b < 2; ++b) {
+        if (mGraph[b])
+            hipGraphExecDestroy(mGraph[b]);
+        if (mDone[b])
+            hipEventDestroy(mDone[b]);
+        if (mHostSched[b])
+            hipHostFree(mHostSched[b]);
+        if (mDevSched[b])
+            hipFree(mDevSched[b]);
+    }
+    for (auto &m : mMembers)
+        if (m && m->slab)
+            hipFree(m->slab);
+    if (mDevSys)
+        hipFree(mDevSys);
+    if (mDevCheckBlk)
+        hipFree(mDevCheckBlk);
+    if (mDevPerformBlk)
+        hipFree(mDevPerformBlk);
+    if (mStream)
+        hipStreamDestroy(mStream);
+}
+
+size_t BatchEngine::bytesFor(const HostSystem &sys) { return slab_layout(sys).total; }
+
+uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age)
+{
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    auto m = std::make_unique<Member>();
+    m->sys = std::move(sys);
+    m->seed = seed;
+    m->age = age;
+    const SlabLayout L = slab_layout(m->sys);
+    hipError_t e = hipMalloc(&m->slab, L.total);
+    if (e != hipSuccess) {
+        m->slab = nullptr;
+        hip_check(e, "hipMalloc (batch member)");
+    }
+    m->bytes = L.total;
+    try {
+        m->dev.buf = upload_slab(m->sys, m->slab, L, seed, mStream);
+        // the host copies must outlive the asynchronous uploads
+        hip_check(hipStreamSynchronize(mStream), "hipStreamSynchronize (batch add)");
+    } catch (...) {
+        hipFree(m->slab);
+        throw;
+    }
+    m->dev.seg_begin = static_cast<const uint32_t *>(slab_part(m->slab, L, kPartSegBegin));
+    m->dev.seg_count = static_cast<const uint32_t *>(slab_part(m->slab, L, kPartSegCount));
+    m->check_blocks_max = check_blocks(std::max<uint32_t>(m->sys.max_segment, 1));
+    m->perform_blocks_max = (m->check_blocks_max + kPerformGroup - 1) / kPerformGroup;
+    mBytes += L.total;
+    mMembers.push_back(std::move(m));
+    return uint32_t(mMembers.size() - 1);
+}
+
+void BatchEngine::remove(uint32_t id)
+{
+    Member &m = *mMembers.at(id);
+    if (!m.slab)
+        return;
+    synchronize();
+    hipFree(m.slab);
+    m.slab = nullptr;
+    mBytes -= m.bytes;
+    mCompIds.clear();   // forces a rebuild of the tables
+}
+
+void BatchEngine::synchronize()
+{
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    hip_check(hipStreamSynchronize(mStream), "hipStreamSynchronize (batch)");
+}
+
+template <class T>
+static void ensure_device(T *&p, size_t &cap, size_t n, const char *what)
+{
+    if (n <= cap && p)
+        return;
+    if (p)
+        hip_check(hipFree(p), "hipFree");
+    p = nullptr;
+    cap = std::max<size_t>(n, 1);
+    hip_check(hipMalloc(&p, sizeof(T) * cap), what);
+}
+
+void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
+{
+    synchronize();   // tables may still be read by in-flight launches
+    std::vector<BatchSystem> sys;
+    std::vector<uint32_t> cblk, pblk;
+    mMaxColours = 0;
+    for (uint32_t j = 0; j < ids.size(); ++j) {
+        Member &m = *mMembers.at(ids[j]);
+        if (!m.slab)
+            throw std::invalid_argument("batch: member was removed");
+        BatchSystem d = m.dev;
+        d.check_base = uint32_t(cblk.size());
+        d.perform_base = uint32_t(pblk.size());
+        cblk.insert(cblk.end(), m.check_blocks_max, j);
+        pblk.insert(pblk.end(), m.perform_blocks_max, j);
+        sys.push_back(d);
+        mMaxColours = std::max(mMaxColours, m.sys.n_colours);
+    }
+    ensure_device(mDevSys, mSysCap, sys.size(), "hipMalloc (batch systems)");
+    ensure_device(mDevCheckBlk, mCheckCap, cblk.size(), "hipMalloc (batch blocks)");
+    ensure_device(mDevPerformBlk, mPerformCap, pblk.size(), "hipMalloc (batch blocks)");
+    hip_check(hipMemcpy(mDevSys, sys.data(), sizeof(BatchSystem) * sys.size(), hipMemcpyHostToDevice),
+              "upload batch systems");
+    hip_check(hipMemcpy(mDevCheckBlk, cblk.data(), sizeof(uint32_t) * cblk.size(), hipMemcpyHostToDevice),
+              "upload batch blocks");
+    hip_check(hipMemcpy(mDevPerformBlk, pblk.data(), sizeof(uint32_t) * pblk.size(), hipMemcpyHostToDevice),
+              "upload batch blocks");
+    mCheckBlocks = uint32_t(cblk.size());
+    mPerformBlocks = uint32_t(pblk.size());
+    mCompIds = ids;
+    ++mCompVersion;
+}
+
+void BatchEngine::runChunk(uint32_t mcs)
+{
+    const uint32_t n_sys = uint32_t(mCompIds.size());
+    const uint32_t slots = mcs * mMaxColours;
+    const size_t entries = size_t(slots) * n_sys;
+    const int b = mBuf;
+    mBuf ^= 1;
+
+    // the host buffer b was last read by the copy of chunk k-2
+    if (mDone[b])
+        hip_check(hipEventSynchronize(mDone[b]), "hipEventSynchronize (schedule)");
+    else
+        hip_check(hipEventCreateWithFlags(&mDone[b], hipEventDisableTiming), "hipEventCreate");
+    if (entries > mSchedCap[b]) {
+        if (mHostSched[b])
+            hip_check(hipHostFree(mHostSched[b]), "hipHostFree");
+        if (mDevSched[b])
+            hip_check(hipFree(mDevSched[b]), "hipFree");
+        mHostSched[b] = nullptr;
+        mDevSched[b] = nullptr;
+        hip_check(hipHostMalloc(&mHostSched[b], sizeof(SchedEntry) * entries, hipHostMallocDefault),
+                  "hipHostMalloc (schedule)");
+        hip_check(hipMalloc(&mDevSched[b], sizeof(SchedEntry) * entries), "hipMalloc (schedule)");
+        mSchedCap[b] = entries;
+        if (mGraph[b]) {
+            hip_check(hipGraphExecDestroy(mGraph[b]), "hipGraphExecDestroy");
+            mGraph[b] = nullptr;
+        }
+    }
+
+    // schedule: slot (m, k) of system j sweeps colour order[k] of MCS age+m
+    SchedEntry *sched = mHostSched[b];
+    std::vector<uint32_t> order(std::max<uint32_t>(mMaxColours, 1));
+    for (uint32_t j = 0; j < n_sys; ++j) {
+        Member &M = *mMembers[mCompIds[j]];
+        const uint32_t nc = M.sys.n_colours;
+        for (uint32_t m = 0; m < mcs; ++m) {
+            const uint64_t age = uint64_t(M.age) + m;
+            colour_order(M.seed, age, nc, order.data());
+            for (uint32_t k = 0; k < mMaxColours; ++k) {
+                SchedEntry &e = sched[size_t(m * mMaxColours + k) * n_sys + j];
+                e.pad = 0;
+                e.age = age;
+                if (k < nc) {
+                    const uint32_t c = order[k];
+                    e.colour = c;
+                    e.parity = uint32_t(M.substeps & 1u);
+                    e.prev_blocks = M.prev_blocks;
+                    ++M.substeps;
+                    M.prev_blocks = check_blocks(M.sys.seg_count[c]);
+                } else {
+                    e.colour = kNoColour;
+                    e.parity = 0;
+                    e.prev_blocks = 0;
+                }
+            }
+        }
+        M.age += mcs;
+    }
+    hip_check(hipMemcpyAsync(mDevSched[b], sched, sizeof(SchedEntry) * entries, hipMemcpyHostToDevice, mStream),
+              "upload schedule");
+
+    auto launch_all = [&]() {
+        for (uint32_t slot = 0; slot < slots; ++slot) {
+            hip_check(launch_check_batch(mDevSys, mDevCheckBlk, mCheckBlocks, mDevSched[b], n_sys, slot, mStream),
+                      "batched check kernel");
+            hip_check(launch_perform_batch(mDevSys, mDevPerformBlk, mPerformBlocks, mDevSched[b], n_sys, slot,
+                                           mStream),
+                      "batched perform kernel");
+        }
+    };
+    // capture a chunk only once its composition has run before: a server
+    // whose composition changes every round launches directly
+    const bool reuse = mRanVersion == mCompVersion && mRanMcs == mcs;
+    mRanVersion = mCompVersion;
+    mRanMcs = mcs;
+    const bool have = mGraph[b] && mGraphVersion[b] == mCompVersion && mGraphMcs[b] == mcs;
+    if (mUseGraphs && (have || reuse)) {
+        if (!have) {
+            if (mGraph[b]) {
+                hip_check(hipGraphExecDestroy(mGraph[b]), "hipGraphExecDestroy");
+                mGraph[b] = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            hip_check(hipStreamBeginCapture(mStream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                launch_all();
+            } catch (...) {
+                hipStreamEndCapture(mStream, &g);
+                if (g)
+                    hipGraphDestroy(g);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(mStream, &g), "hipStreamEndCapture");
+            hipError_t e = hipGraphInstantiate(&mGraph[b], g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            hip_check(e, "hipGraphInstantiate");
+            mGraphVersion[b] = mCompVersion;
+            mGraphMcs[b] = mcs;
+        }
+        hip_check(hipGraphLaunch(mGraph[b], mStream), "hipGraphLaunch");
+        ++mGraphLaunches;
+    } else {
+        launch_all();
+    }
+    mKernelLaunches += 2ull * slots;
+    hip_check(hipEventRecord(mDone[b], mStream), "hipEventRecord");
+}
+
+void BatchEngine::runMembers(const std::vector<uint32_t> &ids, uint32_t n_mcs, uint32_t chunk)
+{
+    if (ids.empty() || n_mcs == 0)
+        return;
+    if (chunk == 0)
+        throw std::invalid_argument("batch: chunk must be positive");
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    if (ids != mCompIds)
+        rebuildTables(ids);
+    for (uint32_t done = 0; done < n_mcs;) {
+        const uint32_t m = std::min(chunk, n_mcs - done);
+        runChunk(m);
+        done += m;
+    }
+}
+
+void BatchEngine::run(uint32_t n_mcs, uint32_t chunk)
+{
+    std::vector<uint32_t> ids;
+    for (uint32_t i = 0; i < mMembers.size(); ++i)
+        if (mMembers[i]->slab)
+            ids.push_back(i);
+    runMembers(ids, n_mcs, chunk);
+}
+
+void BatchEngine::copyBack(uint32_t id)
+{
+    Member &m = *mMembers.at(id);
+    if (!m.slab)
+        throw std::invalid_argument("batch: member was removed");
+    synchronize();
+    HostSystem &s = m.sys;
+    hip_check(hipMemcpy(s.pos_sorted.data(), m.dev.buf.pos, sizeof(int4) * s.n_pad, hipMemcpyDeviceToHost),
+              "download pos");
+    for (uint32_t i = 0; i < s.n; ++i) {
+        const uint32_t t = s.to_sorted[i];
+        for (int a = 0; a < 3; ++a)
+            s.xyz[3 * size_t(i) + a] = s.pos_sorted[4 * size_t(t) + a];
+    }
+}
+
+void BatchEngine::counters(uint32_t id, uint64_t &accepted, uint64_t &proposals)
+{
+    Member &m = *mMembers.at(id);
+    accepted = proposals = 0;
+    if (!m.slab)
+        return;
+    synchronize();
+    unsigned long long slots[2 * kStatSlots];
+    hip_check(hipMemcpy(slots, m.dev.buf.accepted, sizeof(unsigned long long) * kStatSlots, hipMemcpyDeviceToHost),
+              "download counters");
+    hip_check(hipMemcpy(slots + kStatSlots, m.dev.buf.proposals, sizeof(unsigned long long) * kStatSlots,
+                        hipMemcpyDeviceToHost),
+              "download counters");
+    for (uint32_t k = 0; k < kStatSlots; ++k) {
+        accepted += slots[k];
+        proposals += slots[kStatSlots + k];
+    }
+}
+
+}  // namespace pscbfm

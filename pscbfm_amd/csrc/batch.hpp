@@ -1,0 +1,111 @@
+// batch.hpp -- many independent systems swept together on one device.
+//
+// Every member keeps its own device slab (positions, partners, lattices,
+// survivor and collision lists: the same layout as one plain updater).  A
+// substep slot of a chunk is one k_check_batch and one k_perform_batch launch
+// over all members at once; which colour each member sweeps in each slot, its
+// parity, its previous check grid and its age come from a schedule the host
+// writes before the chunk.  The launches of a chunk are captured once into a
+// hipGraph and replayed for every chunk with the same composition, so the
+// whole chunk costs one graph launch on the host.
+//
+// A member's trajectory depends only on its own seed and age (the RNG is keyed
+// on seed, MCS and monomer id), so it is the same alone, in any batch and in
+// any chunking -- the plain updater's trajectory, bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "bfm_kernels.hpp"
+#include "host_system.hpp"
+
+namespace pscbfm {
+
+class BatchEngine {
+public:
+    explicit BatchEngine(int gpu);
+    ~BatchEngine();
+    BatchEngine(const BatchEngine &) = delete;
+    BatchEngine &operator=(const BatchEngine &) = delete;
+
+    // Takes a planned system (HostSystem after plan()); allocates and uploads
+    // its device slab at once.  Returns the member id.
+    uint32_t add(HostSystem &&sys, uint64_t seed, int64_t age);
+    // Device bytes a planned system would take as a member.
+    static size_t bytesFor(const HostSystem &sys);
+
+    // Runs n_mcs MCS on every active member, in chunks of `chunk` MCS.
+    void run(uint32_t n_mcs, uint32_t chunk);
+    // Runs n_mcs MCS on the members listed (others idle).
+    void runMembers(const std::vector<uint32_t> &ids, uint32_t n_mcs, uint32_t chunk);
+
+    // Copies a member's positions back into its HostSystem (unsorted, by id).
+    void copyBack(uint32_t id);
+    // Frees a member's device memory; its id is not reused.
+    void remove(uint32_t id);
+
+    const HostSystem &system(uint32_t id) const { return mMembers.at(id)->sys; }
+    int64_t age(uint32_t id) const { return mMembers.at(id)->age; }
+    bool active(uint32_t id) const { return mMembers.at(id)->slab != nullptr; }
+    size_t size() const { return mMembers.size(); }
+    size_t deviceBytes() const { return mBytes; }
+    void counters(uint32_t id, uint64_t &accepted, uint64_t &proposals);
+    hipStream_t stream() const { return mStream; }
+    void setUseGraphs(bool on) { mUseGraphs = on; }
+    uint64_t graphLaunches() const { return mGraphLaunches; }
+    uint64_t kernelLaunches() const { return mKernelLaunches; }
+    void synchronize();
+
+private:
+    struct Member {
+        HostSystem sys;
+        uint64_t seed = 0;
+        int64_t age = 0;
+        uint64_t substeps = 0;
+        uint32_t prev_blocks = 0;
+        void *slab = nullptr;
+        size_t bytes = 0;
+        BatchSystem dev{};
+        uint32_t check_blocks_max = 0;
+        uint32_t perform_blocks_max = 0;
+    };
+
+    void rebuildTables(const std::vector<uint32_t> &ids);
+    void runChunk(uint32_t mcs);
+
+    int mGpu;
+    hipStream_t mStream = nullptr;
+    std::vector<std::unique_ptr<Member>> mMembers;
+    size_t mBytes = 0;
+
+    // tables of the current composition
+    std::vector<uint32_t> mCompIds;          // member ids in launch order
+    BatchSystem *mDevSys = nullptr;
+    uint32_t *mDevCheckBlk = nullptr, *mDevPerformBlk = nullptr;
+    uint32_t mCheckBlocks = 0, mPerformBlocks = 0, mMaxColours = 0;
+    size_t mSysCap = 0, mCheckCap = 0, mPerformCap = 0;
+
+    // schedules of the last two chunks (double-buffered so that the host can
+    // write chunk k+1's schedule while chunk k runs)
+    int mBuf = 0;
+    SchedEntry *mHostSched[2] = {nullptr, nullptr};   // pinned
+    SchedEntry *mDevSched[2] = {nullptr, nullptr};
+    size_t mSchedCap[2] = {0, 0};
+    hipEvent_t mDone[2] = {nullptr, nullptr};
+
+    // captured chunks, one per schedule buffer
+    bool mUseGraphs = true;
+    hipGraphExec_t mGraph[2] = {nullptr, nullptr};
+    uint32_t mGraphMcs[2] = {0, 0};
+    uint64_t mGraphVersion[2] = {~0ull, ~0ull};
+    uint64_t mGraphLaunches = 0, mKernelLaunches = 0;
+    uint64_t mCompVersion = 0;
+    uint64_t mRanVersion = ~0ull;
+    uint32_t mRanMcs = 0;
+};
+
+}  // namespace pscbfm

@@ -139,7 +139,7 @@ __device__ inline void merge_words(uint32_t (&w)[N], uint32_t (&m)[N])
             }
 }
 
-__device__ inline bool bit_test(const uint32_t *words, uint32_t site)
+__device__ inline uint32_t bit_test(const uint32_t *words, uint32_t site)
 {
     return (words[site >> 5] >> (site & 31u)) & 1u;
 }

@@ -22,6 +22,8 @@
 //  * an accepted move flips its 8 lattice bits with one atomic XOR per word.
 #include "bfm_kernels.hpp"
 
+#include <algorithm>
+
 namespace pscbfm {
 namespace {
 
@@ -49,19 +51,25 @@ __device__ inline int4 moved(const int4 &r, int dir)
     return q;
 }
 
+struct CheckShared {
+    uint32_t tables[kTableWords];
+    uint32_t wave[kBlock / 64];
+    uint32_t coll;
+};
+
+// Phase 1 for check block `blk` of `nblk` covering the colour segment
+// [begin, begin+count).
 template <bool kAllPeriodic>
-__global__ __launch_bounds__(kBlock) void k_check(SweepBuffers buf, uint32_t begin, uint32_t count, uint64_t age,
-                                                  uint32_t parity, uint32_t prev_blocks)
+__device__ inline void check_body(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
+                                  uint32_t parity, uint32_t prev_blocks, uint32_t blk, uint32_t nblk,
+                                  CheckShared &sh)
 {
-    __shared__ uint32_t lds[kTableWords];
-    __shared__ uint32_t s_wave[kBlock / 64];
-    __shared__ uint32_t s_coll;
     if (threadIdx.x == 0)
-        s_coll = 0u;
-    load_tables(buf.tables, lds);
+        sh.coll = 0u;
+    load_tables(buf.tables, sh.tables);
     __syncthreads();
-    const uint32_t *dep = lds;
-    const uint32_t *bond = lds + 3 * 64;
+    const uint32_t *dep = sh.tables;
+    const uint32_t *bond = sh.tables + 3 * 64;
     const Geometry &g = buf.geom;
 
     // Collision bits listed by the previous check launch (other parity):
@@ -70,14 +78,14 @@ __global__ __launch_bounds__(kBlock) void k_check(SweepBuffers buf, uint32_t beg
         uint32_t *coll_prev = buf.coll[parity ^ 1u];
         const uint32_t *list_prev = buf.clist[parity ^ 1u];
         const uint32_t *cnt_prev = buf.ccount[parity ^ 1u];
-        for (uint32_t pb = blockIdx.x; pb < prev_blocks; pb += gridDim.x) {
+        for (uint32_t pb = blk; pb < prev_blocks; pb += nblk) {
             const uint32_t n = cnt_prev[pb];
             for (uint32_t k = threadIdx.x; k < n; k += kBlock)
                 coll_prev[list_prev[size_t(pb) * kCollSlots + k]] = 0u;
         }
     }
 
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t t = blk * kBlock + threadIdx.x;
     bool pass = false;
     uint32_t entry = 0;
     if (t < count) {
@@ -131,8 +139,8 @@ __global__ __launch_bounds__(kBlock) void k_check(SweepBuffers buf, uint32_t beg
                     const uint32_t hit = atomicOr(&buf.claim[w[k]], m[k]) & m[k];
                     if (hit) {
                         atomicOr(&buf.coll[parity][w[k]], hit);
-                        const uint32_t slot = atomicAdd(&s_coll, 1u);
-                        buf.clist[parity][size_t(blockIdx.x) * kCollSlots + slot] = w[k];
+                        const uint32_t slot = atomicAdd(&sh.coll, 1u);
+                        buf.clist[parity][size_t(blk) * kCollSlots + slot] = w[k];
                     }
                 }
             pass = true;
@@ -144,42 +152,58 @@ __global__ __launch_bounds__(kBlock) void k_check(SweepBuffers buf, uint32_t beg
     const uint64_t mball = __ballot(pass);
     const uint32_t wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63u) == 0)
-        s_wave[wave] = uint32_t(__popcll(mball));
+        sh.wave[wave] = uint32_t(__popcll(mball));
     __syncthreads();
     uint32_t base = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kBlock / 64; ++k)
-        base += k < wave ? s_wave[k] : 0u;
+        base += k < wave ? sh.wave[k] : 0u;
     if (pass)
-        buf.surv[size_t(blockIdx.x) * kBlock + base + lanes_below(mball)] = entry;
+        buf.surv[size_t(blk) * kBlock + base + lanes_below(mball)] = entry;
     if (threadIdx.x == 0) {
         uint32_t total = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kBlock / 64; ++k)
-            total += s_wave[k];
-        buf.scount[blockIdx.x] = total;
-        buf.ccount[parity][blockIdx.x] = s_coll;
+            total += sh.wave[k];
+        buf.scount[blk] = total;
+        buf.ccount[parity][blk] = sh.coll;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t check_blocks, uint32_t parity)
+template <bool kAllPeriodic>
+__global__ __launch_bounds__(kBlock) void k_check(SweepBuffers buf, uint32_t begin, uint32_t count, uint64_t age,
+                                                  uint32_t parity, uint32_t prev_blocks)
 {
-    __shared__ uint32_t lds[kTableWords];
-    __shared__ uint32_t s_pre[kPerformGroup + 1];
-    __shared__ uint32_t s_acc;
-    load_tables(buf.tables, lds);
+    __shared__ CheckShared sh;
+    check_body<kAllPeriodic>(buf, begin, count, age, parity, prev_blocks, blockIdx.x, gridDim.x, sh);
+}
+
+struct PerformShared {
+    uint32_t tables[kTableWords];
+    uint32_t pre[kPerformGroup + 1];
+    uint32_t acc;
+};
+
+// Phase 2 for perform block `blk`: the survivors of check blocks
+// [blk*kPerformGroup, (blk+1)*kPerformGroup) of a check launch of
+// check_blocks blocks.
+__device__ inline void perform_body(const SweepBuffers &buf, uint32_t check_blocks, uint32_t parity, uint32_t blk,
+                                    PerformShared &sh)
+{
+    uint32_t *s_pre = sh.pre;
+    load_tables(buf.tables, sh.tables);
     if (threadIdx.x == 0) {
         uint32_t acc = 0;
         s_pre[0] = 0;
         for (uint32_t k = 0; k < kPerformGroup; ++k) {
-            const uint32_t cb = blockIdx.x * kPerformGroup + k;
+            const uint32_t cb = blk * kPerformGroup + k;
             acc += cb < check_blocks ? buf.scount[cb] : 0u;
             s_pre[k + 1] = acc;
         }
-        s_acc = 0;
+        sh.acc = 0;
     }
     __syncthreads();
-    const uint32_t *dep = lds;
+    const uint32_t *dep = sh.tables;
     const Geometry &g = buf.geom;
     const uint32_t *coll = buf.coll[parity];
     const uint32_t total = s_pre[kPerformGroup];
@@ -189,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t c
 #pragma unroll
         for (uint32_t q = 1; q < kPerformGroup; ++q)
             k += t >= s_pre[q] ? 1u : 0u;
-        const uint32_t e = buf.surv[size_t(blockIdx.x * kPerformGroup + k) * kBlock + (t - s_pre[k])];
+        const uint32_t e = buf.surv[size_t(blk * kPerformGroup + k) * kBlock + (t - s_pre[k])];
         const uint32_t i = e & kSortedIndexMask;
         const int dir = int(e >> kSortedIndexBits);
         const int4 r0 = buf.pos[i];
@@ -236,16 +260,75 @@ __global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t c
         }
         const uint64_t acc = __ballot(!clash);
         if ((threadIdx.x & 63u) == uint32_t(__ffsll((unsigned long long)__ballot(1)) - 1))
-            atomicAdd(&s_acc, uint32_t(__popcll(acc)));
+            atomicAdd(&sh.acc, uint32_t(__popcll(acc)));
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const uint32_t slot = blockIdx.x & (kStatSlots - 1);
-        if (s_acc)
-            atomicAdd(&buf.accepted[slot], (unsigned long long)s_acc);
+        const uint32_t slot = blk & (kStatSlots - 1);
+        if (sh.acc)
+            atomicAdd(&buf.accepted[slot], (unsigned long long)sh.acc);
         if (total)
             atomicAdd(&buf.proposals[slot], (unsigned long long)total);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t check_blocks, uint32_t parity)
+{
+    __shared__ PerformShared sh;
+    perform_body(buf, check_blocks, parity, blockIdx.x, sh);
+}
+
+// ---- many systems per launch ------------------------------------------------
+// Block b of a batched launch belongs to system blk_sys[b]; the system's own
+// blocks start at its check_base / perform_base.  The schedule says which
+// colour (if any) each system sweeps in this substep slot.
+
+template <bool kAllPeriodic>
+__device__ inline void check_batch_body(const BatchSystem *__restrict__ sys, const uint32_t *__restrict__ blk_sys,
+                                        const SchedEntry *__restrict__ sched, uint32_t n_sys, uint32_t slot,
+                                        CheckShared &sh)
+{
+    const uint32_t s = blk_sys[blockIdx.x];
+    const SchedEntry e = sched[size_t(slot) * n_sys + s];
+    if (e.colour == kNoColour)
+        return;
+    const BatchSystem &S = sys[s];
+    const uint32_t count = S.seg_count[e.colour];
+    const uint32_t nb = check_blocks(count);
+    const uint32_t blk = blockIdx.x - S.check_base;
+    if (blk >= nb)
+        return;
+    check_body<kAllPeriodic>(S.buf, S.seg_begin[e.colour], count, e.age, e.parity, e.prev_blocks, blk, nb, sh);
+}
+
+__global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__restrict__ sys,
+                                                        const uint32_t *__restrict__ blk_sys,
+                                                        const SchedEntry *__restrict__ sched, uint32_t n_sys,
+                                                        uint32_t slot)
+{
+    __shared__ CheckShared sh;
+    if (sys[blk_sys[blockIdx.x]].buf.geom.periodic == 7u)
+        check_batch_body<true>(sys, blk_sys, sched, n_sys, slot, sh);
+    else
+        check_batch_body<false>(sys, blk_sys, sched, n_sys, slot, sh);
+}
+
+__global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__restrict__ sys,
+                                                          const uint32_t *__restrict__ blk_sys,
+                                                          const SchedEntry *__restrict__ sched, uint32_t n_sys,
+                                                          uint32_t slot)
+{
+    __shared__ PerformShared sh;
+    const uint32_t s = blk_sys[blockIdx.x];
+    const SchedEntry e = sched[size_t(slot) * n_sys + s];
+    if (e.colour == kNoColour)
+        return;
+    const BatchSystem &S = sys[s];
+    const uint32_t cb = check_blocks(S.seg_count[e.colour]);
+    const uint32_t blk = blockIdx.x - S.perform_base;
+    if (blk >= (cb + kPerformGroup - 1) / kPerformGroup)
+        return;
+    perform_body(S.buf, cb, e.parity, blk, sh);
 }
 
 }  // namespace
@@ -266,6 +349,44 @@ hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t pari
     const uint32_t cb = check_blocks(count);
     const uint32_t pb = (cb + kPerformGroup - 1) / kPerformGroup;
     hipLaunchKernelGGL(k_perform, dim3(pb), dim3(kBlock), 0, stream, buf, cb, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_batch(const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
+                              const SchedEntry *sched, uint32_t n_sys, uint32_t slot, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_check_batch, dim3(n_blocks), dim3(kBlock), 0, stream, sys, blk_sys, sched, n_sys, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_perform_batch(const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
+                                const SchedEntry *sched, uint32_t n_sys, uint32_t slot, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_perform_batch, dim3(n_blocks), dim3(kBlock), 0, stream, sys, blk_sys, sched, n_sys, slot);
+    return hipGetLastError();
+}
+
+}  // namespace pscbfm
+
+namespace pscbfm {
+namespace {
+
+// Plain streaming copy, 16 bytes per lane per iteration: the reference point
+// for HBM counters (a copy of B bytes reads B and writes B).
+__global__ __launch_bounds__(kBlock) void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock)
+        dst[i] = src[i];
+}
+
+}  // namespace
+
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream)
+{
+    const size_t n = bytes / sizeof(uint4);
+    const uint32_t blocks = uint32_t(std::min<size_t>((n + kBlock - 1) / kBlock, 256 * 64));
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(kBlock), 0, stream, static_cast<const uint4 *>(src),
+                       static_cast<uint4 *>(dst), n);
     return hipGetLastError();
 }
 

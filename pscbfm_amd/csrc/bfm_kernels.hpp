@@ -31,7 +31,7 @@ struct SweepBuffers {
     uint64_t seed;
 };
 
-inline uint32_t check_blocks(uint32_t count) { return count == 0 ? 1u : (count + kBlock - 1) / kBlock; }
+__host__ __device__ inline uint32_t check_blocks(uint32_t count) { return count == 0 ? 1u : (count + kBlock - 1) / kBlock; }
 
 // Phase 1 of a substep of parity `parity` for the colour segment
 // [begin, begin+count): every monomer proposes a move and checks it against
@@ -44,5 +44,38 @@ hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count,
 // Phase 2: every survivor whose face sites no other survivor claimed moves;
 // releases the claims.  count is the colour segment size of the check launch.
 hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t parity, hipStream_t stream);
+
+// ---- many systems per launch -------------------------------------------------
+
+constexpr uint32_t kNoColour = 0xFFFFFFFFu;
+
+// One system of a batch, as the batched kernels see it.
+struct BatchSystem {
+    SweepBuffers buf;
+    const uint32_t *seg_begin;   // per colour, sorted index of the first monomer
+    const uint32_t *seg_count;   // per colour
+    uint32_t check_base;         // first block of this system in a batched check launch
+    uint32_t perform_base;       // first block of this system in a batched perform launch
+};
+
+// What one system does in one substep slot of a chunk.
+struct SchedEntry {
+    uint64_t age;          // MCS that keys the RNG
+    uint32_t colour;       // kNoColour: no substep for this system in this slot
+    uint32_t parity;       // substep parity of this system
+    uint32_t prev_blocks;  // grid of this system's previous check launch
+    uint32_t pad;
+};
+
+// Phase 1 / phase 2 of one substep slot for every system of a batch.
+// n_blocks = sum over systems of their reserved blocks; blk_sys maps a block
+// to its system; sched holds n_sys entries per slot.
+hipError_t launch_check_batch(const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
+                              const SchedEntry *sched, uint32_t n_sys, uint32_t slot, hipStream_t stream);
+hipError_t launch_perform_batch(const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
+                                const SchedEntry *sched, uint32_t n_sys, uint32_t slot, hipStream_t stream);
+
+// Streaming copy of `bytes` (a multiple of 16) from src to dst.
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);
 
 }  // namespace pscbfm

@@ -1,14 +1,24 @@
 // capi.cpp -- extern "C" boundary: status codes and a thread-local message,
 // never an exception or exit() across the ABI.
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 
 #include "../../include/pscbfm_amd.h"
+#include "server.hpp"
 #include "updater.hpp"
 
 struct bfm_updater {
     pscbfm::UpdaterGPUScBFM_AB_Type u;
+};
+
+struct bfm_batch {
+    std::unique_ptr<pscbfm::BatchEngine> e;
+};
+
+struct bfm_server {
+    std::unique_ptr<pscbfm::Server> s;
 };
 
 namespace {
@@ -291,6 +301,290 @@ int bfm_set_kernel_timing(bfm_updater *h, int on)
 {
     NEED_HANDLE(h);
     return guarded([&] { h->u.setKernelTiming(on != 0); });
+}
+
+int bfm_batch_create(bfm_batch **out, int gpu)
+{
+    if (!out)
+        return fail(BFM_E_INVALID_ARGUMENT, "null output pointer");
+    return guarded([&] {
+        auto b = std::make_unique<bfm_batch>();
+        b->e = std::make_unique<pscbfm::BatchEngine>(gpu);
+        *out = b.release();
+    });
+}
+
+int bfm_batch_destroy(bfm_batch *b)
+{
+    delete b;
+    return BFM_OK;
+}
+
+#define NEED(p, what)                                              \
+    do {                                                           \
+        if (!(p))                                                  \
+            return fail(BFM_E_INVALID_ARGUMENT, "null " what);    \
+    } while (0)
+
+int bfm_member_bytes(bfm_updater *h, uint64_t *bytes)
+{
+    NEED(h, "updater handle");
+    NEED(bytes, "output pointer");
+    return guarded([&] {
+        if (!h->u.validated())
+            h->u.validate();
+        *bytes = pscbfm::BatchEngine::bytesFor(h->u.system());
+    });
+}
+
+int bfm_batch_add(bfm_batch *b, bfm_updater *h, uint32_t *member)
+{
+    NEED(b, "batch handle");
+    NEED(h, "updater handle");
+    NEED(member, "output pointer");
+    return guarded([&] {
+        if (!h->u.validated())
+            h->u.validate();
+        *member = b->e->add(pscbfm::HostSystem(h->u.system()), h->u.seed(), h->u.getAge());
+    });
+}
+
+int bfm_batch_run(bfm_batch *b, uint32_t n_mcs, uint32_t chunk)
+{
+    NEED(b, "batch handle");
+    return guarded([&] { b->e->run(n_mcs, chunk); });
+}
+
+int bfm_batch_run_members(bfm_batch *b, const uint32_t *members, uint32_t n_members, uint32_t n_mcs,
+                          uint32_t chunk)
+{
+    NEED(b, "batch handle");
+    if (!members && n_members)
+        return fail(BFM_E_INVALID_ARGUMENT, "null member list");
+    return guarded([&] {
+        std::vector<uint32_t> ids(members, members + n_members);
+        for (uint32_t id : ids)
+            if (id >= b->e->size() || !b->e->active(id))
+                throw std::invalid_argument("batch: no such active member");
+        b->e->runMembers(ids, n_mcs, chunk);
+    });
+}
+
+int bfm_batch_get_positions(bfm_batch *b, uint32_t member, int32_t *xyz, uint32_t n)
+{
+    NEED(b, "batch handle");
+    NEED(xyz, "coordinate array");
+    return guarded([&] {
+        if (member >= b->e->size())
+            throw std::invalid_argument("batch: no such member");
+        b->e->copyBack(member);
+        const auto &s = b->e->system(member);
+        if (n != s.n)
+            throw std::invalid_argument("bfm_batch_get_positions: n must equal the member's monomer count");
+        std::memcpy(xyz, s.xyz.data(), sizeof(int32_t) * 3 * size_t(n));
+    });
+}
+
+int bfm_batch_get_age(bfm_batch *b, uint32_t member, int64_t *age)
+{
+    NEED(b, "batch handle");
+    NEED(age, "output pointer");
+    return guarded([&] {
+        if (member >= b->e->size())
+            throw std::invalid_argument("batch: no such member");
+        *age = b->e->age(member);
+    });
+}
+
+int bfm_batch_get_stats(bfm_batch *b, uint32_t member, bfm_stats *out)
+{
+    NEED(b, "batch handle");
+    NEED(out, "output pointer");
+    return guarded([&] {
+        if (member >= b->e->size())
+            throw std::invalid_argument("batch: no such member");
+        std::memset(out, 0, sizeof *out);
+        uint64_t acc = 0, prop = 0;
+        b->e->counters(member, acc, prop);
+        out->accepted = acc;
+        out->proposals = prop;
+    });
+}
+
+int bfm_batch_remove(bfm_batch *b, uint32_t member)
+{
+    NEED(b, "batch handle");
+    return guarded([&] {
+        if (member >= b->e->size())
+            throw std::invalid_argument("batch: no such member");
+        b->e->remove(member);
+    });
+}
+
+int bfm_batch_set_graphs(bfm_batch *b, int on)
+{
+    NEED(b, "batch handle");
+    return guarded([&] { b->e->setUseGraphs(on != 0); });
+}
+
+int bfm_batch_synchronize(bfm_batch *b)
+{
+    NEED(b, "batch handle");
+    return guarded([&] { b->e->synchronize(); });
+}
+
+int bfm_server_create(bfm_server **out, int gpu, uint64_t hbm_budget, uint32_t chunk_mcs)
+{
+    NEED(out, "output pointer");
+    return guarded([&] {
+        auto s = std::make_unique<bfm_server>();
+        s->s = std::make_unique<pscbfm::Server>(gpu, hbm_budget, chunk_mcs);
+        *out = s.release();
+    });
+}
+
+int bfm_server_destroy(bfm_server *s)
+{
+    delete s;
+    return BFM_OK;
+}
+
+int bfm_server_submit(bfm_server *s, bfm_updater *h, uint32_t n_mcs, uint64_t *job)
+{
+    NEED(s, "server handle");
+    NEED(h, "updater handle");
+    NEED(job, "output pointer");
+    return guarded([&] {
+        if (!h->u.validated())
+            h->u.validate();
+        *job = s->s->submit(h->u.system(), h->u.seed(), h->u.getAge(), n_mcs);
+    });
+}
+
+int bfm_server_step(bfm_server *s, uint32_t *pending)
+{
+    NEED(s, "server handle");
+    return guarded([&] {
+        const uint32_t p = s->s->step();
+        if (pending)
+            *pending = p;
+    });
+}
+
+int bfm_server_job_state(bfm_server *s, uint64_t job, int *state, uint32_t *mcs_done)
+{
+    NEED(s, "server handle");
+    return guarded([&] {
+        if (state)
+            *state = int(s->s->state(job));
+        if (mcs_done)
+            *mcs_done = s->s->mcsDone(job);
+    });
+}
+
+int bfm_server_job_times(bfm_server *s, uint64_t job, double *submitted, double *admitted, double *finished)
+{
+    NEED(s, "server handle");
+    NEED(submitted, "output pointer");
+    NEED(admitted, "output pointer");
+    NEED(finished, "output pointer");
+    return guarded([&] { s->s->times(job, *submitted, *admitted, *finished); });
+}
+
+int bfm_server_job_result(bfm_server *s, uint64_t job, int32_t *xyz, uint32_t n)
+{
+    NEED(s, "server handle");
+    NEED(xyz, "coordinate array");
+    return guarded([&] {
+        const auto &sys = s->s->result(job);
+        if (n != sys.n)
+            throw std::invalid_argument("bfm_server_job_result: n must equal the job's monomer count");
+        std::memcpy(xyz, sys.xyz.data(), sizeof(int32_t) * 3 * size_t(n));
+    });
+}
+
+int bfm_server_job_error(bfm_server *s, uint64_t job, char *buf, uint32_t len)
+{
+    NEED(s, "server handle");
+    NEED(buf, "buffer");
+    return guarded([&] {
+        const std::string &e = s->s->error(job);
+        if (len) {
+            const size_t k = std::min<size_t>(len - 1, e.size());
+            std::memcpy(buf, e.data(), k);
+            buf[k] = 0;
+        }
+    });
+}
+
+int bfm_server_cancel(bfm_server *s, uint64_t job)
+{
+    NEED(s, "server handle");
+    return guarded([&] { s->s->cancel(job); });
+}
+
+int bfm_server_release(bfm_server *s, uint64_t job)
+{
+    NEED(s, "server handle");
+    return guarded([&] { s->s->release(job); });
+}
+
+int bfm_server_info(bfm_server *s, uint32_t *queued, uint32_t *running, uint64_t *bytes_in_use, uint64_t *budget,
+                    uint64_t *attempts)
+{
+    NEED(s, "server handle");
+    return guarded([&] {
+        if (queued)
+            *queued = s->s->queued();
+        if (running)
+            *running = s->s->running();
+        if (bytes_in_use)
+            *bytes_in_use = s->s->bytesInUse();
+        if (budget)
+            *budget = s->s->budget();
+        if (attempts)
+            *attempts = s->s->attemptsDone();
+    });
+}
+
+int bfm_calibration_copy(int gpu, uint64_t bytes, int repeats, double *avg_ms)
+{
+    if (!avg_ms || repeats <= 0 || bytes == 0 || bytes % 16)
+        return fail(BFM_E_INVALID_ARGUMENT, "calibration copy: bytes must be a positive multiple of 16");
+    return guarded([&] {
+        pscbfm::hip_check(hipSetDevice(gpu), "hipSetDevice");
+        void *src = nullptr, *dst = nullptr;
+        hipStream_t st = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto release = [&] {
+            if (e0) hipEventDestroy(e0);
+            if (e1) hipEventDestroy(e1);
+            if (st) hipStreamDestroy(st);
+            if (src) hipFree(src);
+            if (dst) hipFree(dst);
+        };
+        try {
+            pscbfm::hip_check(hipMalloc(&src, bytes), "hipMalloc");
+            pscbfm::hip_check(hipMalloc(&dst, bytes), "hipMalloc");
+            pscbfm::hip_check(hipStreamCreate(&st), "hipStreamCreate");
+            pscbfm::hip_check(hipMemsetAsync(src, 1, bytes, st), "hipMemsetAsync");
+            pscbfm::hip_check(hipEventCreate(&e0), "hipEventCreate");
+            pscbfm::hip_check(hipEventCreate(&e1), "hipEventCreate");
+            pscbfm::hip_check(pscbfm::launch_copy(src, dst, bytes, st), "copy kernel");  // warm-up
+            pscbfm::hip_check(hipEventRecord(e0, st), "hipEventRecord");
+            for (int r = 0; r < repeats; ++r)
+                pscbfm::hip_check(pscbfm::launch_copy(src, dst, bytes, st), "copy kernel");
+            pscbfm::hip_check(hipEventRecord(e1, st), "hipEventRecord");
+            pscbfm::hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0.f;
+            pscbfm::hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            *avg_ms = double(ms) / repeats;
+        } catch (...) {
+            release();
+            throw;
+        }
+        release();
+    });
 }
 
 int bfm_device_count(int *count)

@@ -1,0 +1,78 @@
+// device_slab.cpp -- layout and upload of one system's device arrays.
+#include "device_slab.hpp"
+
+#include <algorithm>
+
+#include "updater.hpp"
+
+namespace pscbfm {
+
+SlabLayout slab_layout(const HostSystem &s)
+{
+    const size_t blocks = check_blocks(std::max<uint32_t>(s.max_segment, 1));
+    SlabLayout L{};
+    L.bytes[kPartPos] = sizeof(int4) * s.n_pad;
+    L.bytes[kPartNbr] = sizeof(uint32_t) * kMaxConnectivity * s.n_pad;
+    L.bytes[kPartNnbr] = size_t(s.n_pad);
+    L.bytes[kPartLattice] = sizeof(uint32_t) * s.lattice_words;
+    L.bytes[kPartClaim] = sizeof(uint32_t) * s.lattice_words;
+    L.bytes[kPartColl0] = sizeof(uint32_t) * s.lattice_words;
+    L.bytes[kPartColl1] = sizeof(uint32_t) * s.lattice_words;
+    L.bytes[kPartClist0] = sizeof(uint32_t) * kCollSlots * blocks;
+    L.bytes[kPartClist1] = sizeof(uint32_t) * kCollSlots * blocks;
+    L.bytes[kPartCcount0] = sizeof(uint32_t) * blocks;
+    L.bytes[kPartCcount1] = sizeof(uint32_t) * blocks;
+    L.bytes[kPartSurv] = sizeof(uint32_t) * kBlock * blocks;
+    L.bytes[kPartScount] = sizeof(uint32_t) * blocks;
+    L.bytes[kPartAccepted] = sizeof(unsigned long long) * kStatSlots;
+    L.bytes[kPartProposals] = sizeof(unsigned long long) * kStatSlots;
+    L.bytes[kPartTables] = sizeof(Tables);
+    L.bytes[kPartSegBegin] = sizeof(uint32_t) * std::max<uint32_t>(s.n_colours, 1);
+    L.bytes[kPartSegCount] = sizeof(uint32_t) * std::max<uint32_t>(s.n_colours, 1);
+    size_t total = 0;
+    for (int p = 0; p < kPartCount; ++p) {
+        L.offset[p] = total;
+        total += (L.bytes[p] + 255) / 256 * 256;
+    }
+    L.total = total;
+    return L;
+}
+
+SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, uint64_t seed, hipStream_t stream)
+{
+    auto up = [&](SlabPart p, const void *src, size_t bytes, const char *what) {
+        hip_check(hipMemcpyAsync(slab_part(slab, L, p), src, bytes, hipMemcpyHostToDevice, stream), what);
+    };
+    hip_check(hipMemsetAsync(slab, 0, L.total, stream), "hipMemsetAsync (slab)");
+    up(kPartPos, s.pos_sorted.data(), L.bytes[kPartPos], "upload pos");
+    up(kPartNbr, s.nbr_sorted.data(), L.bytes[kPartNbr], "upload nbr");
+    up(kPartNnbr, s.nnbr_sorted.data(), L.bytes[kPartNnbr], "upload nnbr");
+    up(kPartLattice, s.lattice.data(), L.bytes[kPartLattice], "upload lattice");
+    up(kPartTables, &s.tables, sizeof(Tables), "upload tables");
+    up(kPartSegBegin, s.seg_begin.data(), sizeof(uint32_t) * s.n_colours, "upload segments");
+    up(kPartSegCount, s.seg_count.data(), sizeof(uint32_t) * s.n_colours, "upload segments");
+
+    SweepBuffers b{};
+    b.pos = static_cast<int4 *>(slab_part(slab, L, kPartPos));
+    b.nbr = static_cast<const uint32_t *>(slab_part(slab, L, kPartNbr));
+    b.nnbr = static_cast<const uint8_t *>(slab_part(slab, L, kPartNnbr));
+    b.n_pad = s.n_pad;
+    b.lattice = static_cast<uint32_t *>(slab_part(slab, L, kPartLattice));
+    b.claim = static_cast<uint32_t *>(slab_part(slab, L, kPartClaim));
+    b.coll[0] = static_cast<uint32_t *>(slab_part(slab, L, kPartColl0));
+    b.coll[1] = static_cast<uint32_t *>(slab_part(slab, L, kPartColl1));
+    b.clist[0] = static_cast<uint32_t *>(slab_part(slab, L, kPartClist0));
+    b.clist[1] = static_cast<uint32_t *>(slab_part(slab, L, kPartClist1));
+    b.ccount[0] = static_cast<uint32_t *>(slab_part(slab, L, kPartCcount0));
+    b.ccount[1] = static_cast<uint32_t *>(slab_part(slab, L, kPartCcount1));
+    b.surv = static_cast<uint32_t *>(slab_part(slab, L, kPartSurv));
+    b.scount = static_cast<uint32_t *>(slab_part(slab, L, kPartScount));
+    b.accepted = static_cast<unsigned long long *>(slab_part(slab, L, kPartAccepted));
+    b.proposals = static_cast<unsigned long long *>(slab_part(slab, L, kPartProposals));
+    b.tables = static_cast<const Tables *>(slab_part(slab, L, kPartTables));
+    b.geom = s.geom;
+    b.seed = seed;
+    return b;
+}
+
+}  // namespace pscbfm

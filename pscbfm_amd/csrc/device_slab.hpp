@@ -1,0 +1,56 @@
+// device_slab.hpp -- one system's device arrays in a single allocation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "bfm_kernels.hpp"
+#include "host_system.hpp"
+
+namespace pscbfm {
+
+enum SlabPart {
+    kPartPos,
+    kPartNbr,
+    kPartNnbr,
+    kPartLattice,
+    kPartClaim,
+    kPartColl0,
+    kPartColl1,
+    kPartClist0,
+    kPartClist1,
+    kPartCcount0,
+    kPartCcount1,
+    kPartSurv,
+    kPartScount,
+    kPartAccepted,
+    kPartProposals,
+    kPartTables,
+    kPartSegBegin,
+    kPartSegCount,
+    kPartCount
+};
+
+struct SlabLayout {
+    size_t offset[kPartCount];
+    size_t bytes[kPartCount];
+    size_t total;
+};
+
+// Offsets (256-byte aligned) of every array of a planned system.
+SlabLayout slab_layout(const HostSystem &s);
+
+// Zeroes the slab, uploads positions, partners, lattice, tables and colour
+// segments on `stream` (asynchronously: the HostSystem must stay alive until
+// the stream has passed this point) and returns the kernels' view of it.
+SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, uint64_t seed, hipStream_t stream);
+
+// Pointer to one part of the slab.
+inline void *slab_part(void *slab, const SlabLayout &L, SlabPart p)
+{
+    return static_cast<char *>(slab) + L.offset[p];
+}
+
+}  // namespace pscbfm

@@ -1,0 +1,178 @@
+// server.cpp -- continuous batching of simulation jobs on one device.
+#include "server.hpp"
+
+#include <algorithm>
+#include <sstream>
+#include <stdexcept>
+
+#include "updater.hpp"
+
+namespace pscbfm {
+
+Server::Server(int gpu, uint64_t hbm_budget, uint32_t chunk)
+    : mEngine(gpu), mBudget(hbm_budget), mChunk(chunk), mT0(std::chrono::steady_clock::now())
+{
+    if (chunk == 0)
+        throw std::invalid_argument("server: chunk must be positive");
+    if (mBudget == 0) {
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        mBudget = uint64_t(double(free_b) * 0.8);
+    }
+}
+
+double Server::now() const
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - mT0).count();
+}
+
+Server::Job &Server::job(uint64_t id)
+{
+    auto it = mJobs.find(id);
+    if (it == mJobs.end())
+        throw std::invalid_argument("server: unknown job id");
+    return *it->second;
+}
+
+const Server::Job &Server::job(uint64_t id) const
+{
+    auto it = mJobs.find(id);
+    if (it == mJobs.end())
+        throw std::invalid_argument("server: unknown job id");
+    return *it->second;
+}
+
+uint64_t Server::submit(const HostSystem &sys, uint64_t seed, int64_t age, uint32_t n_mcs)
+{
+    auto j = std::make_unique<Job>();
+    j->sys = sys;
+    j->seed = seed;
+    j->age = age;
+    j->n_mcs = n_mcs;
+    j->bytes = BatchEngine::bytesFor(j->sys);
+    j->t_submit = now();
+    const uint64_t id = mNext++;
+    if (j->bytes > mBudget) {
+        std::ostringstream msg;
+        msg << "job needs " << j->bytes << " device bytes, more than the whole budget of " << mBudget;
+        j->state = kFailed;
+        j->error = msg.str();
+        j->t_finish = j->t_submit;
+    } else if (n_mcs == 0) {
+        j->state = kDone;
+        j->t_admit = j->t_finish = j->t_submit;
+    } else {
+        mQueue.push_back(id);
+    }
+    mJobs[id] = std::move(j);
+    return id;
+}
+
+uint32_t Server::step()
+{
+    // admission in arrival order while the budget allows (no overtaking, so a
+    // large job is not starved by small ones behind it)
+    while (!mQueue.empty()) {
+        Job &j = job(mQueue.front());
+        if (mInUse + j.bytes > mBudget)
+            break;
+        try {
+            j.member = mEngine.add(HostSystem(j.sys), j.seed, j.age);
+            j.state = kRunning;
+            j.t_admit = now();
+            mInUse += j.bytes;
+            mRunning.push_back(mQueue.front());
+        } catch (const std::exception &e) {
+            j.state = kFailed;
+            j.error = e.what();
+            j.t_finish = now();
+        }
+        mQueue.pop_front();
+    }
+    if (mRunning.empty())
+        return uint32_t(mQueue.size());
+
+    // jobs with less than a chunk left run their remainder first
+    std::vector<uint64_t> full;
+    std::map<uint32_t, std::vector<uint32_t>> partial;
+    std::vector<uint32_t> members;
+    for (uint64_t id : mRunning) {
+        Job &j = job(id);
+        const uint32_t left = j.n_mcs - j.done;
+        if (left >= mChunk)
+            members.push_back(j.member);
+        else
+            partial[left].push_back(j.member);
+    }
+    for (auto &kv : partial)
+        mEngine.runMembers(kv.second, kv.first, kv.first);
+    if (!members.empty())
+        mEngine.runMembers(members, mChunk, mChunk);
+
+    std::vector<uint64_t> still;
+    for (uint64_t id : mRunning) {
+        Job &j = job(id);
+        const uint32_t ran = std::min(mChunk, j.n_mcs - j.done);
+        j.done += ran;
+        mAttempts += uint64_t(ran) * j.sys.n;
+        if (j.done >= j.n_mcs) {
+            mEngine.copyBack(j.member);
+            j.sys.xyz = mEngine.system(j.member).xyz;
+            j.age = mEngine.age(j.member);
+            mEngine.remove(j.member);
+            mInUse -= j.bytes;
+            j.state = kDone;
+            j.t_finish = now();
+        } else {
+            still.push_back(id);
+        }
+    }
+    mRunning.swap(still);
+    return uint32_t(mQueue.size() + mRunning.size());
+}
+
+void Server::cancel(uint64_t id)
+{
+    Job &j = job(id);
+    if (j.state == kQueued) {
+        mQueue.erase(std::find(mQueue.begin(), mQueue.end(), id));
+    } else if (j.state == kRunning) {
+        mEngine.remove(j.member);
+        mInUse -= j.bytes;
+        mRunning.erase(std::find(mRunning.begin(), mRunning.end(), id));
+    } else {
+        return;
+    }
+    j.state = kCancelled;
+    j.t_finish = now();
+}
+
+void Server::release(uint64_t id)
+{
+    Job &j = job(id);
+    if (j.state == kQueued || j.state == kRunning)
+        throw std::invalid_argument("server: release of a job that has not finished");
+    mJobs.erase(id);
+}
+
+Server::State Server::state(uint64_t id) const { return job(id).state; }
+uint32_t Server::mcsDone(uint64_t id) const { return job(id).done; }
+const std::string &Server::error(uint64_t id) const { return job(id).error; }
+
+const HostSystem &Server::result(uint64_t id) const
+{
+    const Job &j = job(id);
+    if (j.state != kDone)
+        throw std::invalid_argument("server: the job has no result (not done)");
+    return j.sys;
+}
+
+void Server::times(uint64_t id, double &submitted, double &admitted, double &finished) const
+{
+    const Job &j = job(id);
+    submitted = j.t_submit;
+    admitted = j.t_admit;
+    finished = j.t_finish;
+}
+
+}  // namespace pscbfm

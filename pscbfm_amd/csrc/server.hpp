@@ -1,0 +1,81 @@
+// server.hpp -- served simulation jobs with continuous batching.
+//
+// Jobs (a validated system plus the number of MCS it asks for) queue in
+// arrival order.  Each scheduling round admits queued jobs while their device
+// bytes fit the HBM budget, runs one chunk of MCS on every admitted job in one
+// batched launch sequence (their work overlaps on the device), and retires the
+// jobs that are done: their positions are copied back and their device memory
+// is freed, which lets the next round admit more.  A job's state stays
+// resident on the device between chunks.  A job's result is bit-identical to
+// running it alone (its trajectory depends only on its seed and age).
+#pragma once
+
+#include <chrono>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "batch.hpp"
+
+namespace pscbfm {
+
+class Server {
+public:
+    enum State { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3, kCancelled = 4 };
+
+    Server(int gpu, uint64_t hbm_budget, uint32_t chunk);
+
+    uint64_t submit(const HostSystem &sys, uint64_t seed, int64_t age, uint32_t n_mcs);
+    // One round; returns the number of jobs still queued or running.
+    uint32_t step();
+    void cancel(uint64_t job);
+    void release(uint64_t job);
+
+    State state(uint64_t job) const;
+    uint32_t mcsDone(uint64_t job) const;
+    const std::string &error(uint64_t job) const;
+    const HostSystem &result(uint64_t job) const;
+    // seconds since the server was created: submission, admission, completion
+    void times(uint64_t job, double &submitted, double &admitted, double &finished) const;
+
+    uint32_t queued() const { return uint32_t(mQueue.size()); }
+    uint32_t running() const { return uint32_t(mRunning.size()); }
+    uint64_t budget() const { return mBudget; }
+    uint64_t bytesInUse() const { return mInUse; }
+    uint64_t attemptsDone() const { return mAttempts; }
+    BatchEngine &engine() { return mEngine; }
+
+private:
+    struct Job {
+        HostSystem sys;
+        uint64_t seed = 0;
+        int64_t age = 0;
+        uint32_t n_mcs = 0;
+        uint32_t done = 0;
+        uint64_t bytes = 0;
+        State state = kQueued;
+        uint32_t member = 0;
+        std::string error;
+        double t_submit = 0, t_admit = -1, t_finish = -1;
+    };
+
+    double now() const;
+    Job &job(uint64_t id);
+    const Job &job(uint64_t id) const;
+
+    BatchEngine mEngine;
+    uint64_t mBudget;
+    uint32_t mChunk;
+    uint64_t mInUse = 0;
+    uint64_t mNext = 1;
+    uint64_t mAttempts = 0;
+    std::map<uint64_t, std::unique_ptr<Job>> mJobs;
+    std::deque<uint64_t> mQueue;
+    std::vector<uint64_t> mRunning;
+    std::chrono::steady_clock::time_point mT0;
+};
+
+}  // namespace pscbfm

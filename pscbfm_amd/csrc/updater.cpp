@@ -1,6 +1,8 @@
 // updater.cpp -- host orchestration of the colour sweep on one device.
 #include "updater.hpp"
 
+#include "device_slab.hpp"
+
 #include <algorithm>
 #include <cstring>
 #include <sstream>
@@ -174,80 +176,21 @@ void UpdaterGPUScBFM_AB_Type::initialize()
         hip_check(hipSetDevice(mGpu), "hipSetDevice");
 
     const HostSystem &s = mSys;
-    const size_t blocks = check_blocks(std::max<uint32_t>(s.max_segment, 1));
-    // one slab, every array 256-byte aligned
-    struct Part {
-        size_t bytes;
-        size_t offset;
-    };
-    auto round = [](size_t b) { return (b + 255) / 256 * 256; };
-    std::vector<Part> parts = {
-        {sizeof(int4) * s.n_pad, 0},                              // 0 pos
-        {sizeof(uint32_t) * kMaxConnectivity * s.n_pad, 0},       // 1 nbr
-        {size_t(s.n_pad), 0},                                     // 2 nnbr
-        {sizeof(uint32_t) * s.lattice_words, 0},                  // 3 lattice
-        {sizeof(uint32_t) * s.lattice_words, 0},                  // 4 claim
-        {sizeof(uint32_t) * s.lattice_words, 0},                  // 5 coll0
-        {sizeof(uint32_t) * s.lattice_words, 0},                  // 6 coll1
-        {sizeof(uint32_t) * kCollSlots * blocks, 0},              // 7 clist0
-        {sizeof(uint32_t) * kCollSlots * blocks, 0},              // 8 clist1
-        {sizeof(uint32_t) * blocks, 0},                           // 9 ccount0
-        {sizeof(uint32_t) * blocks, 0},                           // 10 ccount1
-        {sizeof(uint32_t) * kBlock * blocks, 0},                  // 11 surv
-        {sizeof(uint32_t) * blocks, 0},                           // 12 scount
-        {sizeof(unsigned long long) * kStatSlots, 0},             // 13 accepted
-        {sizeof(unsigned long long) * kStatSlots, 0},             // 14 proposals
-        {sizeof(Tables), 0},                                      // 15 tables
-    };
-    size_t total = 0;
-    for (auto &p : parts) {
-        p.offset = total;
-        total += round(p.bytes);
-    }
-    hipError_t e = hipMalloc(&mDevSlab, total);
+    const SlabLayout L = slab_layout(s);
+    hipError_t e = hipMalloc(&mDevSlab, L.total);
     if (e != hipSuccess) {
         mDevSlab = nullptr;
         hip_check(e, "hipMalloc (system slab)");
     }
-    mDevBytes = total;
-    auto ptr = [&](int k) { return static_cast<char *>(mDevSlab) + parts[k].offset; };
+    mDevBytes = L.total;
     try {
         hip_check(hipStreamCreate(&mStream), "hipStreamCreate");
-        hip_check(hipMemsetAsync(mDevSlab, 0, total, mStream), "hipMemsetAsync");
-        hip_check(hipMemcpyAsync(ptr(0), s.pos_sorted.data(), parts[0].bytes, hipMemcpyHostToDevice, mStream),
-                  "upload pos");
-        hip_check(hipMemcpyAsync(ptr(1), s.nbr_sorted.data(), parts[1].bytes, hipMemcpyHostToDevice, mStream),
-                  "upload nbr");
-        hip_check(hipMemcpyAsync(ptr(2), s.nnbr_sorted.data(), parts[2].bytes, hipMemcpyHostToDevice, mStream),
-                  "upload nnbr");
-        hip_check(hipMemcpyAsync(ptr(3), s.lattice.data(), parts[3].bytes, hipMemcpyHostToDevice, mStream),
-                  "upload lattice");
-        hip_check(hipMemcpyAsync(ptr(15), &s.tables, sizeof(Tables), hipMemcpyHostToDevice, mStream),
-                  "upload tables");
+        mBuf = upload_slab(s, mDevSlab, L, mSeed, mStream);
         hip_check(hipStreamSynchronize(mStream), "hipStreamSynchronize (initialize)");
     } catch (...) {
         freeDevice();
         throw;
     }
-    mBuf.pos = reinterpret_cast<int4 *>(ptr(0));
-    mBuf.nbr = reinterpret_cast<const uint32_t *>(ptr(1));
-    mBuf.nnbr = reinterpret_cast<const uint8_t *>(ptr(2));
-    mBuf.n_pad = s.n_pad;
-    mBuf.lattice = reinterpret_cast<uint32_t *>(ptr(3));
-    mBuf.claim = reinterpret_cast<uint32_t *>(ptr(4));
-    mBuf.coll[0] = reinterpret_cast<uint32_t *>(ptr(5));
-    mBuf.coll[1] = reinterpret_cast<uint32_t *>(ptr(6));
-    mBuf.clist[0] = reinterpret_cast<uint32_t *>(ptr(7));
-    mBuf.clist[1] = reinterpret_cast<uint32_t *>(ptr(8));
-    mBuf.ccount[0] = reinterpret_cast<uint32_t *>(ptr(9));
-    mBuf.ccount[1] = reinterpret_cast<uint32_t *>(ptr(10));
-    mBuf.surv = reinterpret_cast<uint32_t *>(ptr(11));
-    mBuf.scount = reinterpret_cast<uint32_t *>(ptr(12));
-    mBuf.accepted = reinterpret_cast<unsigned long long *>(ptr(13));
-    mBuf.proposals = reinterpret_cast<unsigned long long *>(ptr(14));
-    mBuf.tables = reinterpret_cast<const Tables *>(ptr(15));
-    mBuf.geom = s.geom;
-    mBuf.seed = mSeed;
     mSubsteps = 0;
     mPrevBlocks = 0;
     mAttempts = 0;

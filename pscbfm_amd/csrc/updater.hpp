@@ -54,6 +54,7 @@ public:
     int64_t getAge() const { return mAge; }
     void setRng(int rng);
     void setSeed(uint64_t seed) { mSeed = seed; }
+    uint64_t seed() const { return mSeed; }
     void setStepsBetweenSortings(int64_t steps);
 
     // host half of initialize(): checks and colouring, no device work

@@ -1,0 +1,118 @@
+"""Batched and served forms: every system's trajectory is bit-identical to the
+oracle's serial replay of that system alone (the same trajectory as the plain
+updater), whatever the batch composition, chunking, graph use or admission
+order."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from pscbfm_amd import Batch, Server, member_bytes, systems, updater_from_system
+from pscbfm_amd.batch import JOB_CANCELLED, JOB_DONE, JOB_FAILED
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_systems(k, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for j in range(k):
+        L = int(rng.choice([16, 32, 64]))
+        N = int(rng.choice([8, 16, 32]))
+        phi = rng.uniform(0.3, 0.5)
+        chains = max(1, int(phi * L ** 3 / (8 * N)))
+        chains = min(chains, L ** 3 // (16 * N))
+        out.append((systems.linear_melt(L, chains, N, seed=seed * 1000 + j), seed * 7919 + j, int(rng.integers(0, 5000))))
+    out.append((systems.diamond_network(64, (2, 2, 8), 32, seed=seed), seed + 17, 123))
+    return out
+
+
+def oracle_final(system, seed, age, n_mcs):
+    want, acc, prop = orc.sweep(orc.System(system, seed), system["xyz"], age, n_mcs, 0)
+    return want, acc, prop
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_batch_members_match_oracle(gpu_ready, graphs):
+    cases = mixed_systems(6, seed=3)
+    print(f"case: batch of {len(cases)} systems, graphs={graphs}", flush=True)
+    b = Batch(0)
+    b.set_graphs(graphs)
+    ids = []
+    for s, seed, age in cases:
+        u = updater_from_system(s, seed=seed, age=age, validate_only=True)
+        ids.append(b.add(u, len(s["xyz"])))
+    b.run(60, chunk=25)     # chunks of 25, 25, 10
+    b.run(90, chunk=30)
+    for m, (s, seed, age) in zip(ids, cases):
+        got = b.positions(m)
+        want, acc, prop = oracle_final(s, seed, age, 150)
+        assert np.array_equal(got, want), f"member {m} differs"
+        st = b.stats(m)
+        assert (st["accepted"], st["proposals"]) == (acc, prop)
+        assert b.age(m) == age + 150
+
+
+def test_batch_subset_and_removal(gpu_ready):
+    cases = mixed_systems(4, seed=9)
+    b = Batch(0)
+    ids = [b.add(updater_from_system(s, seed=seed, age=age, validate_only=True), len(s["xyz"]))
+           for s, seed, age in cases]
+    b.run_members([ids[0], ids[2]], 40, chunk=20)
+    b.remove(ids[1])
+    b.run(30, chunk=30)
+    for k, (m, (s, seed, age)) in enumerate(zip(ids, cases)):
+        if k == 1:
+            continue
+        mcs = 70 if k in (0, 2) else 30
+        want, _, _ = oracle_final(s, seed, age, mcs)
+        assert np.array_equal(b.positions(m), want)
+
+
+def test_server_jobs_match_alone(gpu_ready):
+    cases = mixed_systems(8, seed=21)
+    mcs = [100, 300, 200, 100, 400, 200, 300, 100, 200]
+    # a budget that holds only a few jobs at once forces admission waves
+    bytes_each = [member_bytes(updater_from_system(s, seed=seed, age=age, validate_only=True))
+                  for s, seed, age in cases]
+    srv = Server(0, hbm_budget=int(2.5 * max(bytes_each)), chunk=100)
+    jobs = []
+    for (s, seed, age), n in zip(cases[:5], mcs):
+        jobs.append(srv.submit(updater_from_system(s, seed=seed, age=age, validate_only=True), n, len(s["xyz"])))
+    srv.step()
+    # jobs arriving while others run join at the next round
+    for (s, seed, age), n in zip(cases[5:], mcs[5:]):
+        jobs.append(srv.submit(updater_from_system(s, seed=seed, age=age, validate_only=True), n, len(s["xyz"])))
+    rounds = 1
+    while srv.step() > 0:
+        rounds += 1
+        assert srv.info()["bytes_in_use"] <= srv.info()["budget"]
+        assert rounds < 100
+    for j, (s, seed, age), n in zip(jobs, cases, mcs):
+        st, done = srv.state(j)
+        assert st == JOB_DONE and done == n
+        want, _, _ = oracle_final(s, seed, age, n)
+        assert np.array_equal(srv.result(j), want), f"job {j} differs from running alone"
+        t_sub, t_adm, t_fin = srv.times(j)
+        assert 0 <= t_sub <= t_adm <= t_fin
+
+
+def test_server_cancel_and_oversized(gpu_ready):
+    cases = mixed_systems(3, seed=33)
+    srv = Server(0, hbm_budget=0, chunk=50)
+    jobs = [srv.submit(updater_from_system(s, seed=seed, age=age, validate_only=True), 200, len(s["xyz"]))
+            for s, seed, age in cases]
+    srv.step()
+    srv.step()                       # 100 of 200 MCS done
+    srv.cancel(jobs[1])
+    assert srv.state(jobs[1])[0] == JOB_CANCELLED
+    while srv.step() > 0:
+        pass
+    for k in (0, 2, 3):
+        s, seed, age = cases[k]
+        want, _, _ = oracle_final(s, seed, age, 200)
+        assert np.array_equal(srv.result(jobs[k]), want)
+    small = Server(0, hbm_budget=4096, chunk=50)
+    s, seed, age = cases[0]
+    j = small.submit(updater_from_system(s, seed=seed, age=age, validate_only=True), 100, len(s["xyz"]))
+    assert small.state(j)[0] == JOB_FAILED
+    assert "budget" in small.error(j)

@@ -170,3 +170,21 @@ def test_network_generator_valid(box, cells, n):
 def test_network_generator_rejects_parity():
     with pytest.raises(InvalidArgument, match="parity"):
         systems.diamond_network(64, (2, 2, 8), 31, seed=1)   # 32 steps vs spacing 4+4+1
+
+
+def test_batch_and_server_need_a_device():
+    if pscbfm_amd.device_count() > 0:
+        pytest.skip("a HIP device is visible")
+    with pytest.raises(pscbfm_amd.NoDeviceError):
+        pscbfm_amd.Batch(0)
+    with pytest.raises(pscbfm_amd.NoDeviceError):
+        pscbfm_amd.Server(0)
+
+
+def test_member_bytes_cover_the_layout():
+    s = systems.linear_melt(64, 512, 32, seed=1)
+    u = updater_from_system(s, seed=1, validate_only=True)
+    b = pscbfm_amd.member_bytes(u)
+    n = len(s["xyz"])
+    assert b >= 16 * n + 4 * 8 * n + 4 * 64 ** 3 // 8   # positions, partner slots, four bit lattices
+    assert b < 200 * n + 8 * 64 ** 3
