@@ -61,6 +61,113 @@ def cpu_baseline(system: dict, seed: int, target_s: float) -> dict:
                       f"({dt:.1f} s on 1 host core)"}
 
 
+def batched_leg(args) -> dict:
+    """Mixed-size batch: every system swept in the same launches (hipGraph per
+    chunk) versus a plain per-system launch sequence, with bit-exact
+    comparison of every final state."""
+    import numpy as np
+    from pscbfm_amd import Batch, updater_from_system, workloads
+    t = time.perf_counter()
+    specs = workloads.mixed_batch(args.batch_systems, seed=0xBA7C4)
+    built = [workloads.build(sp) for sp in specs]
+    n_tot = sum(len(x["xyz"]) for x in built)
+    log(f"batched: {len(built)} systems, {n_tot} monomers, built in {time.perf_counter() - t:.1f} s")
+
+    # plain: one initialized updater per system, run one after the other
+    plain = [updater_from_system(x, seed=sp["seed"], gpu=0) for x, sp in zip(built, specs)]
+    for u in plain:
+        u.getStats()
+    t0 = time.perf_counter()
+    for u in plain:
+        u.runResident(args.plain_mcs)
+    for u in plain:
+        u.getStats()                  # synchronizes each updater's stream
+    t_plain = time.perf_counter() - t0
+    plain_rate = n_tot * args.plain_mcs / t_plain
+    log(f"plain per-system: {args.plain_mcs} MCS each in {t_plain:.2f} s -> {plain_rate:.4g} attempts/s")
+
+    b = Batch(0)
+    ids = [b.add(updater_from_system(x, seed=sp["seed"], validate_only=True), len(x["xyz"]))
+           for x, sp in zip(built, specs)]
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(args.plain_mcs, chunk=args.chunk)
+    b.synchronize()
+    t_a = time.perf_counter() - t0
+    exact = 0
+    for u, m, x in zip(plain, ids, built):
+        u.copyBack()
+        exact += int(np.array_equal(u.getMonomerPositions(len(x["xyz"])), b.positions(m)))
+        u.cleanup()
+    rest = args.batch_mcs - args.plain_mcs
+    t0 = time.perf_counter()
+    if rest > 0:
+        b.run(rest, chunk=args.chunk)
+    b.synchronize()
+    t_batched = t_a + time.perf_counter() - t0
+    rate = n_tot * args.batch_mcs / t_batched
+    log(f"batched: {args.batch_mcs} MCS in {t_batched:.2f} s -> {rate:.4g} attempts/s, "
+        f"{exact}/{len(built)} states bit-exact vs plain")
+    return {"systems": len(built), "monomers": n_tot, "mcs": args.batch_mcs, "chunk": args.chunk,
+            "value": rate, "unit": "attempts/s", "plain_value": plain_rate, "plain_mcs": args.plain_mcs,
+            "speedup_vs_plain": rate / plain_rate, "bit_exact_systems": exact}
+
+
+def served_leg(args, capacity: float) -> dict:
+    """Poisson job arrivals at rates swept around the batched capacity;
+    continuous batching admitted by free HBM; p50/p99 completion latency."""
+    import numpy as np
+    from pscbfm_amd import Server, updater_from_system, workloads
+    points = []
+    mean_work = None
+    for k, factor in enumerate(args.load_factors):
+        jobs = workloads.job_stream(args.jobs, 1.0, seed=0x5E4E + k)   # arrival times rescaled below
+        built = [workloads.build(j) for j in jobs]
+        work = np.array([len(x["xyz"]) * j["mcs"] for x, j in zip(built, jobs)], dtype=float)
+        mean_work = float(work.mean())
+        rate = factor * capacity / mean_work                          # jobs per second
+        arrivals = np.array([j["arrival"] for j in jobs]) / rate       # unit-rate stream rescaled
+        upd = [updater_from_system(x, seed=j["seed"], validate_only=True) for x, j in zip(built, jobs)]
+        srv = Server(0, hbm_budget=0, chunk=args.chunk)
+        t0 = time.perf_counter()
+        ids = [None] * len(jobs)
+        finish = [None] * len(jobs)
+        running = set()
+        nxt = 0
+        while True:
+            now = time.perf_counter() - t0
+            while nxt < len(jobs) and arrivals[nxt] <= now:
+                ids[nxt] = srv.submit(upd[nxt], jobs[nxt]["mcs"], len(built[nxt]["xyz"]))
+                running.add(nxt)
+                nxt += 1
+            pending = srv.step()
+            now = time.perf_counter() - t0
+            for j in list(running):
+                if srv.state(ids[j])[0] >= 2:      # done, failed or cancelled
+                    finish[j] = now
+                    running.discard(j)
+            if pending == 0:
+                if nxt == len(jobs):
+                    break
+                time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+        lat = [finish[j] - arrivals[j] for j in range(len(jobs))]
+        makespan = max(finish) - arrivals[0]
+        exact = 0
+        for j in range(0, len(jobs), max(1, len(jobs) // 8)):
+            u = updater_from_system(built[j], seed=jobs[j]["seed"], gpu=0)
+            u.runSimulationOnGPU(jobs[j]["mcs"])
+            exact += int(np.array_equal(u.getMonomerPositions(len(built[j]["xyz"])), srv.result(ids[j])))
+            u.cleanup()
+        lat = np.array(lat)
+        pt = {"load_factor": factor, "rate_jobs_per_s": round(rate, 2), "jobs": len(jobs),
+              "throughput": float(work.sum() / makespan), "p50_latency_s": float(np.percentile(lat, 50)),
+              "p99_latency_s": float(np.percentile(lat, 99)), "bit_exact_sampled": f"{exact}/{len(range(0, len(jobs), max(1, len(jobs) // 8)))}"}
+        log(f"served at {factor}x capacity ({rate:.1f} jobs/s): {pt['throughput']:.4g} attempts/s, "
+            f"p50 {pt['p50_latency_s']*1e3:.0f} ms, p99 {pt['p99_latency_s']*1e3:.0f} ms, exact {pt['bit_exact_sampled']}")
+        points.append(pt)
+    return {"unit": "attempts/s", "chunk": args.chunk, "mean_job_attempts": mean_work, "points": points}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +176,14 @@ def main() -> int:
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large_melt")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-systems", type=int, default=512)
+    ap.add_argument("--batch-mcs", type=int, default=1000)
+    ap.add_argument("--plain-mcs", type=int, default=200)
+    ap.add_argument("--chunk", type=int, default=100)
+    ap.add_argument("--jobs", type=int, default=96, help="jobs per served-load point")
+    ap.add_argument("--load-factors", type=float, nargs="*", default=[0.5, 1.0, 2.0])
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the plain copy kernel that calibrates HBM counters (0 = skip)")
     args = ap.parse_args()
@@ -149,6 +264,10 @@ def main() -> int:
         "roofline": roofline,
     }
     u.cleanup()
+    if not args.no_batched:
+        out["batched"] = batched_leg(args)
+        if not args.no_served:
+            out["served"] = served_leg(args, out["batched"]["value"])
     if not args.no_cpu:
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(system, w["seed"], args.cpu_seconds)
