@@ -288,13 +288,12 @@ void BatchEngine::copyBack(uint32_t id)
         throw std::invalid_argument("batch: member was removed");
     synchronize();
     HostSystem &s = m.sys;
-    hip_check(hipMemcpy(s.pos_sorted.data(), m.dev.buf.pos, sizeof(int4) * s.n_pad, hipMemcpyDeviceToHost),
-              "download pos");
-    for (uint32_t i = 0; i < s.n; ++i) {
-        const uint32_t t = s.to_sorted[i];
-        for (int a = 0; a < 3; ++a)
-            s.xyz[3 * size_t(i) + a] = s.pos_sorted[4 * size_t(t) + a];
-    }
+    hip_check(hipMemcpy(s.ppos_sorted.data(), m.dev.buf.ppos, sizeof(uint32_t) * s.n_pad, hipMemcpyDeviceToHost),
+              "download positions");
+    hip_check(hipMemcpy(s.image_sorted.data(), m.dev.buf.image, sizeof(int32_t) * 3 * size_t(s.n_pad),
+                        hipMemcpyDeviceToHost),
+              "download images");
+    unpack_positions(s);
 }
 
 void BatchEngine::counters(uint32_t id, uint64_t &accepted, uint64_t &proposals)

@@ -2,8 +2,12 @@
 //
 // Layout of one system in HBM (all arrays are indexed by the *sorted* monomer
 // index: monomers grouped by colour, each colour segment padded to a multiple
-// of kSegmentAlign so that every colour starts on a 1 KiB boundary):
-//   int4     pos[nPad]         x, y, z (unwrapped int32), w = original id
+// of kSegmentAlign so that every colour starts on an aligned boundary):
+//   uint32_t ppos[nPad]        wrapped x | y << 10 | z << 20 (box edges <= 1024)
+//   uint32_t ids[nPad]         original monomer id (keys the RNG)
+//   int32_t  image[3][nPad]    periodic image of each axis: unwrapped = wrapped
+//                              + L * image; changed only when a move crosses
+//                              the box boundary
 //   uint32_t nbr[8][nPad]      sorted index of the j-th bond partner (slot-major,
 //                              so lane i reads nbr[j*nPad+i] coalesced)
 //   uint8_t  nnbr[nPad]        number of bond partners
@@ -11,8 +15,13 @@
 //                              monomer cube, compact Morton order
 //   uint32_t claim[W]          sites entered by this substep's proposals
 //   uint32_t coll[2][W]        sites entered by more than one proposal
-//   uint32_t clist[2][cap]     words of coll[] that hold a bit (for clearing)
-//   uint32_t surv[maxColour]   compacted proposals: sorted index | dir << 29
+//   uint2    surv[blocks*256]  compacted proposals: (sorted index | dir << 29,
+//                              packed position), per check workgroup
+//   uint32_t clist[2][...]     per-workgroup lists of coll[] words to clear
+//
+// Positions are wrapped on the device: a bond vector is only needed modulo 8
+// (the bond-table index (d & 7), UpdaterGPUScBFM_AB_Type.cu:871-889) and every
+// box edge is a multiple of 8, so wrapped coordinates give the same index.
 //
 // The reference keeps 1 lattice byte per monomer (its lower corner) and tests
 // a 9-site front (UpdaterGPUScBFM_AB_Type.cu:420-594); this engine stores the
@@ -31,6 +40,18 @@ constexpr int kBlock = 256;              // threads per workgroup (4 wave64)
 constexpr uint32_t kSortedIndexBits = 29;
 constexpr uint32_t kSortedIndexMask = (1u << kSortedIndexBits) - 1u;
 constexpr uint32_t kMaxBox = 1024;       // 10 bits per axis, 30-bit site index
+constexpr uint32_t kCoordBits = 10;      // bits per axis of a packed position
+constexpr uint32_t kCoordMask = (1u << kCoordBits) - 1u;
+
+__host__ __device__ inline uint32_t pack_xyz(uint32_t x, uint32_t y, uint32_t z)
+{
+    return x | (y << kCoordBits) | (z << (2 * kCoordBits));
+}
+__host__ __device__ inline int4 unpack_xyz(uint32_t p)
+{
+    return make_int4(int32_t(p & kCoordMask), int32_t((p >> kCoordBits) & kCoordMask),
+                     int32_t((p >> (2 * kCoordBits)) & kCoordMask), 0);
+}
 
 // Geometry of one periodic (or walled) power-of-two box.
 struct Geometry {

@@ -87,11 +87,12 @@ __device__ inline void check_body(const SweepBuffers &buf, uint32_t begin, uint3
 
     const uint32_t t = blk * kBlock + threadIdx.x;
     bool pass = false;
-    uint32_t entry = 0;
+    uint2 entry = make_uint2(0u, 0u);
     if (t < count) {
         const uint32_t i = begin + t;
-        const int4 r0 = buf.pos[i];
-        const uint32_t rn = philox_word0(buf.seed, uint32_t(r0.w), age, kTagMove);
+        const uint32_t p0 = buf.ppos[i];
+        const int4 r0 = unpack_xyz(p0);
+        const uint32_t rn = philox_word0(buf.seed, buf.ids[i], age, kTagMove);
         const int dir = int(rn % 6u);
         const int4 r1 = moved(r0, dir);
 
@@ -114,7 +115,7 @@ __device__ inline void check_body(const SweepBuffers &buf, uint32_t begin, uint3
 #pragma unroll
         for (int j = 0; j < kMaxConnectivity; ++j)
             if (uint32_t(j) < nn) {
-                const int4 q = buf.pos[partner[j]];
+                const int4 q = unpack_xyz(buf.ppos[partner[j]]);
                 if (bond_forbidden(bond, q.x - r1.x, q.y - r1.y, q.z - r1.z))
                     ok = false;
             }
@@ -144,7 +145,7 @@ __device__ inline void check_body(const SweepBuffers &buf, uint32_t begin, uint3
                     }
                 }
             pass = true;
-            entry = i | (uint32_t(dir) << kSortedIndexBits);
+            entry = make_uint2(i | (uint32_t(dir) << kSortedIndexBits), p0);
         }
     }
 
@@ -213,10 +214,10 @@ __device__ inline void perform_body(const SweepBuffers &buf, uint32_t check_bloc
 #pragma unroll
         for (uint32_t q = 1; q < kPerformGroup; ++q)
             k += t >= s_pre[q] ? 1u : 0u;
-        const uint32_t e = buf.surv[size_t(blk * kPerformGroup + k) * kBlock + (t - s_pre[k])];
-        const uint32_t i = e & kSortedIndexMask;
-        const int dir = int(e >> kSortedIndexBits);
-        const int4 r0 = buf.pos[i];
+        const uint2 e = buf.surv[size_t(blk * kPerformGroup + k) * kBlock + (t - s_pre[k])];
+        const uint32_t i = e.x & kSortedIndexMask;
+        const int dir = int(e.x >> kSortedIndexBits);
+        const int4 r0 = unpack_xyz(e.y);
         uint32_t s[4], d_face;
         face_sites(g, dep, r0, dir, s, d_face);
         const bool clash = bit_test(coll, s[0]) | bit_test(coll, s[1]) | bit_test(coll, s[2]) | bit_test(coll, s[3]);
@@ -256,7 +257,15 @@ __device__ inline void perform_body(const SweepBuffers &buf, uint32_t check_bloc
             for (int q = 0; q < 8; ++q)
                 if (m[q])
                     atomicXor(&buf.lattice[w[q]], m[q]);
-            buf.pos[i] = moved(r0, dir);
+            // new wrapped coordinate on the move axis; crossing the box
+            // boundary changes the periodic image (only on periodic axes:
+            // walls keep the others inside)
+            const uint32_t am = sel3(g.mask, a);
+            const uint32_t v = uint32_t(comp(r0, a) + ((dir & 1) ? 1 : -1)) & am;
+            const uint32_t shift = kCoordBits * uint32_t(a);
+            buf.ppos[i] = (e.y & ~(kCoordMask << shift)) | (v << shift);
+            if ((dir & 1) ? v == 0u : v == am)
+                buf.image[size_t(a) * buf.n_pad + i] += (dir & 1) ? 1 : -1;
         }
         const uint64_t acc = __ballot(!clash);
         if ((threadIdx.x & 63u) == uint32_t(__ffsll((unsigned long long)__ballot(1)) - 1))

@@ -13,7 +13,9 @@ constexpr uint32_t kStatSlots = 256;    // spread of the accepted / proposed cou
 
 // Everything one substep of one system needs on the device.
 struct SweepBuffers {
-    int4 *pos;
+    uint32_t *ppos;           // packed wrapped positions
+    const uint32_t *ids;      // original ids
+    int32_t *image;           // [3][n_pad] periodic images
     const uint32_t *nbr;
     const uint8_t *nnbr;
     uint32_t n_pad;
@@ -22,7 +24,7 @@ struct SweepBuffers {
     uint32_t *coll[2];
     uint32_t *clist[2];       // [check block][kCollSlots] collision words
     uint32_t *ccount[2];      // [check block] entries in clist
-    uint32_t *surv;           // [check block][kBlock] compacted proposals
+    uint2 *surv;              // [check block][kBlock] compacted proposals
     uint32_t *scount;         // [check block] proposals
     unsigned long long *accepted;   // [kStatSlots]
     unsigned long long *proposals;  // [kStatSlots]

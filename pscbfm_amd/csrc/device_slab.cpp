@@ -11,7 +11,9 @@ SlabLayout slab_layout(const HostSystem &s)
 {
     const size_t blocks = check_blocks(std::max<uint32_t>(s.max_segment, 1));
     SlabLayout L{};
-    L.bytes[kPartPos] = sizeof(int4) * s.n_pad;
+    L.bytes[kPartPPos] = sizeof(uint32_t) * s.n_pad;
+    L.bytes[kPartIds] = sizeof(uint32_t) * s.n_pad;
+    L.bytes[kPartImage] = sizeof(int32_t) * 3 * size_t(s.n_pad);
     L.bytes[kPartNbr] = sizeof(uint32_t) * kMaxConnectivity * s.n_pad;
     L.bytes[kPartNnbr] = size_t(s.n_pad);
     L.bytes[kPartLattice] = sizeof(uint32_t) * s.lattice_words;
@@ -22,7 +24,7 @@ SlabLayout slab_layout(const HostSystem &s)
     L.bytes[kPartClist1] = sizeof(uint32_t) * kCollSlots * blocks;
     L.bytes[kPartCcount0] = sizeof(uint32_t) * blocks;
     L.bytes[kPartCcount1] = sizeof(uint32_t) * blocks;
-    L.bytes[kPartSurv] = sizeof(uint32_t) * kBlock * blocks;
+    L.bytes[kPartSurv] = sizeof(uint2) * kBlock * blocks;
     L.bytes[kPartScount] = sizeof(uint32_t) * blocks;
     L.bytes[kPartAccepted] = sizeof(unsigned long long) * kStatSlots;
     L.bytes[kPartProposals] = sizeof(unsigned long long) * kStatSlots;
@@ -44,7 +46,9 @@ SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, u
         hip_check(hipMemcpyAsync(slab_part(slab, L, p), src, bytes, hipMemcpyHostToDevice, stream), what);
     };
     hip_check(hipMemsetAsync(slab, 0, L.total, stream), "hipMemsetAsync (slab)");
-    up(kPartPos, s.pos_sorted.data(), L.bytes[kPartPos], "upload pos");
+    up(kPartPPos, s.ppos_sorted.data(), L.bytes[kPartPPos], "upload positions");
+    up(kPartIds, s.ids_sorted.data(), L.bytes[kPartIds], "upload ids");
+    up(kPartImage, s.image_sorted.data(), L.bytes[kPartImage], "upload images");
     up(kPartNbr, s.nbr_sorted.data(), L.bytes[kPartNbr], "upload nbr");
     up(kPartNnbr, s.nnbr_sorted.data(), L.bytes[kPartNnbr], "upload nnbr");
     up(kPartLattice, s.lattice.data(), L.bytes[kPartLattice], "upload lattice");
@@ -53,7 +57,9 @@ SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, u
     up(kPartSegCount, s.seg_count.data(), sizeof(uint32_t) * s.n_colours, "upload segments");
 
     SweepBuffers b{};
-    b.pos = static_cast<int4 *>(slab_part(slab, L, kPartPos));
+    b.ppos = static_cast<uint32_t *>(slab_part(slab, L, kPartPPos));
+    b.ids = static_cast<const uint32_t *>(slab_part(slab, L, kPartIds));
+    b.image = static_cast<int32_t *>(slab_part(slab, L, kPartImage));
     b.nbr = static_cast<const uint32_t *>(slab_part(slab, L, kPartNbr));
     b.nnbr = static_cast<const uint8_t *>(slab_part(slab, L, kPartNnbr));
     b.n_pad = s.n_pad;
@@ -65,7 +71,7 @@ SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, u
     b.clist[1] = static_cast<uint32_t *>(slab_part(slab, L, kPartClist1));
     b.ccount[0] = static_cast<uint32_t *>(slab_part(slab, L, kPartCcount0));
     b.ccount[1] = static_cast<uint32_t *>(slab_part(slab, L, kPartCcount1));
-    b.surv = static_cast<uint32_t *>(slab_part(slab, L, kPartSurv));
+    b.surv = static_cast<uint2 *>(slab_part(slab, L, kPartSurv));
     b.scount = static_cast<uint32_t *>(slab_part(slab, L, kPartScount));
     b.accepted = static_cast<unsigned long long *>(slab_part(slab, L, kPartAccepted));
     b.proposals = static_cast<unsigned long long *>(slab_part(slab, L, kPartProposals));

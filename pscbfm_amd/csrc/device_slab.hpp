@@ -12,7 +12,9 @@
 namespace pscbfm {
 
 enum SlabPart {
-    kPartPos,
+    kPartPPos,
+    kPartIds,
+    kPartImage,
     kPartNbr,
     kPartNnbr,
     kPartLattice,

@@ -31,7 +31,9 @@ void HostSystem::reset()
     n_pad = 0;
     max_segment = 0;
     to_sorted.clear();
-    pos_sorted.clear();
+    ppos_sorted.clear();
+    ids_sorted.clear();
+    image_sorted.clear();
     nbr_sorted.clear();
     nnbr_sorted.clear();
     lattice.clear();
@@ -271,20 +273,39 @@ void plan(HostSystem &s)
         for (uint32_t i = 0; i < s.n; ++i)
             s.to_sorted[i] = next[s.colour[i]]++;
     }
-    s.pos_sorted.assign(4 * size_t(s.n_pad), 0);
+    s.ppos_sorted.assign(s.n_pad, 0u);
+    s.ids_sorted.assign(s.n_pad, 0xFFFFFFFFu);
+    s.image_sorted.assign(3 * size_t(s.n_pad), 0);
     s.nbr_sorted.assign(size_t(kMaxConnectivity) * s.n_pad, 0);
     s.nnbr_sorted.assign(s.n_pad, 0);
-    for (size_t k = 0; k < s.n_pad; ++k)
-        s.pos_sorted[4 * k + 3] = -1;
     for (uint32_t i = 0; i < s.n; ++i) {
         const uint32_t t = s.to_sorted[i];
-        s.pos_sorted[4 * size_t(t) + 0] = s.xyz[3 * size_t(i) + 0];
-        s.pos_sorted[4 * size_t(t) + 1] = s.xyz[3 * size_t(i) + 1];
-        s.pos_sorted[4 * size_t(t) + 2] = s.xyz[3 * size_t(i) + 2];
-        s.pos_sorted[4 * size_t(t) + 3] = int32_t(i);
+        uint32_t w[3];
+        for (int a = 0; a < 3; ++a) {
+            const int64_t v = s.xyz[3 * size_t(i) + a];
+            const int64_t L = int64_t(s.box[a]);
+            const int64_t img = (v >= 0 ? v / L : -((-v + L - 1) / L));   // floor(v / L)
+            w[a] = uint32_t(v - img * L);
+            s.image_sorted[size_t(a) * s.n_pad + t] = int32_t(img);
+        }
+        s.ppos_sorted[t] = pack_xyz(w[0], w[1], w[2]);
+        s.ids_sorted[t] = i;
         s.nnbr_sorted[t] = s.nnbr[i];
         for (uint32_t j = 0; j < s.nnbr[i]; ++j)
             s.nbr_sorted[size_t(j) * s.n_pad + t] = s.to_sorted[s.nbr[size_t(i) * kMaxConnectivity + j]];
+    }
+}
+
+void unpack_positions(HostSystem &s)
+{
+    for (uint32_t i = 0; i < s.n; ++i) {
+        const uint32_t t = s.to_sorted[i];
+        const uint32_t p = s.ppos_sorted[t];
+        for (int a = 0; a < 3; ++a) {
+            const int64_t w = (p >> (kCoordBits * uint32_t(a))) & kCoordMask;
+            const int64_t img = s.image_sorted[size_t(a) * s.n_pad + t];
+            s.xyz[3 * size_t(i) + a] = int32_t(w + img * int64_t(s.box[a]));
+        }
     }
 }
 

@@ -36,7 +36,9 @@ struct HostSystem {
     uint32_t n_pad = 0;              // padded sorted length
     uint32_t max_segment = 0;
     std::vector<uint32_t> to_sorted; // original id -> sorted index
-    std::vector<int32_t> pos_sorted; // 4 * n_pad: x, y, z, original id
+    std::vector<uint32_t> ppos_sorted;  // n_pad packed wrapped positions
+    std::vector<uint32_t> ids_sorted;   // n_pad original ids (padding: 0xFFFFFFFF)
+    std::vector<int32_t> image_sorted;  // 3 * n_pad periodic images, axis-major
     std::vector<uint32_t> nbr_sorted;  // kMaxConnectivity * n_pad, slot-major
     std::vector<uint8_t> nnbr_sorted;  // n_pad
     std::vector<uint32_t> lattice;     // lattice_words occupancy bits
@@ -66,6 +68,10 @@ uint32_t host_site(const Geometry &g, const Tables &t, int32_t x, int32_t y, int
 // partners has.  Throws std::runtime_error above 255 colours.
 uint32_t greedy_colouring(uint32_t n, const std::vector<uint8_t> &nnbr, const std::vector<uint32_t> &nbr,
                           std::vector<uint32_t> &colour);
+
+// Unwrapped coordinates of every monomer (by original id) from the packed
+// wrapped positions and periodic images in sorted order.
+void unpack_positions(HostSystem &s);
 
 // Runs every check of initialize() and fills the derived fields.
 // Throws std::invalid_argument / std::runtime_error like the reference.

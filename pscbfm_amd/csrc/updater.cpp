@@ -274,16 +274,15 @@ void UpdaterGPUScBFM_AB_Type::copyBack()
         throw StateError("copy-back before initialize");
     hip_check(hipSetDevice(mGpu), "hipSetDevice");
     HostSystem &s = mSys;
-    hip_check(hipMemcpyAsync(s.pos_sorted.data(), mBuf.pos, sizeof(int4) * s.n_pad, hipMemcpyDeviceToHost, mStream),
-              "download pos");
+    hip_check(hipMemcpyAsync(s.ppos_sorted.data(), mBuf.ppos, sizeof(uint32_t) * s.n_pad, hipMemcpyDeviceToHost,
+                             mStream),
+              "download positions");
+    hip_check(hipMemcpyAsync(s.image_sorted.data(), mBuf.image, sizeof(int32_t) * 3 * size_t(s.n_pad),
+                             hipMemcpyDeviceToHost, mStream),
+              "download images");
     hip_check(hipStreamSynchronize(mStream), "hipStreamSynchronize (copy-back)");
     collectTimings();
-    for (uint32_t i = 0; i < s.n; ++i) {
-        const uint32_t t = s.to_sorted[i];
-        s.xyz[3 * size_t(i) + 0] = s.pos_sorted[4 * size_t(t) + 0];
-        s.xyz[3 * size_t(i) + 1] = s.pos_sorted[4 * size_t(t) + 1];
-        s.xyz[3 * size_t(i) + 2] = s.pos_sorted[4 * size_t(t) + 2];
-    }
+    unpack_positions(s);
 }
 
 void UpdaterGPUScBFM_AB_Type::runSimulationOnGPU(uint32_t n_mcs)
