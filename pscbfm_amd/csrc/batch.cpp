@@ -43,10 +43,12 @@ BatchEngine::~BatchEngine()
             hipFree(m->slab);
     if (mDevSys)
         hipFree(mDevSys);
-    if (mDevCheckBlk)
-        hipFree(mDevCheckBlk);
-    if (mDevPerformBlk)
-        hipFree(mDevPerformBlk);
+    for (uint32_t c = 0; c < kClasses; ++c) {
+        if (mDevCheckBlk[c])
+            hipFree(mDevCheckBlk[c]);
+        if (mDevPerformBlk[c])
+            hipFree(mDevPerformBlk[c]);
+    }
     if (mStream)
         hipStreamDestroy(mStream);
 }
@@ -78,7 +80,7 @@ uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age)
     m->dev.seg_begin = static_cast<const uint32_t *>(slab_part(m->slab, L, kPartSegBegin));
     m->dev.seg_count = static_cast<const uint32_t *>(slab_part(m->slab, L, kPartSegCount));
     m->check_blocks_max = check_blocks(std::max<uint32_t>(m->sys.max_segment, 1));
-    m->perform_blocks_max = (m->check_blocks_max + kPerformGroup - 1) / kPerformGroup;
+    m->perform_blocks_max = perform_blocks(m->check_blocks_max);
     mBytes += L.total;
     mMembers.push_back(std::move(m));
     return uint32_t(mMembers.size() - 1);
@@ -118,31 +120,37 @@ void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
 {
     synchronize();   // tables may still be read by in-flight launches
     std::vector<BatchSystem> sys;
-    std::vector<uint32_t> cblk, pblk;
+    std::vector<uint32_t> cblk[kClasses], pblk[kClasses];
     mMaxColours = 0;
     for (uint32_t j = 0; j < ids.size(); ++j) {
         Member &m = *mMembers.at(ids[j]);
         if (!m.slab)
             throw std::invalid_argument("batch: member was removed");
+        const uint32_t cls = kernel_class(m.sys.geom);
         BatchSystem d = m.dev;
-        d.check_base = uint32_t(cblk.size());
-        d.perform_base = uint32_t(pblk.size());
-        cblk.insert(cblk.end(), m.check_blocks_max, j);
-        pblk.insert(pblk.end(), m.perform_blocks_max, j);
+        d.check_base = uint32_t(cblk[cls].size());
+        d.perform_base = uint32_t(pblk[cls].size());
+        cblk[cls].insert(cblk[cls].end(), m.check_blocks_max, j);
+        pblk[cls].insert(pblk[cls].end(), m.perform_blocks_max, j);
         sys.push_back(d);
         mMaxColours = std::max(mMaxColours, m.sys.n_colours);
     }
     ensure_device(mDevSys, mSysCap, sys.size(), "hipMalloc (batch systems)");
-    ensure_device(mDevCheckBlk, mCheckCap, cblk.size(), "hipMalloc (batch blocks)");
-    ensure_device(mDevPerformBlk, mPerformCap, pblk.size(), "hipMalloc (batch blocks)");
     hip_check(hipMemcpy(mDevSys, sys.data(), sizeof(BatchSystem) * sys.size(), hipMemcpyHostToDevice),
               "upload batch systems");
-    hip_check(hipMemcpy(mDevCheckBlk, cblk.data(), sizeof(uint32_t) * cblk.size(), hipMemcpyHostToDevice),
-              "upload batch blocks");
-    hip_check(hipMemcpy(mDevPerformBlk, pblk.data(), sizeof(uint32_t) * pblk.size(), hipMemcpyHostToDevice),
-              "upload batch blocks");
-    mCheckBlocks = uint32_t(cblk.size());
-    mPerformBlocks = uint32_t(pblk.size());
+    for (uint32_t c = 0; c < kClasses; ++c) {
+        mCheckBlocks[c] = uint32_t(cblk[c].size());
+        mPerformBlocks[c] = uint32_t(pblk[c].size());
+        if (cblk[c].empty())
+            continue;
+        ensure_device(mDevCheckBlk[c], mCheckCap[c], cblk[c].size(), "hipMalloc (batch blocks)");
+        ensure_device(mDevPerformBlk[c], mPerformCap[c], pblk[c].size(), "hipMalloc (batch blocks)");
+        hip_check(hipMemcpy(mDevCheckBlk[c], cblk[c].data(), sizeof(uint32_t) * cblk[c].size(), hipMemcpyHostToDevice),
+                  "upload batch blocks");
+        hip_check(hipMemcpy(mDevPerformBlk[c], pblk[c].data(), sizeof(uint32_t) * pblk[c].size(),
+                            hipMemcpyHostToDevice),
+                  "upload batch blocks");
+    }
     mCompIds = ids;
     ++mCompVersion;
 }
@@ -210,13 +218,17 @@ void BatchEngine::runChunk(uint32_t mcs)
               "upload schedule");
 
     auto launch_all = [&]() {
-        for (uint32_t slot = 0; slot < slots; ++slot) {
-            hip_check(launch_check_batch(mDevSys, mDevCheckBlk, mCheckBlocks, mDevSched[b], n_sys, slot, mStream),
-                      "batched check kernel");
-            hip_check(launch_perform_batch(mDevSys, mDevPerformBlk, mPerformBlocks, mDevSched[b], n_sys, slot,
-                                           mStream),
-                      "batched perform kernel");
-        }
+        for (uint32_t slot = 0; slot < slots; ++slot)
+            for (uint32_t c = 0; c < kClasses; ++c) {
+                if (!mCheckBlocks[c])
+                    continue;
+                hip_check(launch_check_batch(c, mDevSys, mDevCheckBlk[c], mCheckBlocks[c], mDevSched[b], n_sys, slot,
+                                             mStream),
+                          "batched check kernel");
+                hip_check(launch_perform_batch(c, mDevSys, mDevPerformBlk[c], mPerformBlocks[c], mDevSched[b], n_sys,
+                                               slot, mStream),
+                          "batched perform kernel");
+            }
     };
     // capture a chunk only once its composition has run before: a server
     // whose composition changes every round launches directly
@@ -252,7 +264,10 @@ void BatchEngine::runChunk(uint32_t mcs)
     } else {
         launch_all();
     }
-    mKernelLaunches += 2ull * slots;
+    uint32_t classes = 0;
+    for (uint32_t c = 0; c < kClasses; ++c)
+        classes += mCheckBlocks[c] ? 1u : 0u;
+    mKernelLaunches += 2ull * slots * classes;
     hip_check(hipEventRecord(mDone[b], mStream), "hipEventRecord");
 }
 

@@ -85,9 +85,12 @@ private:
     // tables of the current composition
     std::vector<uint32_t> mCompIds;          // member ids in launch order
     BatchSystem *mDevSys = nullptr;
-    uint32_t *mDevCheckBlk = nullptr, *mDevPerformBlk = nullptr;
-    uint32_t mCheckBlocks = 0, mPerformBlocks = 0, mMaxColours = 0;
-    size_t mSysCap = 0, mCheckCap = 0, mPerformCap = 0;
+    size_t mSysCap = 0;
+    uint32_t mMaxColours = 0;
+    // per kernel-variant class: block -> system tables
+    uint32_t *mDevCheckBlk[kClasses] = {}, *mDevPerformBlk[kClasses] = {};
+    uint32_t mCheckBlocks[kClasses] = {}, mPerformBlocks[kClasses] = {};
+    size_t mCheckCap[kClasses] = {}, mPerformCap[kClasses] = {};
 
     // schedules of the last two chunks (double-buffered so that the host can
     // write chunk k+1's schedule while chunk k runs)

@@ -37,17 +37,19 @@ namespace pscbfm {
 constexpr int kMaxConnectivity = 8;      // MAX_CONNECTIVITY, UpdaterGPUScBFM_AB_Type.h:85
 constexpr uint32_t kSegmentAlign = 64;   // monomers; 64 x 16 B = 1 KiB
 constexpr int kBlock = 256;              // threads per workgroup (4 wave64)
+constexpr int kWave = 64;                // lanes per wavefront
+constexpr uint32_t kWavesPerBlock = kBlock / kWave;
 constexpr uint32_t kSortedIndexBits = 29;
 constexpr uint32_t kSortedIndexMask = (1u << kSortedIndexBits) - 1u;
 constexpr uint32_t kMaxBox = 1024;       // 10 bits per axis, 30-bit site index
 constexpr uint32_t kCoordBits = 10;      // bits per axis of a packed position
 constexpr uint32_t kCoordMask = (1u << kCoordBits) - 1u;
 
-__host__ __device__ inline uint32_t pack_xyz(uint32_t x, uint32_t y, uint32_t z)
+__host__ __device__ __forceinline__ uint32_t pack_xyz(uint32_t x, uint32_t y, uint32_t z)
 {
     return x | (y << kCoordBits) | (z << (2 * kCoordBits));
 }
-__host__ __device__ inline int4 unpack_xyz(uint32_t p)
+__host__ __device__ __forceinline__ int4 unpack_xyz(uint32_t p)
 {
     return make_int4(int32_t(p & kCoordMask), int32_t((p >> kCoordBits) & kCoordMask),
                      int32_t((p >> (2 * kCoordBits)) & kCoordMask), 0);
@@ -59,6 +61,7 @@ struct Geometry {
     uint32_t dmask[3];   // index bits that belong to axis a (deposited L_a - 1)
     int32_t wall_hi[3];  // L_a - 2: largest lower-corner coordinate between walls
     uint32_t periodic;   // bit a set: axis a is periodic
+    uint32_t cubic;      // 1: all edges equal, the index is the classic Z-curve
 };
 
 // Small per-system tables; each workgroup copies them to LDS.
@@ -69,7 +72,7 @@ struct Tables {
 constexpr int kTableWords = 3 * 64 + 16;
 
 // ---- Philox4x32-10 (Salmon et al., SC'11) -----------------------------------
-__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+__host__ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
@@ -87,7 +90,7 @@ __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
 }
 
 // Word 0 of Philox(key = seed, ctr = (c0, age_lo, age_hi, tag)).
-__host__ __device__ inline uint32_t philox_word0(uint64_t seed, uint32_t c0, uint64_t age, uint32_t tag)
+__host__ __device__ __forceinline__ uint32_t philox_word0(uint64_t seed, uint32_t c0, uint64_t age, uint32_t tag)
 {
     uint32_t c[4] = {c0, uint32_t(age), uint32_t(age >> 32), tag};
     philox4x32_10(c, uint32_t(seed), uint32_t(seed >> 32));
@@ -111,32 +114,46 @@ inline void colour_order(uint64_t seed, uint64_t age, uint32_t n, uint32_t *orde
 }
 
 // ---- dilated (Morton) integer arithmetic ------------------------------------
-__device__ inline uint32_t deposit(const uint32_t *dep, int a, uint32_t v)
+// Spreads the low 10 bits of v to bits 0, 3, 6, ... (part1by2).
+__host__ __device__ __forceinline__ uint32_t spread3(uint32_t v)
 {
-    return dep[a * 64 + (v & 31u)] | dep[a * 64 + 32 + (v >> 5)];
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
 }
-__host__ __device__ inline uint32_t dilated_inc(uint32_t d, uint32_t m) { return ((d | ~m) + 1u) & m; }
-__host__ __device__ inline uint32_t dilated_dec(uint32_t d, uint32_t m) { return (d - 1u) & m; }
 
-__host__ __device__ inline int32_t comp(const int4 &r, int a) { return a == 0 ? r.x : (a == 1 ? r.y : r.z); }
+// Index bits of coordinate v (already masked) on axis a for a cubic box: the
+// compact Morton order is then the classic Z-curve, x in bit 0, computed in
+// registers.  Non-cubic boxes use per-axis tables in LDS (bfm_kernels.hip).
+struct DepositCubic {
+    __device__ __forceinline__ uint32_t operator()(int a, uint32_t v) const { return spread3(v) << a; }
+};
+__host__ __device__ __forceinline__ uint32_t dilated_inc(uint32_t d, uint32_t m) { return ((d | ~m) + 1u) & m; }
+__host__ __device__ __forceinline__ uint32_t dilated_dec(uint32_t d, uint32_t m) { return (d - 1u) & m; }
+
+__host__ __device__ __forceinline__ int32_t comp(const int4 &r, int a) { return a == 0 ? r.x : (a == 1 ? r.y : r.z); }
 // v[a] for a per-lane axis without indexing memory (kernel arguments live in
-// SGPRs; a dynamic index would turn them into vector loads).
-__host__ __device__ inline uint32_t sel3(const uint32_t v[3], int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); }
+// SGPRs; a dynamic index would turn them into scratch or vector loads).
+__host__ __device__ __forceinline__ uint32_t sel3(const uint32_t v[3], int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); }
 
 // Site indices of the 2x2 face a cube with lower corner r enters when it
 // moves in direction dir (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z, as the reference's
 // DXTable_d/DYTable_d/DZTable_d, UpdaterGPUScBFM_AB_Type.cu:1537-1541).
 // Also returns the dilated coordinate of the face plane along the move axis.
-__device__ inline void face_sites(const Geometry &g, const uint32_t *dep, const int4 &r, int dir,
-                                  uint32_t s[4], uint32_t &d_face)
+template <class Dep>
+__device__ __forceinline__ void face_sites(const Geometry &g, const Dep &dep, const int4 &r, int dir, uint32_t s[4],
+                                  uint32_t &d_face)
 {
     const int a = dir >> 1;
     const int b = a == 0 ? 1 : 0;
     const int c = a == 2 ? 1 : 2;
     const int32_t fa = (dir & 1) ? comp(r, a) + 2 : comp(r, a) - 1;
-    d_face = deposit(dep, a, uint32_t(fa) & sel3(g.mask, a));
-    const uint32_t b0 = deposit(dep, b, uint32_t(comp(r, b)) & sel3(g.mask, b));
-    const uint32_t c0 = deposit(dep, c, uint32_t(comp(r, c)) & sel3(g.mask, c));
+    d_face = dep(a, uint32_t(fa) & sel3(g.mask, a));
+    const uint32_t b0 = dep(b, uint32_t(comp(r, b)) & sel3(g.mask, b));
+    const uint32_t c0 = dep(c, uint32_t(comp(r, c)) & sel3(g.mask, c));
     const uint32_t b1 = dilated_inc(b0, sel3(g.dmask, b));
     const uint32_t c1 = dilated_inc(c0, sel3(g.dmask, c));
     s[0] = d_face | b0 | c0;
@@ -148,7 +165,7 @@ __device__ inline void face_sites(const Geometry &g, const uint32_t *dep, const 
 // Merges bit updates that fall into the same 32-bit word: on return every
 // word appears once, in an entry with a non-zero mask; other masks are zero.
 template <int N>
-__device__ inline void merge_words(uint32_t (&w)[N], uint32_t (&m)[N])
+__device__ __forceinline__ void merge_words(uint32_t (&w)[N], uint32_t (&m)[N])
 {
 #pragma unroll
     for (int k = 1; k < N; ++k)
@@ -160,15 +177,29 @@ __device__ inline void merge_words(uint32_t (&w)[N], uint32_t (&m)[N])
             }
 }
 
-__device__ inline uint32_t bit_test(const uint32_t *words, uint32_t site)
+__device__ __forceinline__ uint32_t bit_test(const uint32_t *words, uint32_t site)
 {
     return (words[site >> 5] >> (site & 31u)) & 1u;
 }
 
-__device__ inline bool bond_forbidden(const uint32_t *bond, int dx, int dy, int dz)
+// 512-bit forbidden-bond mask as 8 64-bit words held in SGPRs; selected with
+// constant indices only.
+struct BondMask {
+    uint64_t w[8];
+};
+__device__ __forceinline__ bool bond_forbidden(const BondMask &m, int dx, int dy, int dz)
 {
     const uint32_t idx = uint32_t(dx & 7) | (uint32_t(dy & 7) << 3) | (uint32_t(dz & 7) << 6);
-    return (bond[idx >> 5] >> (idx & 31u)) & 1u;
+    const uint32_t k = idx >> 6;
+    uint64_t w = m.w[0];
+    w = k == 1u ? m.w[1] : w;
+    w = k == 2u ? m.w[2] : w;
+    w = k == 3u ? m.w[3] : w;
+    w = k == 4u ? m.w[4] : w;
+    w = k == 5u ? m.w[5] : w;
+    w = k == 6u ? m.w[6] : w;
+    w = k == 7u ? m.w[7] : w;
+    return (w >> (idx & 63u)) & 1u;
 }
 
 }  // namespace pscbfm

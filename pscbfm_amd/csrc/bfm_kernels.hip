@@ -13,34 +13,30 @@
 //  * conflicts are found with face-site claims: a survivor ORs its 4 entry
 //    sites into `claim`; bits that were already claimed are ORed into `coll`;
 //    phase 2 rejects a survivor with any of its sites in `coll`;
-//  * survivors are compacted per workgroup (wave64 ballot, mbcnt and an LDS
-//    prefix over the 4 waves), so phase 2 only runs over moves that can still
-//    happen (the reference's TODO, :1222-1228) and no global counter is
-//    contended;
+//  * survivors are compacted per wavefront (wave64 ballot + mbcnt), so phase 2
+//    only runs over moves that can still happen (the reference's TODO,
+//    :1222-1228), no global counter is contended and neither kernel needs a
+//    workgroup barrier (for cubic boxes they use no LDS at all);
 //  * no memset: claims are released by their owners in phase 2 and collision
-//    words are cleared from per-workgroup lists one substep later;
+//    words are cleared from per-wave lists one substep later;
 //  * an accepted move flips its 8 lattice bits with one atomic XOR per word.
 #include "bfm_kernels.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace pscbfm {
 namespace {
 
-__device__ inline void load_tables(const Tables *t, uint32_t *lds)
-{
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(t);
-    for (int k = threadIdx.x; k < kTableWords; k += blockDim.x)
-        lds[k] = src[k];
-}
-
 // Number of set bits of `mask` below the calling lane.
-__device__ inline uint32_t lanes_below(uint64_t mask)
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
 {
     return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
 }
 
-__device__ inline int4 moved(const int4 &r, int dir)
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ int4 moved(const int4 &r, int dir)
 {
     const int32_t s = (dir & 1) ? 1 : -1;
     int4 q = r;
@@ -51,49 +47,69 @@ __device__ inline int4 moved(const int4 &r, int dir)
     return q;
 }
 
-struct CheckShared {
-    uint32_t tables[kTableWords];
-    uint32_t wave[kBlock / 64];
-    uint32_t coll;
+// Morton tables of a non-cubic box, copied to LDS by every workgroup that
+// sweeps one (a file-scope __shared__ array keeps the LDS address space
+// visible to the compiler).
+__shared__ uint32_t s_dep[3 * 64];
+
+__device__ __forceinline__ void load_dep(const Tables *t)
+{
+    const uint32_t *src = t->dep[0];
+    for (int k = threadIdx.x; k < 3 * 64; k += blockDim.x)
+        s_dep[k] = src[k];
+}
+
+struct DepositTable {
+    __device__ __forceinline__ uint32_t operator()(int a, uint32_t v) const
+    {
+        return s_dep[a * 64 + (v & 31u)] | s_dep[a * 64 + 32 + (v >> 5)];
+    }
 };
 
-// Phase 1 for check block `blk` of `nblk` covering the colour segment
-// [begin, begin+count).
-template <bool kAllPeriodic>
-__device__ inline void check_body(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
-                                  uint32_t parity, uint32_t prev_blocks, uint32_t blk, uint32_t nblk,
-                                  CheckShared &sh)
-{
-    if (threadIdx.x == 0)
-        sh.coll = 0u;
-    load_tables(buf.tables, sh.tables);
-    __syncthreads();
-    const uint32_t *dep = sh.tables;
-    const uint32_t *bond = sh.tables + 3 * 64;
-    const Geometry &g = buf.geom;
+template <bool kCubic>
+using DepositFor = typename std::conditional<kCubic, DepositCubic, DepositTable>::type;
 
-    // Collision bits listed by the previous check launch (other parity):
-    // coll[parity^1] is not read in this kernel.
-    {
-        uint32_t *coll_prev = buf.coll[parity ^ 1u];
-        const uint32_t *list_prev = buf.clist[parity ^ 1u];
-        const uint32_t *cnt_prev = buf.ccount[parity ^ 1u];
-        for (uint32_t pb = blk; pb < prev_blocks; pb += nblk) {
-            const uint32_t n = cnt_prev[pb];
-            for (uint32_t k = threadIdx.x; k < n; k += kBlock)
-                coll_prev[list_prev[size_t(pb) * kCollSlots + k]] = 0u;
-        }
-    }
+// Phase 1 for check block `blk` of `nblk` covering the colour segment
+// [begin, begin+count).  Every wavefront compacts its own survivors and
+// collision words; prev_blocks is the grid of the previous check launch of
+// this system (other parity), whose collision words this launch clears.
+template <bool kAllPeriodic, bool kCubic>
+__device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
+                                  uint32_t parity, uint32_t prev_blocks, uint32_t blk, uint32_t nblk)
+{
+    const DepositFor<kCubic> dep{};
+    const Geometry &g = buf.geom;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t gwave = wave_uniform(blk * kWavesPerBlock + (threadIdx.x >> 6));
+    const uint32_t nwaves = nblk * kWavesPerBlock;
+    const uint32_t prev_waves = prev_blocks * kWavesPerBlock;
+
+    uint32_t *coll_cur = pick(parity, buf.coll0, buf.coll1);
+    uint32_t *list_cur = pick(parity, buf.clist0, buf.clist1);
+    uint32_t *coll_prev = pick(parity, buf.coll1, buf.coll0);
+    const uint32_t *list_prev = pick(parity, buf.clist1, buf.clist0);
+    const uint32_t *cnt_prev = pick(parity, buf.ccount1, buf.ccount0);
+    // collision count of the previous launch's wave with this index, issued early
+    const uint32_t n_prev_own = gwave < prev_waves ? cnt_prev[gwave] : 0u;
 
     const uint32_t t = blk * kBlock + threadIdx.x;
     bool pass = false;
     uint2 entry = make_uint2(0u, 0u);
+    uint32_t cword[4] = {0u, 0u, 0u, 0u};   // collision words of this lane
+    uint32_t ncoll = 0;
     if (t < count) {
         const uint32_t i = begin + t;
         const uint32_t p0 = buf.ppos[i];
+        const uint32_t id = buf.ids[i];
+        const uint32_t nn = buf.nnbr[i];
+        uint32_t partner[kMaxConnectivity];
+#pragma unroll
+        for (int j = 0; j < kMaxConnectivity; ++j)
+            if (uint32_t(j) < nn)
+                partner[j] = buf.nbr[size_t(j) * buf.n_pad + i];
+
         const int4 r0 = unpack_xyz(p0);
-        const uint32_t rn = philox_word0(buf.seed, buf.ids[i], age, kTagMove);
-        const int dir = int(rn % 6u);
+        const int dir = int(philox_word0(buf.seed, id, age, kTagMove) % 6u);
         const int4 r1 = moved(r0, dir);
 
         bool ok = true;
@@ -105,25 +121,20 @@ __device__ inline void check_body(const SweepBuffers &buf, uint32_t begin, uint3
                 ok = false;
         }
 
+        uint32_t s[4], d_face;
+        face_sites(g, dep, r0, dir, s, d_face);
+        const uint32_t *lat = buf.lattice;
+        const uint32_t l0 = lat[s[0] >> 5], l1 = lat[s[1] >> 5], l2 = lat[s[2] >> 5], l3 = lat[s[3] >> 5];
+
         // bond partners belong to other colours and do not move this substep
-        const uint32_t nn = buf.nnbr[i];
-        uint32_t partner[kMaxConnectivity];
-#pragma unroll
-        for (int j = 0; j < kMaxConnectivity; ++j)
-            if (uint32_t(j) < nn)
-                partner[j] = buf.nbr[size_t(j) * buf.n_pad + i];
 #pragma unroll
         for (int j = 0; j < kMaxConnectivity; ++j)
             if (uint32_t(j) < nn) {
                 const int4 q = unpack_xyz(buf.ppos[partner[j]]);
-                if (bond_forbidden(bond, q.x - r1.x, q.y - r1.y, q.z - r1.z))
+                if (bond_forbidden(buf.bond, q.x - r1.x, q.y - r1.y, q.z - r1.z))
                     ok = false;
             }
-
-        uint32_t s[4], d_face;
-        face_sites(g, dep, r0, dir, s, d_face);
-        const uint32_t *lat = buf.lattice;
-        if (bit_test(lat, s[0]) | bit_test(lat, s[1]) | bit_test(lat, s[2]) | bit_test(lat, s[3]))
+        if (((l0 >> (s[0] & 31u)) | (l1 >> (s[1] & 31u)) | (l2 >> (s[2] & 31u)) | (l3 >> (s[3] & 31u))) & 1u)
             ok = false;
 
         if (ok) {
@@ -139,9 +150,9 @@ __device__ inline void check_body(const SweepBuffers &buf, uint32_t begin, uint3
                 if (m[k]) {
                     const uint32_t hit = atomicOr(&buf.claim[w[k]], m[k]) & m[k];
                     if (hit) {
-                        atomicOr(&buf.coll[parity][w[k]], hit);
-                        const uint32_t slot = atomicAdd(&sh.coll, 1u);
-                        buf.clist[parity][size_t(blk) * kCollSlots + slot] = w[k];
+                        atomicOr(&coll_cur[w[k]], hit);
+                        cword[k] = w[k] | 0x80000000u;   // marks a listed word (site index < 2^30)
+                        ++ncoll;
                     }
                 }
             pass = true;
@@ -149,78 +160,80 @@ __device__ inline void check_body(const SweepBuffers &buf, uint32_t begin, uint3
         }
     }
 
-    // per-workgroup compaction: ballot within the wave, LDS prefix over waves
+    // wave compaction of the survivors
     const uint64_t mball = __ballot(pass);
-    const uint32_t wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63u) == 0)
-        sh.wave[wave] = uint32_t(__popcll(mball));
-    __syncthreads();
-    uint32_t base = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < kBlock / 64; ++k)
-        base += k < wave ? sh.wave[k] : 0u;
     if (pass)
-        buf.surv[size_t(blk) * kBlock + base + lanes_below(mball)] = entry;
-    if (threadIdx.x == 0) {
-        uint32_t total = 0;
+        buf.surv[size_t(gwave) * kWave + lanes_below(mball)] = entry;
+    // wave prefix of the collision words (0..4 per lane)
+    const uint64_t c0 = __ballot(ncoll & 1u), c1 = __ballot(ncoll & 2u), c2 = __ballot(ncoll & 4u);
+    uint32_t slot = lanes_below(c0) + 2u * lanes_below(c1) + 4u * lanes_below(c2);
 #pragma unroll
-        for (uint32_t k = 0; k < kBlock / 64; ++k)
-            total += sh.wave[k];
-        buf.scount[blk] = total;
-        buf.ccount[parity][blk] = sh.coll;
+    for (int k = 0; k < 4; ++k)
+        if (cword[k]) {
+            list_cur[size_t(gwave) * kCollSlotsWave + slot] = cword[k] & 0x7FFFFFFFu;
+            ++slot;
+        }
+    if (lane == 0) {
+        buf.scount[gwave] = uint32_t(__popcll(mball));
+        pick(parity, buf.ccount0, buf.ccount1)[gwave] =
+            uint32_t(__popcll(c0)) + 2u * uint32_t(__popcll(c1)) + 4u * uint32_t(__popcll(c2));
+    }
+
+    // collision bits listed by the previous check launch: nobody reads
+    // coll[parity^1] in this kernel
+    for (uint32_t pw = gwave; pw < prev_waves; pw += nwaves) {
+        const uint32_t n = pw == gwave ? n_prev_own : cnt_prev[pw];
+        for (uint32_t k = lane; k < n; k += kWave)
+            coll_prev[list_prev[size_t(pw) * kCollSlotsWave + k]] = 0u;
     }
 }
 
-template <bool kAllPeriodic>
+template <bool kAllPeriodic, bool kCubic>
 __global__ __launch_bounds__(kBlock) void k_check(SweepBuffers buf, uint32_t begin, uint32_t count, uint64_t age,
                                                   uint32_t parity, uint32_t prev_blocks)
 {
-    __shared__ CheckShared sh;
-    check_body<kAllPeriodic>(buf, begin, count, age, parity, prev_blocks, blockIdx.x, gridDim.x, sh);
+    if (!kCubic) {
+        load_dep(buf.tables);
+        __syncthreads();
+    }
+    check_body<kAllPeriodic, kCubic>(buf, begin, count, age, parity, prev_blocks, blockIdx.x, gridDim.x);
 }
 
-struct PerformShared {
-    uint32_t tables[kTableWords];
-    uint32_t pre[kPerformGroup + 1];
-    uint32_t acc;
-};
-
-// Phase 2 for perform block `blk`: the survivors of check blocks
-// [blk*kPerformGroup, (blk+1)*kPerformGroup) of a check launch of
-// check_blocks blocks.
-__device__ inline void perform_body(const SweepBuffers &buf, uint32_t check_blocks, uint32_t parity, uint32_t blk,
-                                    PerformShared &sh)
+// Phase 2: every wave of perform block `blk` takes the survivors of
+// kPerformGroup consecutive check waves of a check launch of check_blocks blocks.
+template <bool kCubic>
+__device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t check_blocks_, uint32_t parity,
+                                             uint32_t blk)
 {
-    uint32_t *s_pre = sh.pre;
-    load_tables(buf.tables, sh.tables);
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        s_pre[0] = 0;
-        for (uint32_t k = 0; k < kPerformGroup; ++k) {
-            const uint32_t cb = blk * kPerformGroup + k;
-            acc += cb < check_blocks ? buf.scount[cb] : 0u;
-            s_pre[k + 1] = acc;
-        }
-        sh.acc = 0;
-    }
-    __syncthreads();
-    const uint32_t *dep = sh.tables;
+    const DepositFor<kCubic> dep{};
     const Geometry &g = buf.geom;
-    const uint32_t *coll = buf.coll[parity];
-    const uint32_t total = s_pre[kPerformGroup];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t pwave = wave_uniform(blk * kWavesPerBlock + (threadIdx.x >> 6));
+    const uint32_t check_waves = check_blocks_ * kWavesPerBlock;
+    const uint32_t cw0 = pwave * kPerformGroup;
+    const uint32_t *coll = pick(parity, buf.coll0, buf.coll1);
 
-    for (uint32_t t = threadIdx.x; t < total; t += kBlock) {
-        uint32_t k = 0;
-#pragma unroll
-        for (uint32_t q = 1; q < kPerformGroup; ++q)
-            k += t >= s_pre[q] ? 1u : 0u;
-        const uint2 e = buf.surv[size_t(blk * kPerformGroup + k) * kBlock + (t - s_pre[k])];
+    uint32_t c = 0;
+    if (lane < kPerformGroup && cw0 + lane < check_waves)
+        c = buf.scount[cw0 + lane];
+    const uint32_t p1 = __shfl(c, 0);
+    const uint32_t p2 = p1 + __shfl(c, 1);
+    const uint32_t p3 = p2 + __shfl(c, 2);
+    const uint32_t total = p3 + __shfl(c, 3);
+
+    uint32_t acc = 0;
+    for (uint32_t t = lane; t < total; t += kWave) {
+        const uint32_t k = (t >= p1 ? 1u : 0u) + (t >= p2 ? 1u : 0u) + (t >= p3 ? 1u : 0u);
+        const uint32_t base = k == 0 ? 0u : (k == 1 ? p1 : (k == 2 ? p2 : p3));
+        const uint2 e = buf.surv[size_t(cw0 + k) * kWave + (t - base)];
         const uint32_t i = e.x & kSortedIndexMask;
         const int dir = int(e.x >> kSortedIndexBits);
         const int4 r0 = unpack_xyz(e.y);
         uint32_t s[4], d_face;
         face_sites(g, dep, r0, dir, s, d_face);
-        const bool clash = bit_test(coll, s[0]) | bit_test(coll, s[1]) | bit_test(coll, s[2]) | bit_test(coll, s[3]);
+        const uint32_t k0 = coll[s[0] >> 5], k1 = coll[s[1] >> 5], k2 = coll[s[2] >> 5], k3 = coll[s[3] >> 5];
+        const bool clash =
+            (((k0 >> (s[0] & 31u)) | (k1 >> (s[1] & 31u)) | (k2 >> (s[2] & 31u)) | (k3 >> (s[3] & 31u))) & 1u) != 0;
 
         uint32_t w[8], m[8];
 #pragma unroll
@@ -267,24 +280,22 @@ __device__ inline void perform_body(const SweepBuffers &buf, uint32_t check_bloc
             if ((dir & 1) ? v == 0u : v == am)
                 buf.image[size_t(a) * buf.n_pad + i] += (dir & 1) ? 1 : -1;
         }
-        const uint64_t acc = __ballot(!clash);
-        if ((threadIdx.x & 63u) == uint32_t(__ffsll((unsigned long long)__ballot(1)) - 1))
-            atomicAdd(&sh.acc, uint32_t(__popcll(acc)));
+        acc += uint32_t(__popcll(__ballot(!clash)));
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t slot = blk & (kStatSlots - 1);
-        if (sh.acc)
-            atomicAdd(&buf.accepted[slot], (unsigned long long)sh.acc);
-        if (total)
-            atomicAdd(&buf.proposals[slot], (unsigned long long)total);
+    if (lane == 0 && total) {
+        atomicAdd(&buf.accepted[pwave & (kStatSlots - 1)], (unsigned long long)acc);
+        atomicAdd(&buf.proposals[pwave & (kStatSlots - 1)], (unsigned long long)total);
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t check_blocks, uint32_t parity)
+template <bool kCubic>
+__global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t check_blocks_, uint32_t parity)
 {
-    __shared__ PerformShared sh;
-    perform_body(buf, check_blocks, parity, blockIdx.x, sh);
+    if (!kCubic) {
+        load_dep(buf.tables);
+        __syncthreads();
+    }
+    perform_body<kCubic>(buf, check_blocks_, parity, blockIdx.x);
 }
 
 // ---- many systems per launch ------------------------------------------------
@@ -292,10 +303,11 @@ __global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t c
 // blocks start at its check_base / perform_base.  The schedule says which
 // colour (if any) each system sweeps in this substep slot.
 
-template <bool kAllPeriodic>
-__device__ inline void check_batch_body(const BatchSystem *__restrict__ sys, const uint32_t *__restrict__ blk_sys,
-                                        const SchedEntry *__restrict__ sched, uint32_t n_sys, uint32_t slot,
-                                        CheckShared &sh)
+template <bool kAllPeriodic, bool kCubic>
+__global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__restrict__ sys,
+                                                        const uint32_t *__restrict__ blk_sys,
+                                                        const SchedEntry *__restrict__ sched, uint32_t n_sys,
+                                                        uint32_t slot)
 {
     const uint32_t s = blk_sys[blockIdx.x];
     const SchedEntry e = sched[size_t(slot) * n_sys + s];
@@ -307,27 +319,19 @@ __device__ inline void check_batch_body(const BatchSystem *__restrict__ sys, con
     const uint32_t blk = blockIdx.x - S.check_base;
     if (blk >= nb)
         return;
-    check_body<kAllPeriodic>(S.buf, S.seg_begin[e.colour], count, e.age, e.parity, e.prev_blocks, blk, nb, sh);
+    if (!kCubic) {
+        load_dep(S.buf.tables);
+        __syncthreads();
+    }
+    check_body<kAllPeriodic, kCubic>(S.buf, S.seg_begin[e.colour], count, e.age, e.parity, e.prev_blocks, blk, nb);
 }
 
-__global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__restrict__ sys,
-                                                        const uint32_t *__restrict__ blk_sys,
-                                                        const SchedEntry *__restrict__ sched, uint32_t n_sys,
-                                                        uint32_t slot)
-{
-    __shared__ CheckShared sh;
-    if (sys[blk_sys[blockIdx.x]].buf.geom.periodic == 7u)
-        check_batch_body<true>(sys, blk_sys, sched, n_sys, slot, sh);
-    else
-        check_batch_body<false>(sys, blk_sys, sched, n_sys, slot, sh);
-}
-
+template <bool kCubic>
 __global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__restrict__ sys,
                                                           const uint32_t *__restrict__ blk_sys,
                                                           const SchedEntry *__restrict__ sched, uint32_t n_sys,
                                                           uint32_t slot)
 {
-    __shared__ PerformShared sh;
     const uint32_t s = blk_sys[blockIdx.x];
     const SchedEntry e = sched[size_t(slot) * n_sys + s];
     if (e.colour == kNoColour)
@@ -335,50 +339,14 @@ __global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__r
     const BatchSystem &S = sys[s];
     const uint32_t cb = check_blocks(S.seg_count[e.colour]);
     const uint32_t blk = blockIdx.x - S.perform_base;
-    if (blk >= (cb + kPerformGroup - 1) / kPerformGroup)
+    if (blk >= perform_blocks(cb))
         return;
-    perform_body(S.buf, cb, e.parity, blk, sh);
+    if (!kCubic) {
+        load_dep(S.buf.tables);
+        __syncthreads();
+    }
+    perform_body<kCubic>(S.buf, cb, e.parity, blk);
 }
-
-}  // namespace
-
-hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t parity,
-                        uint32_t prev_blocks, hipStream_t stream)
-{
-    const dim3 grid(check_blocks(count)), block(kBlock);
-    if (buf.geom.periodic == 7u)
-        hipLaunchKernelGGL(k_check<true>, grid, block, 0, stream, buf, begin, count, age, parity, prev_blocks);
-    else
-        hipLaunchKernelGGL(k_check<false>, grid, block, 0, stream, buf, begin, count, age, parity, prev_blocks);
-    return hipGetLastError();
-}
-
-hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t parity, hipStream_t stream)
-{
-    const uint32_t cb = check_blocks(count);
-    const uint32_t pb = (cb + kPerformGroup - 1) / kPerformGroup;
-    hipLaunchKernelGGL(k_perform, dim3(pb), dim3(kBlock), 0, stream, buf, cb, parity);
-    return hipGetLastError();
-}
-
-hipError_t launch_check_batch(const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
-                              const SchedEntry *sched, uint32_t n_sys, uint32_t slot, hipStream_t stream)
-{
-    hipLaunchKernelGGL(k_check_batch, dim3(n_blocks), dim3(kBlock), 0, stream, sys, blk_sys, sched, n_sys, slot);
-    return hipGetLastError();
-}
-
-hipError_t launch_perform_batch(const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
-                                const SchedEntry *sched, uint32_t n_sys, uint32_t slot, hipStream_t stream)
-{
-    hipLaunchKernelGGL(k_perform_batch, dim3(n_blocks), dim3(kBlock), 0, stream, sys, blk_sys, sched, n_sys, slot);
-    return hipGetLastError();
-}
-
-}  // namespace pscbfm
-
-namespace pscbfm {
-namespace {
 
 // Plain streaming copy, 16 bytes per lane per iteration: the reference point
 // for HBM counters (a copy of B bytes reads B and writes B).
@@ -389,6 +357,67 @@ __global__ __launch_bounds__(kBlock) void k_copy(const uint4 *__restrict__ src, 
 }
 
 }  // namespace
+
+hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t parity,
+                        uint32_t prev_blocks, hipStream_t stream)
+{
+    const dim3 grid(check_blocks(count)), block(kBlock);
+    const bool periodic = buf.geom.periodic == 7u, cubic = buf.geom.cubic != 0u;
+    if (periodic && cubic)
+        hipLaunchKernelGGL((k_check<true, true>), grid, block, 0, stream, buf, begin, count, age, parity, prev_blocks);
+    else if (periodic)
+        hipLaunchKernelGGL((k_check<true, false>), grid, block, 0, stream, buf, begin, count, age, parity,
+                           prev_blocks);
+    else if (cubic)
+        hipLaunchKernelGGL((k_check<false, true>), grid, block, 0, stream, buf, begin, count, age, parity,
+                           prev_blocks);
+    else
+        hipLaunchKernelGGL((k_check<false, false>), grid, block, 0, stream, buf, begin, count, age, parity,
+                           prev_blocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t parity, hipStream_t stream)
+{
+    const uint32_t cb = check_blocks(count);
+    if (buf.geom.cubic != 0u)
+        hipLaunchKernelGGL((k_perform<true>), dim3(perform_blocks(cb)), dim3(kBlock), 0, stream, buf, cb, parity);
+    else
+        hipLaunchKernelGGL((k_perform<false>), dim3(perform_blocks(cb)), dim3(kBlock), 0, stream, buf, cb, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
+                              const SchedEntry *sched, uint32_t n_sys, uint32_t slot, hipStream_t stream)
+{
+    const dim3 grid(n_blocks), block(kBlock);
+    switch (cls) {
+    case kClassPeriodicCubic:
+        hipLaunchKernelGGL((k_check_batch<true, true>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot);
+        break;
+    case kClassPeriodic:
+        hipLaunchKernelGGL((k_check_batch<true, false>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot);
+        break;
+    case kClassCubic:
+        hipLaunchKernelGGL((k_check_batch<false, true>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot);
+        break;
+    default:
+        hipLaunchKernelGGL((k_check_batch<false, false>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_perform_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
+                                const SchedEntry *sched, uint32_t n_sys, uint32_t slot, hipStream_t stream)
+{
+    const dim3 grid(n_blocks), block(kBlock);
+    if (cls == kClassPeriodicCubic || cls == kClassCubic)
+        hipLaunchKernelGGL((k_perform_batch<true>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot);
+    else
+        hipLaunchKernelGGL((k_perform_batch<false>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot);
+    return hipGetLastError();
+}
 
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream)
 {

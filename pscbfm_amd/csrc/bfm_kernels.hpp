@@ -7,9 +7,10 @@
 
 namespace pscbfm {
 
-constexpr uint32_t kPerformGroup = 4;   // check blocks whose survivors one perform block takes
-constexpr uint32_t kCollSlots = 4 * kBlock;  // collision-list slots of one check block
-constexpr uint32_t kStatSlots = 256;    // spread of the accepted / proposed counters
+constexpr uint32_t kPerformGroup = 4;         // check waves whose survivors one perform wave takes
+constexpr uint32_t kCollSlotsWave = 4 * kWave;  // collision-list slots of one check wave
+constexpr uint32_t kCollSlots = 4 * kBlock;     // ... of one check block (all its waves)
+constexpr uint32_t kStatSlots = 256;           // spread of the accepted / proposed counters
 
 // Everything one substep of one system needs on the device.
 struct SweepBuffers {
@@ -21,19 +22,30 @@ struct SweepBuffers {
     uint32_t n_pad;
     uint32_t *lattice;
     uint32_t *claim;
-    uint32_t *coll[2];
-    uint32_t *clist[2];       // [check block][kCollSlots] collision words
-    uint32_t *ccount[2];      // [check block] entries in clist
-    uint2 *surv;              // [check block][kBlock] compacted proposals
-    uint32_t *scount;         // [check block] proposals
+    // double-buffered by substep parity; selected with pick(), never indexed
+    // (a runtime index into an argument struct forces it into scratch memory)
+    uint32_t *coll0, *coll1;
+    uint32_t *clist0, *clist1;    // [check wave][kCollSlotsWave] collision words
+    uint32_t *ccount0, *ccount1;  // [check wave] entries in clist
+    uint2 *surv;              // [check wave][kWave] compacted proposals
+    uint32_t *scount;         // [check wave] proposals
     unsigned long long *accepted;   // [kStatSlots]
     unsigned long long *proposals;  // [kStatSlots]
-    const Tables *tables;
+    const Tables *tables;     // Morton tables (read only for non-cubic boxes)
+    BondMask bond;            // forbidden-bond mask
     Geometry geom;
     uint64_t seed;
 };
 
+template <class T>
+__host__ __device__ inline T pick(uint32_t parity, T even, T odd)
+{
+    return parity ? odd : even;
+}
+
 __host__ __device__ inline uint32_t check_blocks(uint32_t count) { return count == 0 ? 1u : (count + kBlock - 1) / kBlock; }
+// perform blocks that cover the survivors of a check launch of cb blocks
+__host__ __device__ inline uint32_t perform_blocks(uint32_t cb) { return (cb + kPerformGroup - 1) / kPerformGroup; }
 
 // Phase 1 of a substep of parity `parity` for the colour segment
 // [begin, begin+count): every monomer proposes a move and checks it against
@@ -69,12 +81,21 @@ struct SchedEntry {
     uint32_t pad;
 };
 
-// Phase 1 / phase 2 of one substep slot for every system of a batch.
-// n_blocks = sum over systems of their reserved blocks; blk_sys maps a block
-// to its system; sched holds n_sys entries per slot.
-hipError_t launch_check_batch(const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
+// Kernel variant class of a system: periodic on every axis, cubic box.
+enum : uint32_t { kClassPeriodicCubic = 0, kClassPeriodic = 1, kClassCubic = 2, kClassOther = 3, kClasses = 4 };
+inline uint32_t kernel_class(const Geometry &g)
+{
+    const bool p = g.periodic == 7u, c = g.cubic != 0u;
+    return p && c ? kClassPeriodicCubic : (p ? kClassPeriodic : (c ? kClassCubic : kClassOther));
+}
+
+// Phase 1 / phase 2 of one substep slot for every system of one class of a
+// batch.  n_blocks = sum over the class's systems of their reserved blocks;
+// blk_sys maps a block to its system (index into sys and into each slot's
+// n_sys schedule entries).
+hipError_t launch_check_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
                               const SchedEntry *sched, uint32_t n_sys, uint32_t slot, hipStream_t stream);
-hipError_t launch_perform_batch(const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
+hipError_t launch_perform_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
                                 const SchedEntry *sched, uint32_t n_sys, uint32_t slot, hipStream_t stream);
 
 // Streaming copy of `bytes` (a multiple of 16) from src to dst.

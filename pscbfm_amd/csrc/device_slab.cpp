@@ -22,10 +22,10 @@ SlabLayout slab_layout(const HostSystem &s)
     L.bytes[kPartColl1] = sizeof(uint32_t) * s.lattice_words;
     L.bytes[kPartClist0] = sizeof(uint32_t) * kCollSlots * blocks;
     L.bytes[kPartClist1] = sizeof(uint32_t) * kCollSlots * blocks;
-    L.bytes[kPartCcount0] = sizeof(uint32_t) * blocks;
-    L.bytes[kPartCcount1] = sizeof(uint32_t) * blocks;
+    L.bytes[kPartCcount0] = sizeof(uint32_t) * kWavesPerBlock * blocks;
+    L.bytes[kPartCcount1] = sizeof(uint32_t) * kWavesPerBlock * blocks;
     L.bytes[kPartSurv] = sizeof(uint2) * kBlock * blocks;
-    L.bytes[kPartScount] = sizeof(uint32_t) * blocks;
+    L.bytes[kPartScount] = sizeof(uint32_t) * kWavesPerBlock * blocks;
     L.bytes[kPartAccepted] = sizeof(unsigned long long) * kStatSlots;
     L.bytes[kPartProposals] = sizeof(unsigned long long) * kStatSlots;
     L.bytes[kPartTables] = sizeof(Tables);
@@ -65,17 +65,19 @@ SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, u
     b.n_pad = s.n_pad;
     b.lattice = static_cast<uint32_t *>(slab_part(slab, L, kPartLattice));
     b.claim = static_cast<uint32_t *>(slab_part(slab, L, kPartClaim));
-    b.coll[0] = static_cast<uint32_t *>(slab_part(slab, L, kPartColl0));
-    b.coll[1] = static_cast<uint32_t *>(slab_part(slab, L, kPartColl1));
-    b.clist[0] = static_cast<uint32_t *>(slab_part(slab, L, kPartClist0));
-    b.clist[1] = static_cast<uint32_t *>(slab_part(slab, L, kPartClist1));
-    b.ccount[0] = static_cast<uint32_t *>(slab_part(slab, L, kPartCcount0));
-    b.ccount[1] = static_cast<uint32_t *>(slab_part(slab, L, kPartCcount1));
+    b.coll0 = static_cast<uint32_t *>(slab_part(slab, L, kPartColl0));
+    b.coll1 = static_cast<uint32_t *>(slab_part(slab, L, kPartColl1));
+    b.clist0 = static_cast<uint32_t *>(slab_part(slab, L, kPartClist0));
+    b.clist1 = static_cast<uint32_t *>(slab_part(slab, L, kPartClist1));
+    b.ccount0 = static_cast<uint32_t *>(slab_part(slab, L, kPartCcount0));
+    b.ccount1 = static_cast<uint32_t *>(slab_part(slab, L, kPartCcount1));
     b.surv = static_cast<uint2 *>(slab_part(slab, L, kPartSurv));
     b.scount = static_cast<uint32_t *>(slab_part(slab, L, kPartScount));
     b.accepted = static_cast<unsigned long long *>(slab_part(slab, L, kPartAccepted));
     b.proposals = static_cast<unsigned long long *>(slab_part(slab, L, kPartProposals));
     b.tables = static_cast<const Tables *>(slab_part(slab, L, kPartTables));
+    for (int k = 0; k < 8; ++k)
+        b.bond.w[k] = uint64_t(s.tables.bond[2 * k]) | (uint64_t(s.tables.bond[2 * k + 1]) << 32);
     b.geom = s.geom;
     b.seed = seed;
     return b;

@@ -97,6 +97,7 @@ void make_geometry(const uint64_t box[3], const bool periodic[3], Geometry &g, T
         return d;
     };
     g.periodic = 0;
+    g.cubic = (box[0] == box[1] && box[1] == box[2]) ? 1u : 0u;
     for (int a = 0; a < 3; ++a) {
         g.mask[a] = uint32_t(box[a] - 1);
         g.dmask[a] = dep(a, g.mask[a]);
