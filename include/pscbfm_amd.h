@@ -94,6 +94,14 @@ int bfm_set_seed(bfm_updater *h, uint64_t seed);
  * RNG is keyed on the original monomer id. */
 int bfm_set_steps_between_sortings(bfm_updater *h, int64_t steps);
 
+/* Layout of each colour segment (no counterpart in the reference's API; its
+ * doSpatialSorting, UpdaterGPUScBFM_AB_Type.cu:1836-1935, is always on).  On
+ * (the default), monomers of a colour are stored in the lattice's Morton
+ * order of their positions, so a wavefront sweeps a compact patch of space;
+ * off, in order of id.  The trajectory is the same either way.  Before
+ * initialize / bfm_validate only. */
+int bfm_set_spatial_sorting(bfm_updater *h, int on);
+
 /* The host half of initialize (no device work): the checks of checkSystem
  * (UpdaterGPUScBFM_AB_Type.cu:2789-2900) and the bond-set count (:1526), then
  * the colouring of initializeSpeciesSorting (:1561-1686).  Lets a caller
@@ -147,6 +155,13 @@ int bfm_get_stats(const bfm_updater *h, bfm_stats *out);
  * handle's stream and the durations are summed into bfm_stats. */
 int bfm_set_kernel_timing(bfm_updater *h, int on);
 
+/* Block-to-XCD placement of the sweep kernels (no counterpart in the
+ * reference).  On (the default), each of the 8 XCDs sweeps one contiguous
+ * range of the colour segment, so spatially close monomers share one L2;
+ * off, blocks are dealt to the XCDs round-robin as launched.  The trajectory
+ * is the same either way. */
+int bfm_set_xcd_ranges(bfm_updater *h, int on);
+
 /* Runs n_mcs steps without the copy-back of runSimulationOnGPU (positions
  * stay on the device; the age still advances).  For timing loops. */
 int bfm_run_resident(bfm_updater *h, uint32_t n_mcs);
@@ -180,6 +195,9 @@ int bfm_batch_get_age(bfm_batch *b, uint32_t member, int64_t *age);
 int bfm_batch_get_stats(bfm_batch *b, uint32_t member, bfm_stats *out);
 int bfm_batch_remove(bfm_batch *b, uint32_t member);
 int bfm_batch_set_graphs(bfm_batch *b, int on);
+/* As bfm_set_xcd_ranges, over the batch's whole grid (each member's blocks
+ * stay on one or two XCDs). */
+int bfm_batch_set_xcd_ranges(bfm_batch *b, int on);
 int bfm_batch_synchronize(bfm_batch *b);
 
 /* ---- served form: continuous batching of simulation jobs --------------------

@@ -99,6 +99,8 @@ SIGNATURES = {
     "bfm_get_colors": (c_int, [_H, c_void_p, c_uint32, POINTER(c_uint32)]),
     "bfm_get_stats": (c_int, [_H, POINTER(BfmStats)]),
     "bfm_set_kernel_timing": (c_int, [_H, c_int]),
+    "bfm_set_spatial_sorting": (c_int, [_H, c_int]),
+    "bfm_set_xcd_ranges": (c_int, [_H, c_int]),
     "bfm_run_resident": (c_int, [_H, c_uint32]),
     "bfm_copy_back": (c_int, [_H]),
     "bfm_batch_create": (c_int, [POINTER(c_void_p), c_int]),
@@ -112,6 +114,7 @@ SIGNATURES = {
     "bfm_batch_get_stats": (c_int, [_H, c_uint32, POINTER(BfmStats)]),
     "bfm_batch_remove": (c_int, [_H, c_uint32]),
     "bfm_batch_set_graphs": (c_int, [_H, c_int]),
+    "bfm_batch_set_xcd_ranges": (c_int, [_H, c_int]),
     "bfm_batch_synchronize": (c_int, [_H]),
     "bfm_server_create": (c_int, [POINTER(c_void_p), c_int, c_uint64, c_uint32]),
     "bfm_server_destroy": (c_int, [_H]),

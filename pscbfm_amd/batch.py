@@ -72,6 +72,9 @@ class Batch:
     def set_graphs(self, on: bool) -> None:
         _capi.check(self._lib.bfm_batch_set_graphs(self._h, int(bool(on))))
 
+    def set_xcd_ranges(self, on: bool) -> None:
+        _capi.check(self._lib.bfm_batch_set_xcd_ranges(self._h, int(bool(on))))
+
     def synchronize(self) -> None:
         _capi.check(self._lib.bfm_batch_synchronize(self._h))
 

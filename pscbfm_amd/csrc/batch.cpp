@@ -223,10 +223,10 @@ void BatchEngine::runChunk(uint32_t mcs)
                 if (!mCheckBlocks[c])
                     continue;
                 hip_check(launch_check_batch(c, mDevSys, mDevCheckBlk[c], mCheckBlocks[c], mDevSched[b], n_sys, slot,
-                                             mStream),
+                                             mXcdRanges ? 1u : 0u, mStream),
                           "batched check kernel");
                 hip_check(launch_perform_batch(c, mDevSys, mDevPerformBlk[c], mPerformBlocks[c], mDevSched[b], n_sys,
-                                               slot, mStream),
+                                               slot, mXcdRanges ? 1u : 0u, mStream),
                           "batched perform kernel");
             }
     };
@@ -235,7 +235,8 @@ void BatchEngine::runChunk(uint32_t mcs)
     const bool reuse = mRanVersion == mCompVersion && mRanMcs == mcs;
     mRanVersion = mCompVersion;
     mRanMcs = mcs;
-    const bool have = mGraph[b] && mGraphVersion[b] == mCompVersion && mGraphMcs[b] == mcs;
+    const bool have = mGraph[b] && mGraphVersion[b] == mCompVersion && mGraphMcs[b] == mcs &&
+                      mGraphXcd[b] == mXcdRanges;
     if (mUseGraphs && (have || reuse)) {
         if (!have) {
             if (mGraph[b]) {
@@ -258,6 +259,7 @@ void BatchEngine::runChunk(uint32_t mcs)
             hip_check(e, "hipGraphInstantiate");
             mGraphVersion[b] = mCompVersion;
             mGraphMcs[b] = mcs;
+            mGraphXcd[b] = mXcdRanges;
         }
         hip_check(hipGraphLaunch(mGraph[b], mStream), "hipGraphLaunch");
         ++mGraphLaunches;

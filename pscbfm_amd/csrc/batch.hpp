@@ -56,6 +56,7 @@ public:
     void counters(uint32_t id, uint64_t &accepted, uint64_t &proposals);
     hipStream_t stream() const { return mStream; }
     void setUseGraphs(bool on) { mUseGraphs = on; }
+    void setXcdRanges(bool on) { mXcdRanges = on; }
     uint64_t graphLaunches() const { return mGraphLaunches; }
     uint64_t kernelLaunches() const { return mKernelLaunches; }
     void synchronize();
@@ -105,6 +106,8 @@ private:
     hipGraphExec_t mGraph[2] = {nullptr, nullptr};
     uint32_t mGraphMcs[2] = {0, 0};
     uint64_t mGraphVersion[2] = {~0ull, ~0ull};
+    bool mGraphXcd[2] = {false, false};
+    bool mXcdRanges = true;
     uint64_t mGraphLaunches = 0, mKernelLaunches = 0;
     uint64_t mCompVersion = 0;
     uint64_t mRanVersion = ~0ull;

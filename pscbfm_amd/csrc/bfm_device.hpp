@@ -4,13 +4,13 @@
 // index: monomers grouped by colour, each colour segment padded to a multiple
 // of kSegmentAlign so that every colour starts on an aligned boundary):
 //   uint32_t ppos[nPad]        wrapped x | y << 10 | z << 20 (box edges <= 1024)
-//   uint32_t ids[nPad]         original monomer id (keys the RNG)
+//   uint32_t ids[nPad]         original monomer id (keys the RNG) | number of
+//                              bond partners << 28
 //   int32_t  image[3][nPad]    periodic image of each axis: unwrapped = wrapped
 //                              + L * image; changed only when a move crosses
 //                              the box boundary
 //   uint32_t nbr[8][nPad]      sorted index of the j-th bond partner (slot-major,
 //                              so lane i reads nbr[j*nPad+i] coalesced)
-//   uint8_t  nnbr[nPad]        number of bond partners
 //   uint32_t lattice[W]        occupancy bit lattice: the 8 sites of every
 //                              monomer cube, compact Morton order
 //   uint32_t claim[W]          sites entered by this substep's proposals
@@ -41,6 +41,10 @@ constexpr int kWave = 64;                // lanes per wavefront
 constexpr uint32_t kWavesPerBlock = kBlock / kWave;
 constexpr uint32_t kSortedIndexBits = 29;
 constexpr uint32_t kSortedIndexMask = (1u << kSortedIndexBits) - 1u;
+// The id word of a sorted slot holds the original monomer id in its low
+// kIdBits bits and the number of bond partners (0..kMaxConnectivity) above.
+constexpr uint32_t kIdBits = 28;
+constexpr uint32_t kIdMask = (1u << kIdBits) - 1u;
 constexpr uint32_t kMaxBox = 1024;       // 10 bits per axis, 30-bit site index
 constexpr uint32_t kCoordBits = 10;      // bits per axis of a packed position
 constexpr uint32_t kCoordMask = (1u << kCoordBits) - 1u;
@@ -182,24 +186,10 @@ __device__ __forceinline__ uint32_t bit_test(const uint32_t *words, uint32_t sit
     return (words[site >> 5] >> (site & 31u)) & 1u;
 }
 
-// 512-bit forbidden-bond mask as 8 64-bit words held in SGPRs; selected with
-// constant indices only.
-struct BondMask {
-    uint64_t w[8];
-};
-__device__ __forceinline__ bool bond_forbidden(const BondMask &m, int dx, int dy, int dz)
+// Bond-table index of a bond vector: (dx & 7) | (dy & 7) << 3 | (dz & 7) << 6.
+__host__ __device__ __forceinline__ uint32_t bond_index(int dx, int dy, int dz)
 {
-    const uint32_t idx = uint32_t(dx & 7) | (uint32_t(dy & 7) << 3) | (uint32_t(dz & 7) << 6);
-    const uint32_t k = idx >> 6;
-    uint64_t w = m.w[0];
-    w = k == 1u ? m.w[1] : w;
-    w = k == 2u ? m.w[2] : w;
-    w = k == 3u ? m.w[3] : w;
-    w = k == 4u ? m.w[4] : w;
-    w = k == 5u ? m.w[5] : w;
-    w = k == 6u ? m.w[6] : w;
-    w = k == 7u ? m.w[7] : w;
-    return (w >> (idx & 63u)) & 1u;
+    return uint32_t(dx & 7) | (uint32_t(dy & 7) << 3) | (uint32_t(dz & 7) << 6);
 }
 
 }  // namespace pscbfm

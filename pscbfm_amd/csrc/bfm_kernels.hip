@@ -15,8 +15,8 @@
 //    phase 2 rejects a survivor with any of its sites in `coll`;
 //  * survivors are compacted per wavefront (wave64 ballot + mbcnt), so phase 2
 //    only runs over moves that can still happen (the reference's TODO,
-//    :1222-1228), no global counter is contended and neither kernel needs a
-//    workgroup barrier (for cubic boxes they use no LDS at all);
+//    :1222-1228) and no global counter is contended; the only workgroup
+//    barrier is the one after the LDS tables are filled at kernel start;
 //  * no memset: claims are released by their owners in phase 2 and collision
 //    words are cleared from per-wave lists one substep later;
 //  * an accepted move flips its 8 lattice bits with one atomic XOR per word.
@@ -47,16 +47,29 @@ __device__ __forceinline__ int4 moved(const int4 &r, int dir)
     return q;
 }
 
-// Morton tables of a non-cubic box, copied to LDS by every workgroup that
-// sweeps one (a file-scope __shared__ array keeps the LDS address space
-// visible to the compiler).
+// Per-system tables in LDS (file-scope __shared__ arrays keep the LDS address
+// space visible to the compiler): the 512-bit forbidden-bond mask, read by
+// every check workgroup, and the Morton tables of a non-cubic box.
+__shared__ uint32_t s_bond[16];
 __shared__ uint32_t s_dep[3 * 64];
+
+__device__ __forceinline__ void load_bond(const Tables *t)
+{
+    if (threadIdx.x < 16)
+        s_bond[threadIdx.x] = t->bond[threadIdx.x];
+}
 
 __device__ __forceinline__ void load_dep(const Tables *t)
 {
     const uint32_t *src = t->dep[0];
     for (int k = threadIdx.x; k < 3 * 64; k += blockDim.x)
         s_dep[k] = src[k];
+}
+
+__device__ __forceinline__ bool bond_forbidden(int dx, int dy, int dz)
+{
+    const uint32_t b = bond_index(dx, dy, dz);
+    return (s_bond[b >> 5] >> (b & 31u)) & 1u;
 }
 
 struct DepositTable {
@@ -100,12 +113,18 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
     if (t < count) {
         const uint32_t i = begin + t;
         const uint32_t p0 = buf.ppos[i];
-        const uint32_t id = buf.ids[i];
-        const uint32_t nn = buf.nnbr[i];
+        const uint32_t idn = buf.ids[i];
+        const uint32_t id = idn & kIdMask, nn = idn >> kIdBits;
+        // partners of the wave's busiest lane; lanes with fewer read their
+        // own slot instead (in bounds, and the result is unused)
+        uint32_t nn_wave = 0;
+#pragma unroll
+        for (int j = 0; j < kMaxConnectivity; ++j)
+            nn_wave = __ballot(nn > uint32_t(j)) ? uint32_t(j + 1) : nn_wave;
         uint32_t partner[kMaxConnectivity];
 #pragma unroll
         for (int j = 0; j < kMaxConnectivity; ++j)
-            if (uint32_t(j) < nn)
+            if (uint32_t(j) < nn_wave)
                 partner[j] = buf.nbr[size_t(j) * buf.n_pad + i];
 
         const int4 r0 = unpack_xyz(p0);
@@ -126,35 +145,44 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
         const uint32_t *lat = buf.lattice;
         const uint32_t l0 = lat[s[0] >> 5], l1 = lat[s[1] >> 5], l2 = lat[s[2] >> 5], l3 = lat[s[3] >> 5];
 
-        // bond partners belong to other colours and do not move this substep
+        // bond partners belong to other colours and do not move this substep;
+        // every position load is issued before the first is used
+        uint32_t qpos[kMaxConnectivity];
 #pragma unroll
         for (int j = 0; j < kMaxConnectivity; ++j)
-            if (uint32_t(j) < nn) {
-                const int4 q = unpack_xyz(buf.ppos[partner[j]]);
-                if (bond_forbidden(buf.bond, q.x - r1.x, q.y - r1.y, q.z - r1.z))
+            if (uint32_t(j) < nn_wave)
+                qpos[j] = buf.ppos[uint32_t(j) < nn ? partner[j] : i];
+#pragma unroll
+        for (int j = 0; j < kMaxConnectivity; ++j)
+            if (uint32_t(j) < nn_wave && uint32_t(j) < nn) {
+                const int4 q = unpack_xyz(qpos[j]);
+                if (bond_forbidden(q.x - r1.x, q.y - r1.y, q.z - r1.z))
                     ok = false;
             }
         if (((l0 >> (s[0] & 31u)) | (l1 >> (s[1] & 31u)) | (l2 >> (s[2] & 31u)) | (l3 >> (s[3] & 31u))) & 1u)
             ok = false;
 
         if (ok) {
-            uint32_t w[4], m[4];
+            uint32_t w[4], m[4], old[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 w[k] = s[k] >> 5;
                 m[k] = 1u << (s[k] & 31u);
             }
             merge_words(w, m);
+            // all claims in flight before the first result is used
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                if (m[k]) {
-                    const uint32_t hit = atomicOr(&buf.claim[w[k]], m[k]) & m[k];
-                    if (hit) {
-                        atomicOr(&coll_cur[w[k]], hit);
-                        cword[k] = w[k] | 0x80000000u;   // marks a listed word (site index < 2^30)
-                        ++ncoll;
-                    }
+                old[k] = m[k] ? atomicOr(&buf.claim[w[k]], m[k]) : 0u;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t hit = old[k] & m[k];
+                if (hit) {
+                    atomicOr(&coll_cur[w[k]], hit);
+                    cword[k] = w[k] | 0x80000000u;   // marks a listed word (site index < 2^30)
+                    ++ncoll;
                 }
+            }
             pass = true;
             entry = make_uint2(i | (uint32_t(dir) << kSortedIndexBits), p0);
         }
@@ -192,11 +220,12 @@ template <bool kAllPeriodic, bool kCubic>
 __global__ __launch_bounds__(kBlock) void k_check(SweepBuffers buf, uint32_t begin, uint32_t count, uint64_t age,
                                                   uint32_t parity, uint32_t prev_blocks)
 {
-    if (!kCubic) {
+    load_bond(buf.tables);
+    if (!kCubic)
         load_dep(buf.tables);
-        __syncthreads();
-    }
-    check_body<kAllPeriodic, kCubic>(buf, begin, count, age, parity, prev_blocks, blockIdx.x, gridDim.x);
+    __syncthreads();
+    const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
+    check_body<kAllPeriodic, kCubic>(buf, begin, count, age, parity, prev_blocks, blk, gridDim.x);
 }
 
 // Phase 2: every wave of perform block `blk` takes the survivors of
@@ -295,7 +324,8 @@ __global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t c
         load_dep(buf.tables);
         __syncthreads();
     }
-    perform_body<kCubic>(buf, check_blocks_, parity, blockIdx.x);
+    const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
+    perform_body<kCubic>(buf, check_blocks_, parity, blk);
 }
 
 // ---- many systems per launch ------------------------------------------------
@@ -307,22 +337,23 @@ template <bool kAllPeriodic, bool kCubic>
 __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__restrict__ sys,
                                                         const uint32_t *__restrict__ blk_sys,
                                                         const SchedEntry *__restrict__ sched, uint32_t n_sys,
-                                                        uint32_t slot)
+                                                        uint32_t slot, uint32_t xcd_ranges)
 {
-    const uint32_t s = blk_sys[blockIdx.x];
+    const uint32_t lb = xcd_ranges ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t s = blk_sys[lb];
     const SchedEntry e = sched[size_t(slot) * n_sys + s];
     if (e.colour == kNoColour)
         return;
     const BatchSystem &S = sys[s];
     const uint32_t count = S.seg_count[e.colour];
     const uint32_t nb = check_blocks(count);
-    const uint32_t blk = blockIdx.x - S.check_base;
+    const uint32_t blk = lb - S.check_base;
     if (blk >= nb)
         return;
-    if (!kCubic) {
+    load_bond(S.buf.tables);
+    if (!kCubic)
         load_dep(S.buf.tables);
-        __syncthreads();
-    }
+    __syncthreads();
     check_body<kAllPeriodic, kCubic>(S.buf, S.seg_begin[e.colour], count, e.age, e.parity, e.prev_blocks, blk, nb);
 }
 
@@ -330,15 +361,16 @@ template <bool kCubic>
 __global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__restrict__ sys,
                                                           const uint32_t *__restrict__ blk_sys,
                                                           const SchedEntry *__restrict__ sched, uint32_t n_sys,
-                                                          uint32_t slot)
+                                                          uint32_t slot, uint32_t xcd_ranges)
 {
-    const uint32_t s = blk_sys[blockIdx.x];
+    const uint32_t lb = xcd_ranges ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t s = blk_sys[lb];
     const SchedEntry e = sched[size_t(slot) * n_sys + s];
     if (e.colour == kNoColour)
         return;
     const BatchSystem &S = sys[s];
     const uint32_t cb = check_blocks(S.seg_count[e.colour]);
-    const uint32_t blk = blockIdx.x - S.perform_base;
+    const uint32_t blk = lb - S.perform_base;
     if (blk >= perform_blocks(cb))
         return;
     if (!kCubic) {
@@ -388,34 +420,42 @@ hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t pari
 }
 
 hipError_t launch_check_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
-                              const SchedEntry *sched, uint32_t n_sys, uint32_t slot, hipStream_t stream)
+                              const SchedEntry *sched, uint32_t n_sys, uint32_t slot, uint32_t xcd_ranges,
+                              hipStream_t stream)
 {
     const dim3 grid(n_blocks), block(kBlock);
     switch (cls) {
     case kClassPeriodicCubic:
-        hipLaunchKernelGGL((k_check_batch<true, true>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot);
+        hipLaunchKernelGGL((k_check_batch<true, true>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot,
+                           xcd_ranges);
         break;
     case kClassPeriodic:
-        hipLaunchKernelGGL((k_check_batch<true, false>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot);
+        hipLaunchKernelGGL((k_check_batch<true, false>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot,
+                           xcd_ranges);
         break;
     case kClassCubic:
-        hipLaunchKernelGGL((k_check_batch<false, true>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot);
+        hipLaunchKernelGGL((k_check_batch<false, true>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot,
+                           xcd_ranges);
         break;
     default:
-        hipLaunchKernelGGL((k_check_batch<false, false>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot);
+        hipLaunchKernelGGL((k_check_batch<false, false>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot,
+                           xcd_ranges);
         break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_perform_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
-                                const SchedEntry *sched, uint32_t n_sys, uint32_t slot, hipStream_t stream)
+                                const SchedEntry *sched, uint32_t n_sys, uint32_t slot, uint32_t xcd_ranges,
+                                hipStream_t stream)
 {
     const dim3 grid(n_blocks), block(kBlock);
     if (cls == kClassPeriodicCubic || cls == kClassCubic)
-        hipLaunchKernelGGL((k_perform_batch<true>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot);
+        hipLaunchKernelGGL((k_perform_batch<true>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot,
+                           xcd_ranges);
     else
-        hipLaunchKernelGGL((k_perform_batch<false>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot);
+        hipLaunchKernelGGL((k_perform_batch<false>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot,
+                           xcd_ranges);
     return hipGetLastError();
 }
 

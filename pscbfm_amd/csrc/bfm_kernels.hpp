@@ -15,10 +15,9 @@ constexpr uint32_t kStatSlots = 256;           // spread of the accepted / propo
 // Everything one substep of one system needs on the device.
 struct SweepBuffers {
     uint32_t *ppos;           // packed wrapped positions
-    const uint32_t *ids;      // original ids
+    const uint32_t *ids;      // original id | partner count << kIdBits
     int32_t *image;           // [3][n_pad] periodic images
     const uint32_t *nbr;
-    const uint8_t *nnbr;
     uint32_t n_pad;
     uint32_t *lattice;
     uint32_t *claim;
@@ -32,10 +31,22 @@ struct SweepBuffers {
     unsigned long long *accepted;   // [kStatSlots]
     unsigned long long *proposals;  // [kStatSlots]
     const Tables *tables;     // Morton tables (read only for non-cubic boxes)
-    BondMask bond;            // forbidden-bond mask
     Geometry geom;
     uint64_t seed;
+    uint32_t xcd_ranges;      // 1: logical blocks laid out as one contiguous range per XCD
 };
+
+constexpr uint32_t kXcds = 8;   // accelerator complex dies of an MI355X
+
+// Workgroups are dealt to the XCDs round-robin by block index, and each XCD
+// has its own L2.  Logical block of physical block b in a grid of n blocks
+// such that every XCD sweeps one contiguous range of logical blocks (so
+// neighbouring monomers, which share lattice lines, stay on one L2).
+__host__ __device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t n)
+{
+    const uint32_t x = b % kXcds, i = b / kXcds, per = n / kXcds, rem = n % kXcds;
+    return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
 
 template <class T>
 __host__ __device__ inline T pick(uint32_t parity, T even, T odd)
@@ -94,9 +105,11 @@ inline uint32_t kernel_class(const Geometry &g)
 // blk_sys maps a block to its system (index into sys and into each slot's
 // n_sys schedule entries).
 hipError_t launch_check_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
-                              const SchedEntry *sched, uint32_t n_sys, uint32_t slot, hipStream_t stream);
+                              const SchedEntry *sched, uint32_t n_sys, uint32_t slot, uint32_t xcd_ranges,
+                              hipStream_t stream);
 hipError_t launch_perform_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
-                                const SchedEntry *sched, uint32_t n_sys, uint32_t slot, hipStream_t stream);
+                                const SchedEntry *sched, uint32_t n_sys, uint32_t slot, uint32_t xcd_ranges,
+                                hipStream_t stream);
 
 // Streaming copy of `bytes` (a multiple of 16) from src to dst.
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);

@@ -166,6 +166,12 @@ int bfm_set_steps_between_sortings(bfm_updater *h, int64_t steps)
     return guarded([&] { h->u.setStepsBetweenSortings(steps); });
 }
 
+int bfm_set_spatial_sorting(bfm_updater *h, int on)
+{
+    NEED_HANDLE(h);
+    return guarded([&] { h->u.setSpatialSorting(on != 0); });
+}
+
 int bfm_validate(bfm_updater *h)
 {
     NEED_HANDLE(h);
@@ -303,6 +309,12 @@ int bfm_set_kernel_timing(bfm_updater *h, int on)
     return guarded([&] { h->u.setKernelTiming(on != 0); });
 }
 
+int bfm_set_xcd_ranges(bfm_updater *h, int on)
+{
+    NEED_HANDLE(h);
+    return guarded([&] { h->u.setXcdRanges(on != 0); });
+}
+
 int bfm_batch_create(bfm_batch **out, int gpu)
 {
     if (!out)
@@ -425,6 +437,12 @@ int bfm_batch_set_graphs(bfm_batch *b, int on)
 {
     NEED(b, "batch handle");
     return guarded([&] { b->e->setUseGraphs(on != 0); });
+}
+
+int bfm_batch_set_xcd_ranges(bfm_batch *b, int on)
+{
+    NEED(b, "batch handle");
+    return guarded([&] { b->e->setXcdRanges(on != 0); });
 }
 
 int bfm_batch_synchronize(bfm_batch *b)

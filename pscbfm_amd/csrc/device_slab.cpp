@@ -15,7 +15,6 @@ SlabLayout slab_layout(const HostSystem &s)
     L.bytes[kPartIds] = sizeof(uint32_t) * s.n_pad;
     L.bytes[kPartImage] = sizeof(int32_t) * 3 * size_t(s.n_pad);
     L.bytes[kPartNbr] = sizeof(uint32_t) * kMaxConnectivity * s.n_pad;
-    L.bytes[kPartNnbr] = size_t(s.n_pad);
     L.bytes[kPartLattice] = sizeof(uint32_t) * s.lattice_words;
     L.bytes[kPartClaim] = sizeof(uint32_t) * s.lattice_words;
     L.bytes[kPartColl0] = sizeof(uint32_t) * s.lattice_words;
@@ -50,7 +49,6 @@ SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, u
     up(kPartIds, s.ids_sorted.data(), L.bytes[kPartIds], "upload ids");
     up(kPartImage, s.image_sorted.data(), L.bytes[kPartImage], "upload images");
     up(kPartNbr, s.nbr_sorted.data(), L.bytes[kPartNbr], "upload nbr");
-    up(kPartNnbr, s.nnbr_sorted.data(), L.bytes[kPartNnbr], "upload nnbr");
     up(kPartLattice, s.lattice.data(), L.bytes[kPartLattice], "upload lattice");
     up(kPartTables, &s.tables, sizeof(Tables), "upload tables");
     up(kPartSegBegin, s.seg_begin.data(), sizeof(uint32_t) * s.n_colours, "upload segments");
@@ -61,7 +59,6 @@ SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, u
     b.ids = static_cast<const uint32_t *>(slab_part(slab, L, kPartIds));
     b.image = static_cast<int32_t *>(slab_part(slab, L, kPartImage));
     b.nbr = static_cast<const uint32_t *>(slab_part(slab, L, kPartNbr));
-    b.nnbr = static_cast<const uint8_t *>(slab_part(slab, L, kPartNnbr));
     b.n_pad = s.n_pad;
     b.lattice = static_cast<uint32_t *>(slab_part(slab, L, kPartLattice));
     b.claim = static_cast<uint32_t *>(slab_part(slab, L, kPartClaim));
@@ -76,8 +73,6 @@ SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, u
     b.accepted = static_cast<unsigned long long *>(slab_part(slab, L, kPartAccepted));
     b.proposals = static_cast<unsigned long long *>(slab_part(slab, L, kPartProposals));
     b.tables = static_cast<const Tables *>(slab_part(slab, L, kPartTables));
-    for (int k = 0; k < 8; ++k)
-        b.bond.w[k] = uint64_t(s.tables.bond[2 * k]) | (uint64_t(s.tables.bond[2 * k + 1]) << 32);
     b.geom = s.geom;
     b.seed = seed;
     return b;

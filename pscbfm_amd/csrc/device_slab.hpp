@@ -16,7 +16,6 @@ enum SlabPart {
     kPartIds,
     kPartImage,
     kPartNbr,
-    kPartNnbr,
     kPartLattice,
     kPartClaim,
     kPartColl0,

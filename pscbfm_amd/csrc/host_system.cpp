@@ -35,7 +35,6 @@ void HostSystem::reset()
     ids_sorted.clear();
     image_sorted.clear();
     nbr_sorted.clear();
-    nnbr_sorted.clear();
     lattice.clear();
     mean_bonds = 0.0;
 }
@@ -179,8 +178,8 @@ void plan(HostSystem &s)
         throw std::runtime_error(msg.str());
     }
     make_geometry(s.box, s.periodic, s.geom, s.tables, s.lattice_words);
-    if (s.n > kSortedIndexMask / 2)
-        throw std::runtime_error("too many monomers for the 29-bit sorted index");
+    if (s.n > kIdMask)
+        throw std::runtime_error("too many monomers for the 28-bit monomer id");
     for (int i = 0; i < 512; ++i)
         if (s.forbidden[i])
             s.tables.bond[i >> 5] |= 1u << (i & 31);
@@ -269,7 +268,28 @@ void plan(HostSystem &s)
         throw std::runtime_error("padded system exceeds the 29-bit sorted index");
     s.n_pad = uint32_t(std::max<uint64_t>(offset, kSegmentAlign));
     s.to_sorted.assign(s.n, 0);
-    {
+    if (s.spatial_order) {
+        // within a colour, monomers follow the lattice's Morton order of their
+        // lower corners (ties by id): a wavefront then covers a compact patch
+        // of space, so its lattice words, claims and partners share cache
+        // lines (the reference's doSpatialSorting, :1836-1935, sorts each
+        // species by the linearized position the same way)
+        std::vector<uint64_t> key(s.n);
+        for (uint32_t i = 0; i < s.n; ++i) {
+            const int32_t *r = &s.xyz[3 * size_t(i)];
+            key[i] = (uint64_t(host_site(s.geom, s.tables, r[0], r[1], r[2])) << 32) | i;
+        }
+        std::vector<std::vector<uint64_t>> per(s.n_colours);
+        for (uint32_t c = 0; c < s.n_colours; ++c)
+            per[c].reserve(s.seg_count[c]);
+        for (uint32_t i = 0; i < s.n; ++i)
+            per[s.colour[i]].push_back(key[i]);
+        for (uint32_t c = 0; c < s.n_colours; ++c) {
+            std::sort(per[c].begin(), per[c].end());
+            for (uint32_t k = 0; k < per[c].size(); ++k)
+                s.to_sorted[uint32_t(per[c][k])] = s.seg_begin[c] + k;
+        }
+    } else {
         std::vector<uint32_t> next(s.seg_begin);
         for (uint32_t i = 0; i < s.n; ++i)
             s.to_sorted[i] = next[s.colour[i]]++;
@@ -278,7 +298,6 @@ void plan(HostSystem &s)
     s.ids_sorted.assign(s.n_pad, 0xFFFFFFFFu);
     s.image_sorted.assign(3 * size_t(s.n_pad), 0);
     s.nbr_sorted.assign(size_t(kMaxConnectivity) * s.n_pad, 0);
-    s.nnbr_sorted.assign(s.n_pad, 0);
     for (uint32_t i = 0; i < s.n; ++i) {
         const uint32_t t = s.to_sorted[i];
         uint32_t w[3];
@@ -290,8 +309,7 @@ void plan(HostSystem &s)
             s.image_sorted[size_t(a) * s.n_pad + t] = int32_t(img);
         }
         s.ppos_sorted[t] = pack_xyz(w[0], w[1], w[2]);
-        s.ids_sorted[t] = i;
-        s.nnbr_sorted[t] = s.nnbr[i];
+        s.ids_sorted[t] = i | (uint32_t(s.nnbr[i]) << kIdBits);
         for (uint32_t j = 0; j < s.nnbr[i]; ++j)
             s.nbr_sorted[size_t(j) * s.n_pad + t] = s.to_sorted[s.nbr[size_t(i) * kMaxConnectivity + j]];
     }

@@ -24,6 +24,7 @@ struct HostSystem {
     std::vector<uint8_t> nnbr;       // n
     std::vector<uint32_t> nbr;       // n * kMaxConnectivity, directed links as given
     uint8_t forbidden[512];          // 1 = forbidden (copyBondSet)
+    bool spatial_order = true;       // lay colours out in Morton order of position
 
     // ---- derived by plan() --------------------------------------------------
     Geometry geom{};
@@ -37,10 +38,9 @@ struct HostSystem {
     uint32_t max_segment = 0;
     std::vector<uint32_t> to_sorted; // original id -> sorted index
     std::vector<uint32_t> ppos_sorted;  // n_pad packed wrapped positions
-    std::vector<uint32_t> ids_sorted;   // n_pad original ids (padding: 0xFFFFFFFF)
+    std::vector<uint32_t> ids_sorted;   // n_pad: original id | partner count << kIdBits (padding: ~0)
     std::vector<int32_t> image_sorted;  // 3 * n_pad periodic images, axis-major
     std::vector<uint32_t> nbr_sorted;  // kMaxConnectivity * n_pad, slot-major
-    std::vector<uint8_t> nnbr_sorted;  // n_pad
     std::vector<uint32_t> lattice;     // lattice_words occupancy bits
     double mean_bonds = 0.0;           // mean number of listed partners per monomer
 

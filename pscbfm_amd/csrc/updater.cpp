@@ -157,6 +157,15 @@ void UpdaterGPUScBFM_AB_Type::setStepsBetweenSortings(int64_t steps)
 {
     if (steps <= 0)
         throw std::invalid_argument("setStepsBetweenSortings: steps must be positive");
+    mStepsBetweenSortings = steps;
+}
+
+void UpdaterGPUScBFM_AB_Type::setSpatialSorting(bool on)
+{
+    if (mInitialized)
+        throw StateError("setSpatialSorting after initialize; call cleanup first");
+    mSys.spatial_order = on;
+    mValidated = false;
 }
 
 void UpdaterGPUScBFM_AB_Type::validate()
@@ -186,6 +195,7 @@ void UpdaterGPUScBFM_AB_Type::initialize()
     try {
         hip_check(hipStreamCreate(&mStream), "hipStreamCreate");
         mBuf = upload_slab(s, mDevSlab, L, mSeed, mStream);
+        mBuf.xcd_ranges = mXcdRanges ? 1u : 0u;
         hip_check(hipStreamSynchronize(mStream), "hipStreamSynchronize (initialize)");
     } catch (...) {
         freeDevice();

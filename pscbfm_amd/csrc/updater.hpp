@@ -56,6 +56,7 @@ public:
     void setSeed(uint64_t seed) { mSeed = seed; }
     uint64_t seed() const { return mSeed; }
     void setStepsBetweenSortings(int64_t steps);
+    void setSpatialSorting(bool on);
 
     // host half of initialize(): checks and colouring, no device work
     void validate();
@@ -70,6 +71,7 @@ public:
     void runResident(uint32_t n_mcs);
     void copyBack();
     void setKernelTiming(bool on) { mTiming = on; }
+    void setXcdRanges(bool on) { mBuf.xcd_ranges = on ? 1u : 0u; mXcdRanges = on; }
     const HostSystem &system() const { return mSys; }
     bool initialized() const { return mInitialized; }
     bool validated() const { return mValidated; }
@@ -104,6 +106,8 @@ private:
     uint64_t mAttempts = 0;
 
     bool mTiming = false;
+    bool mXcdRanges = true;
+    int64_t mStepsBetweenSortings = 5000;   // mnStepsBetweenSortings, UpdaterGPUScBFM_AB_Type.cu:1379
     std::vector<hipEvent_t> mEventPool;          // reusable events
     mutable std::vector<hipEvent_t> mEvents;     // pending (start, mid, end) triples
     mutable double mCheckMs = 0.0, mPerformMs = 0.0;

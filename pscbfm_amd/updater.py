@@ -130,6 +130,16 @@ class UpdaterGPUScBFM_AB_Type:
     def setKernelTiming(self, on: bool) -> None:
         _capi.check(self._lib.bfm_set_kernel_timing(self._h, int(bool(on))))
 
+    def setSpatialSorting(self, on: bool) -> None:
+        """Morton order of positions within each colour (default) or id order;
+        the trajectory is the same either way.  Before initialize."""
+        _capi.check(self._lib.bfm_set_spatial_sorting(self._h, int(bool(on))))
+
+    def setXcdRanges(self, on: bool) -> None:
+        """Contiguous block range per XCD (default) or round-robin placement;
+        the trajectory is the same either way."""
+        _capi.check(self._lib.bfm_set_xcd_ranges(self._h, int(bool(on))))
+
     def runResident(self, n_mcs: int) -> None:
         _capi.check(self._lib.bfm_run_resident(self._h, n_mcs))
 
@@ -138,7 +148,7 @@ class UpdaterGPUScBFM_AB_Type:
 
 
 def updater_from_system(system: dict, seed: int, gpu: int | None = 0, age: int = 0,
-                        validate_only: bool = False) -> UpdaterGPUScBFM_AB_Type:
+                        validate_only: bool = False, spatial_sorting: bool = True) -> UpdaterGPUScBFM_AB_Type:
     """Builds an updater the way the LeMonADE adapter does
     (GPUScBFM_AB_Type.h:120-184): box, periodicity, age, positions, both
     directions of every bond, the 512-entry bond table, then initialize."""
@@ -165,6 +175,7 @@ def updater_from_system(system: dict, seed: int, gpu: int | None = 0, age: int =
             for dz in range(-4, 4):
                 idx = (dx & 7) | ((dy & 7) << 3) | ((dz & 7) << 6)
                 u.copyBondSet(dx, dy, dz, bool(forbidden[idx]))
+    u.setSpatialSorting(spatial_sorting)
     if validate_only:
         u.validate()
     else:

@@ -31,12 +31,13 @@ def oracle_final(system, seed, age, n_mcs):
     return want, acc, prop
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_batch_members_match_oracle(gpu_ready, graphs):
+@pytest.mark.parametrize("graphs,xcd_ranges", [(True, True), (False, True), (True, False)])
+def test_batch_members_match_oracle(gpu_ready, graphs, xcd_ranges):
     cases = mixed_systems(6, seed=3)
-    print(f"case: batch of {len(cases)} systems, graphs={graphs}", flush=True)
+    print(f"case: batch of {len(cases)} systems, graphs={graphs} xcd_ranges={xcd_ranges}", flush=True)
     b = Batch(0)
     b.set_graphs(graphs)
+    b.set_xcd_ranges(xcd_ranges)
     ids = []
     for s, seed, age in cases:
         u = updater_from_system(s, seed=seed, age=age, validate_only=True)

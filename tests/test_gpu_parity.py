@@ -14,11 +14,12 @@ from pscbfm_amd import systems, updater_from_system
 pytestmark = pytest.mark.gpu
 
 
-def run_pair(system, seed, chunks, age0=0):
+def run_pair(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True):
     n = len(system["xyz"])
     print(f"case: n={n} box={system['box']} periodic={system.get('periodic')} seed={seed:#x} "
-          f"chunks={chunks} age0={age0}", flush=True)
-    u = updater_from_system(system, seed=seed, gpu=0, age=age0)
+          f"chunks={chunks} age0={age0} xcd_ranges={xcd_ranges} spatial={spatial}", flush=True)
+    u = updater_from_system(system, seed=seed, gpu=0, age=age0, spatial_sorting=spatial)
+    u.setXcdRanges(xcd_ranges)
     for c in chunks:
         u.runSimulationOnGPU(c)
     got = u.getMonomerPositions(n)
@@ -30,8 +31,8 @@ def run_pair(system, seed, chunks, age0=0):
     return got, want, stats, acc, prop, osys
 
 
-def assert_parity(system, seed, chunks, age0=0):
-    got, want, stats, acc, prop, osys = run_pair(system, seed, chunks, age0)
+def assert_parity(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True):
+    got, want, stats, acc, prop, osys = run_pair(system, seed, chunks, age0, xcd_ranges, spatial)
     diff = np.nonzero((got != want).any(axis=1))[0]
     assert diff.size == 0, f"{diff.size} monomers differ, first {diff[:5]}: gpu {got[diff[:3]]} oracle {want[diff[:3]]}"
     assert stats["accepted"] == acc
@@ -52,6 +53,18 @@ def test_small_melt_1000_mcs(gpu_ready):
     # the plain small-melt workload (64^3, 512 x N=32, 2 colours) for 1000 MCS
     s = systems.linear_melt(64, 512, 32, seed=0x5EED000000000001)
     assert_parity(s, seed=0x5EED000000000001, chunks=[1000])
+
+
+def test_round_robin_block_placement(gpu_ready):
+    # the XCD placement changes which hardware runs a block, never the result
+    s = systems.linear_melt(64, 512, 32, seed=21)
+    assert_parity(s, seed=77, chunks=[300], xcd_ranges=False)
+
+
+def test_id_order_layout(gpu_ready):
+    # monomers stored in id order instead of Morton order: same trajectory
+    s = systems.linear_melt(64, 512, 32, seed=22)
+    assert_parity(s, seed=78, chunks=[300], spatial=False)
 
 
 def test_resume_is_continuous(gpu_ready):
