@@ -133,7 +133,7 @@ __host__ __device__ __forceinline__ uint32_t spread3(uint32_t v)
 // compact Morton order is then the classic Z-curve, x in bit 0, computed in
 // registers.  Non-cubic boxes use per-axis tables in LDS (bfm_kernels.hip).
 struct DepositCubic {
-    __device__ __forceinline__ uint32_t operator()(int a, uint32_t v) const { return spread3(v) << a; }
+    __host__ __device__ __forceinline__ uint32_t operator()(int a, uint32_t v) const { return spread3(v) << a; }
 };
 __host__ __device__ __forceinline__ uint32_t dilated_inc(uint32_t d, uint32_t m) { return ((d | ~m) + 1u) & m; }
 __host__ __device__ __forceinline__ uint32_t dilated_dec(uint32_t d, uint32_t m) { return (d - 1u) & m; }
@@ -143,47 +143,81 @@ __host__ __device__ __forceinline__ int32_t comp(const int4 &r, int a) { return 
 // SGPRs; a dynamic index would turn them into scratch or vector loads).
 __host__ __device__ __forceinline__ uint32_t sel3(const uint32_t v[3], int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); }
 
-// Site indices of the 2x2 face a cube with lower corner r enters when it
-// moves in direction dir (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z, as the reference's
-// DXTable_d/DYTable_d/DZTable_d, UpdaterGPUScBFM_AB_Type.cu:1537-1541).
-// Also returns the dilated coordinate of the face plane along the move axis.
+// The lattice holds one bit per monomer, at its lower corner.  A cube with
+// corner r moving in direction dir (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z, as the
+// reference's DXTable_d/DYTable_d/DZTable_d, UpdaterGPUScBFM_AB_Type.cu:
+// 1537-1541) can only overlap a cube whose corner lies in the 3x3 "front":
+// the plane two sites ahead of r on the move axis, one site either side of r
+// on the other two (checkFront, UpdaterGPUScBFM_AB_Type.cu:420-594).  The
+// same 9 sites are where another moved corner makes two moved cubes overlap.
+//
+// Index bits 0..4 of a site are x0 y0 z0 x1 y1 for every box (edges >= 8), so
+// a 32-bit word spans 4 x 4 x 2 sites and 3 consecutive coordinates of one
+// axis fall into at most 2 words: the 9 sites lie in at most 4 words.  w[q]
+// are those words, m[q] the bits of the front in each (m[q] == 0: not needed).
+struct Front {
+    uint32_t w[4], m[4];
+    uint32_t new_site;   // corner after the move
+    uint32_t old_site;   // corner before the move
+};
+
+// Product of two factors below 2^24 (one v_mul_u32_u24 on the device).
+__host__ __device__ __forceinline__ uint32_t mul24(uint32_t a, uint32_t b)
+{
+    return (a & 0xFFFFFFu) * (b & 0xFFFFFFu);
+}
+
 template <class Dep>
-__device__ __forceinline__ void face_sites(const Geometry &g, const Dep &dep, const int4 &r, int dir, uint32_t s[4],
-                                  uint32_t &d_face)
+__host__ __device__ __forceinline__ Front front_sites(const Geometry &g, const Dep &dep, const int4 &r, int dir)
 {
     const int a = dir >> 1;
     const int b = a == 0 ? 1 : 0;
     const int c = a == 2 ? 1 : 2;
-    const int32_t fa = (dir & 1) ? comp(r, a) + 2 : comp(r, a) - 1;
-    d_face = dep(a, uint32_t(fa) & sel3(g.mask, a));
-    const uint32_t b0 = dep(b, uint32_t(comp(r, b)) & sel3(g.mask, b));
-    const uint32_t c0 = dep(c, uint32_t(comp(r, c)) & sel3(g.mask, c));
-    const uint32_t b1 = dilated_inc(b0, sel3(g.dmask, b));
-    const uint32_t c1 = dilated_inc(c0, sel3(g.dmask, c));
-    s[0] = d_face | b0 | c0;
-    s[1] = d_face | b1 | c0;
-    s[2] = d_face | b0 | c1;
-    s[3] = d_face | b1 | c1;
+    const bool up = (dir & 1) != 0;
+    const uint32_t dma = sel3(g.dmask, a), dmb = sel3(g.dmask, b), dmc = sel3(g.dmask, c);
+    const uint32_t da = dep(a, uint32_t(up ? comp(r, a) + 2 : comp(r, a) - 2) & sel3(g.mask, a));
+    const uint32_t b0 = dep(b, uint32_t(comp(r, b) - 1) & sel3(g.mask, b));
+    const uint32_t c0 = dep(c, uint32_t(comp(r, c) - 1) & sel3(g.mask, c));
+    const uint32_t b1 = dilated_inc(b0, dmb), b2 = dilated_inc(b1, dmb);
+    const uint32_t c1 = dilated_inc(c0, dmc), c2 = dilated_inc(c1, dmc);
+    // the moved corner is one step behind the front plane, the old one two
+    const uint32_t a1 = up ? dilated_dec(da, dma) : dilated_inc(da, dma);
+    const uint32_t a0 = up ? dilated_dec(a1, dma) : dilated_inc(a1, dma);
+    Front f;
+    f.new_site = a1 | b1 | c1;
+    f.old_site = a0 | b1 | c1;
+    // Site bit = 2^(la + lb + lc) with la, lb, lc the in-word parts of the
+    // three axes (disjoint bit positions, so | is +).  Summing 2^lb over the
+    // b-rows that share a word (sb_*) and 2^lc likewise (sc_*), the mask of a
+    // word is sb * sc << la: the products of distinct powers never carry.
+    // Every factor is < 2^19 (y1 is index bit 4), so 24-bit multiplies do.
+    const uint32_t hb0 = b0 >> 5, hc0 = c0 >> 5;
+    const bool b1_hi = (b1 >> 5) != hb0, b2_hi = (b2 >> 5) != hb0;
+    const bool c1_hi = (c1 >> 5) != hc0, c2_hi = (c2 >> 5) != hc0;
+    const uint32_t pb0 = 1u << (b0 & 31u), pb1 = 1u << (b1 & 31u), pb2 = 1u << (b2 & 31u);
+    const uint32_t pc0 = 1u << (c0 & 31u), pc1 = 1u << (c1 & 31u), pc2 = 1u << (c2 & 31u);
+    const uint32_t sb_lo = pb0 | (b1_hi ? 0u : pb1) | (b2_hi ? 0u : pb2);
+    const uint32_t sb_hi = (b1_hi ? pb1 : 0u) | (b2_hi ? pb2 : 0u);
+    const uint32_t sc_lo = pc0 | (c1_hi ? 0u : pc1) | (c2_hi ? 0u : pc2);
+    const uint32_t sc_hi = (c1_hi ? pc1 : 0u) | (c2_hi ? pc2 : 0u);
+    const uint32_t la = da & 31u, ha = da >> 5;
+    const uint32_t hb2 = b2 >> 5, hc2 = c2 >> 5;
+    f.w[0] = ha | hb0 | hc0;
+    f.w[1] = ha | hb2 | hc0;
+    f.w[2] = ha | hb0 | hc2;
+    f.w[3] = ha | hb2 | hc2;
+    f.m[0] = mul24(sb_lo, sc_lo) << la;
+    f.m[1] = mul24(sb_hi, sc_lo) << la;
+    f.m[2] = mul24(sb_lo, sc_hi) << la;
+    f.m[3] = mul24(sb_hi, sc_hi) << la;
+    return f;
 }
 
-// Merges bit updates that fall into the same 32-bit word: on return every
-// word appears once, in an entry with a non-zero mask; other masks are zero.
-template <int N>
-__device__ __forceinline__ void merge_words(uint32_t (&w)[N], uint32_t (&m)[N])
+// Site of corner (x, y, z) (wrapped coordinates).
+template <class Dep>
+__host__ __device__ __forceinline__ uint32_t corner_site(const Dep &dep, const int4 &r)
 {
-#pragma unroll
-    for (int k = 1; k < N; ++k)
-#pragma unroll
-        for (int j = 0; j < k; ++j)
-            if (m[j] != 0u && m[k] != 0u && w[j] == w[k]) {
-                m[j] |= m[k];
-                m[k] = 0u;
-            }
-}
-
-__device__ __forceinline__ uint32_t bit_test(const uint32_t *words, uint32_t site)
-{
-    return (words[site >> 5] >> (site & 31u)) & 1u;
+    return dep(0, uint32_t(r.x)) | dep(1, uint32_t(r.y)) | dep(2, uint32_t(r.z));
 }
 
 // Bond-table index of a bond vector: (dx & 7) | (dy & 7) << 3 | (dz & 7) << 6.

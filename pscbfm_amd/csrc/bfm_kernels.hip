@@ -7,19 +7,21 @@
 // the same substep enters a site its cube enters.  The outcome depends only on
 // the set of proposals, never on thread order, so every run is bit-exact.
 //
+// Both phases use the reference's own geometry: the lattice holds one bit per
+// monomer at its lower corner, and a move is tested on the 9-site front of
+// the moved cube (checkFront, :420-594), first in the occupancy lattice, then
+// in a lattice of this substep's moved corners (the reference's mLatticeTmp).
 // Differences in mechanism (same result):
-//  * excluded volume reads 4 bits of an 8-site occupancy lattice instead of 9
-//    corner bytes;
-//  * conflicts are found with face-site claims: a survivor ORs its 4 entry
-//    sites into `claim`; bits that were already claimed are ORed into `coll`;
-//    phase 2 rejects a survivor with any of its sites in `coll`;
+//  * 1 bit per site instead of 1 byte, in compact Morton order; the 9 front
+//    sites fall into at most 4 words;
+//  * a survivor marks its moved corner with one atomic OR (no return value);
 //  * survivors are compacted per wavefront (wave64 ballot + mbcnt), so phase 2
 //    only runs over moves that can still happen (the reference's TODO,
 //    :1222-1228) and no global counter is contended; the only workgroup
 //    barrier is the one after the LDS tables are filled at kernel start;
-//  * no memset: claims are released by their owners in phase 2 and collision
-//    words are cleared from per-wave lists one substep later;
-//  * an accepted move flips its 8 lattice bits with one atomic XOR per word.
+//  * no memset: the marks of a substep are cleared by the next check launch
+//    (claims and survivor lists are double-buffered by substep parity);
+//  * an accepted move flips its 2 corner bits with one atomic XOR per word.
 #include "bfm_kernels.hpp"
 
 #include <algorithm>
@@ -83,12 +85,12 @@ template <bool kCubic>
 using DepositFor = typename std::conditional<kCubic, DepositCubic, DepositTable>::type;
 
 // Phase 1 for check block `blk` of `nblk` covering the colour segment
-// [begin, begin+count).  Every wavefront compacts its own survivors and
-// collision words; prev_blocks is the grid of the previous check launch of
-// this system (other parity), whose collision words this launch clears.
+// [begin, begin+count).  Every wavefront compacts its own survivors into
+// surv[parity]; prev_blocks is the grid of the previous check launch of this
+// system (other parity), whose claims this launch clears.
 template <bool kAllPeriodic, bool kCubic>
 __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
-                                  uint32_t parity, uint32_t prev_blocks, uint32_t blk, uint32_t nblk)
+                                           uint32_t parity, uint32_t prev_blocks, uint32_t blk, uint32_t nblk)
 {
     const DepositFor<kCubic> dep{};
     const Geometry &g = buf.geom;
@@ -97,19 +99,17 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
     const uint32_t nwaves = nblk * kWavesPerBlock;
     const uint32_t prev_waves = prev_blocks * kWavesPerBlock;
 
-    uint32_t *coll_cur = pick(parity, buf.coll0, buf.coll1);
-    uint32_t *list_cur = pick(parity, buf.clist0, buf.clist1);
-    uint32_t *coll_prev = pick(parity, buf.coll1, buf.coll0);
-    const uint32_t *list_prev = pick(parity, buf.clist1, buf.clist0);
-    const uint32_t *cnt_prev = pick(parity, buf.ccount1, buf.ccount0);
-    // collision count of the previous launch's wave with this index, issued early
-    const uint32_t n_prev_own = gwave < prev_waves ? cnt_prev[gwave] : 0u;
+    uint32_t *claim_cur = pick(parity, buf.claim0, buf.claim1);
+    uint32_t *claim_prev = pick(parity, buf.claim1, buf.claim0);
+    uint2 *surv_cur = pick(parity, buf.surv0, buf.surv1);
+    const uint2 *surv_prev = pick(parity, buf.surv1, buf.surv0);
+    const uint32_t *scount_prev = pick(parity, buf.scount1, buf.scount0);
+    // survivors of the previous launch's wave with this index, issued early
+    const uint32_t n_prev_own = gwave < prev_waves ? scount_prev[gwave] : 0u;
 
     const uint32_t t = blk * kBlock + threadIdx.x;
     bool pass = false;
     uint2 entry = make_uint2(0u, 0u);
-    uint32_t cword[4] = {0u, 0u, 0u, 0u};   // collision words of this lane
-    uint32_t ncoll = 0;
     if (t < count) {
         const uint32_t i = begin + t;
         const uint32_t p0 = buf.ppos[i];
@@ -140,10 +140,13 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
                 ok = false;
         }
 
-        uint32_t s[4], d_face;
-        face_sites(g, dep, r0, dir, s, d_face);
+        // excluded volume: no corner in the 9-site front
+        const Front f = front_sites(g, dep, r0, dir);
         const uint32_t *lat = buf.lattice;
-        const uint32_t l0 = lat[s[0] >> 5], l1 = lat[s[1] >> 5], l2 = lat[s[2] >> 5], l3 = lat[s[3] >> 5];
+        uint32_t l[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            l[q] = f.m[q] ? lat[f.w[q]] : 0u;
 
         // bond partners belong to other colours and do not move this substep;
         // every position load is issued before the first is used
@@ -159,30 +162,13 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
                 if (bond_forbidden(q.x - r1.x, q.y - r1.y, q.z - r1.z))
                     ok = false;
             }
-        if (((l0 >> (s[0] & 31u)) | (l1 >> (s[1] & 31u)) | (l2 >> (s[2] & 31u)) | (l3 >> (s[3] & 31u))) & 1u)
+        if ((l[0] & f.m[0]) | (l[1] & f.m[1]) | (l[2] & f.m[2]) | (l[3] & f.m[3]))
             ok = false;
 
         if (ok) {
-            uint32_t w[4], m[4], old[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                w[k] = s[k] >> 5;
-                m[k] = 1u << (s[k] & 31u);
-            }
-            merge_words(w, m);
-            // all claims in flight before the first result is used
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                old[k] = m[k] ? atomicOr(&buf.claim[w[k]], m[k]) : 0u;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t hit = old[k] & m[k];
-                if (hit) {
-                    atomicOr(&coll_cur[w[k]], hit);
-                    cword[k] = w[k] | 0x80000000u;   // marks a listed word (site index < 2^30)
-                    ++ncoll;
-                }
-            }
+            // the reference's tentative lattice (dpLatticeTmp, :1012-1023),
+            // one bit per moved corner
+            atomicOr(&claim_cur[f.new_site >> 5], 1u << (f.new_site & 31u));
             pass = true;
             entry = make_uint2(i | (uint32_t(dir) << kSortedIndexBits), p0);
         }
@@ -191,28 +177,23 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
     // wave compaction of the survivors
     const uint64_t mball = __ballot(pass);
     if (pass)
-        buf.surv[size_t(gwave) * kWave + lanes_below(mball)] = entry;
-    // wave prefix of the collision words (0..4 per lane)
-    const uint64_t c0 = __ballot(ncoll & 1u), c1 = __ballot(ncoll & 2u), c2 = __ballot(ncoll & 4u);
-    uint32_t slot = lanes_below(c0) + 2u * lanes_below(c1) + 4u * lanes_below(c2);
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (cword[k]) {
-            list_cur[size_t(gwave) * kCollSlotsWave + slot] = cword[k] & 0x7FFFFFFFu;
-            ++slot;
-        }
-    if (lane == 0) {
-        buf.scount[gwave] = uint32_t(__popcll(mball));
-        pick(parity, buf.ccount0, buf.ccount1)[gwave] =
-            uint32_t(__popcll(c0)) + 2u * uint32_t(__popcll(c1)) + 4u * uint32_t(__popcll(c2));
-    }
+        surv_cur[size_t(gwave) * kWave + lanes_below(mball)] = entry;
+    if (lane == 0)
+        pick(parity, buf.scount0, buf.scount1)[gwave] = uint32_t(__popcll(mball));
 
-    // collision bits listed by the previous check launch: nobody reads
-    // coll[parity^1] in this kernel
+    // claims of the previous check launch (the reference zeroes its tentative
+    // lattice in a third kernel, :1240-1262): nobody reads claim[parity^1] in
+    // this kernel, and its words hold only those claims, so a plain store of
+    // zero clears them
     for (uint32_t pw = gwave; pw < prev_waves; pw += nwaves) {
-        const uint32_t n = pw == gwave ? n_prev_own : cnt_prev[pw];
-        for (uint32_t k = lane; k < n; k += kWave)
-            coll_prev[list_prev[size_t(pw) * kCollSlotsWave + k]] = 0u;
+        const uint32_t n = pw == gwave ? n_prev_own : scount_prev[pw];
+        for (uint32_t k = lane; k < n; k += kWave) {
+            const uint2 e = surv_prev[size_t(pw) * kWave + k];
+            const int4 r = moved(unpack_xyz(e.y), int(e.x >> kSortedIndexBits));
+            const int4 rw = make_int4(int32_t(uint32_t(r.x) & g.mask[0]), int32_t(uint32_t(r.y) & g.mask[1]),
+                                      int32_t(uint32_t(r.z) & g.mask[2]), 0);
+            claim_prev[corner_site(dep, rw) >> 5] = 0u;
+        }
     }
 }
 
@@ -240,11 +221,13 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
     const uint32_t pwave = wave_uniform(blk * kWavesPerBlock + (threadIdx.x >> 6));
     const uint32_t check_waves = check_blocks_ * kWavesPerBlock;
     const uint32_t cw0 = pwave * kPerformGroup;
-    const uint32_t *coll = pick(parity, buf.coll0, buf.coll1);
+    const uint32_t *claim = pick(parity, buf.claim0, buf.claim1);
+    const uint2 *surv = pick(parity, buf.surv0, buf.surv1);
+    const uint32_t *scount = pick(parity, buf.scount0, buf.scount1);
 
     uint32_t c = 0;
     if (lane < kPerformGroup && cw0 + lane < check_waves)
-        c = buf.scount[cw0 + lane];
+        c = scount[cw0 + lane];
     const uint32_t p1 = __shfl(c, 0);
     const uint32_t p2 = p1 + __shfl(c, 1);
     const uint32_t p3 = p2 + __shfl(c, 2);
@@ -254,54 +237,34 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
     for (uint32_t t = lane; t < total; t += kWave) {
         const uint32_t k = (t >= p1 ? 1u : 0u) + (t >= p2 ? 1u : 0u) + (t >= p3 ? 1u : 0u);
         const uint32_t base = k == 0 ? 0u : (k == 1 ? p1 : (k == 2 ? p2 : p3));
-        const uint2 e = buf.surv[size_t(cw0 + k) * kWave + (t - base)];
+        const uint2 e = surv[size_t(cw0 + k) * kWave + (t - base)];
         const uint32_t i = e.x & kSortedIndexMask;
         const int dir = int(e.x >> kSortedIndexBits);
         const int4 r0 = unpack_xyz(e.y);
-        uint32_t s[4], d_face;
-        face_sites(g, dep, r0, dir, s, d_face);
-        const uint32_t k0 = coll[s[0] >> 5], k1 = coll[s[1] >> 5], k2 = coll[s[2] >> 5], k3 = coll[s[3] >> 5];
-        const bool clash =
-            (((k0 >> (s[0] & 31u)) | (k1 >> (s[1] & 31u)) | (k2 >> (s[2] & 31u)) | (k3 >> (s[3] & 31u))) & 1u) != 0;
-
-        uint32_t w[8], m[8];
+        // another survivor's moved corner in the front: the moved cubes
+        // overlap and both stay (the reference's second front check on the
+        // tentative lattice, :1148-1190)
+        const Front f = front_sites(g, dep, r0, dir);
+        uint32_t k4[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            w[q] = s[q] >> 5;
-            m[q] = 1u << (s[q] & 31u);
-        }
-        {
-            // release the claims (claim[] is not read in this kernel)
-            uint32_t cw[4] = {w[0], w[1], w[2], w[3]}, cm[4] = {m[0], m[1], m[2], m[3]};
-            merge_words(cw, cm);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (cm[q])
-                    buf.claim[cw[q]] = 0u;
-        }
+        for (int q = 0; q < 4; ++q)
+            k4[q] = f.m[q] ? claim[f.w[q]] : 0u;
+        const bool clash = ((k4[0] & f.m[0]) | (k4[1] & f.m[1]) | (k4[2] & f.m[2]) | (k4[3] & f.m[3])) != 0u;
         if (!clash) {
-            const int a = dir >> 1;
-            const uint32_t dm = sel3(g.dmask, a);
-            // the face the cube leaves lies two planes behind the one it enters
-            const uint32_t d_back = (dir & 1) ? dilated_dec(dilated_dec(d_face, dm), dm)
-                                              : dilated_inc(dilated_inc(d_face, dm), dm);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t b = (s[q] & ~dm) | d_back;
-                w[4 + q] = b >> 5;
-                m[4 + q] = 1u << (b & 31u);
+            // vacate the old corner, occupy the new one: distinct bits of every
+            // accepted move, so XOR is exact under any interleaving
+            const uint32_t w0 = f.old_site >> 5, w1 = f.new_site >> 5;
+            const uint32_t b0 = 1u << (f.old_site & 31u), b1 = 1u << (f.new_site & 31u);
+            if (w0 == w1) {
+                atomicXor(&buf.lattice[w0], b0 | b1);
+            } else {
+                atomicXor(&buf.lattice[w0], b0);
+                atomicXor(&buf.lattice[w1], b1);
             }
-            // the 8 flipped sites are distinct and every other accepted move
-            // flips different ones, so XOR sets the entered and clears the
-            // vacated bits
-            merge_words(w, m);
-#pragma unroll
-            for (int q = 0; q < 8; ++q)
-                if (m[q])
-                    atomicXor(&buf.lattice[w[q]], m[q]);
             // new wrapped coordinate on the move axis; crossing the box
             // boundary changes the periodic image (only on periodic axes:
             // walls keep the others inside)
+            const int a = dir >> 1;
             const uint32_t am = sel3(g.mask, a);
             const uint32_t v = uint32_t(comp(r0, a) + ((dir & 1) ? 1 : -1)) & am;
             const uint32_t shift = kCoordBits * uint32_t(a);

@@ -8,8 +8,6 @@
 namespace pscbfm {
 
 constexpr uint32_t kPerformGroup = 4;         // check waves whose survivors one perform wave takes
-constexpr uint32_t kCollSlotsWave = 4 * kWave;  // collision-list slots of one check wave
-constexpr uint32_t kCollSlots = 4 * kBlock;     // ... of one check block (all its waves)
 constexpr uint32_t kStatSlots = 256;           // spread of the accepted / proposed counters
 
 // Everything one substep of one system needs on the device.
@@ -19,15 +17,12 @@ struct SweepBuffers {
     int32_t *image;           // [3][n_pad] periodic images
     const uint32_t *nbr;
     uint32_t n_pad;
-    uint32_t *lattice;
-    uint32_t *claim;
+    uint32_t *lattice;        // occupancy: 1 bit per monomer, at its lower corner
     // double-buffered by substep parity; selected with pick(), never indexed
     // (a runtime index into an argument struct forces it into scratch memory)
-    uint32_t *coll0, *coll1;
-    uint32_t *clist0, *clist1;    // [check wave][kCollSlotsWave] collision words
-    uint32_t *ccount0, *ccount1;  // [check wave] entries in clist
-    uint2 *surv;              // [check wave][kWave] compacted proposals
-    uint32_t *scount;         // [check wave] proposals
+    uint32_t *claim0, *claim1;    // moved corners of this substep's survivors
+    uint2 *surv0, *surv1;         // [check wave][kWave] compacted proposals
+    uint32_t *scount0, *scount1;  // [check wave] proposals
     unsigned long long *accepted;   // [kStatSlots]
     unsigned long long *proposals;  // [kStatSlots]
     const Tables *tables;     // Morton tables (read only for non-cubic boxes)
@@ -60,14 +55,14 @@ __host__ __device__ inline uint32_t perform_blocks(uint32_t cb) { return (cb + k
 
 // Phase 1 of a substep of parity `parity` for the colour segment
 // [begin, begin+count): every monomer proposes a move and checks it against
-// walls, bonds and the occupancy lattice; survivors claim their face sites and
-// are compacted per block into buf.surv.  Also clears the collision bits that
-// the previous check launch (prev_blocks blocks, other parity) listed.
+// walls, bonds and the occupancy lattice; survivors mark their moved corner in
+// claim[parity] and are compacted per wave into surv[parity].  Also clears
+// the marks of the previous check launch (prev_blocks blocks, other parity).
 hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t parity,
                         uint32_t prev_blocks, hipStream_t stream);
 
-// Phase 2: every survivor whose face sites no other survivor claimed moves;
-// releases the claims.  count is the colour segment size of the check launch.
+// Phase 2: every survivor with no other moved corner in its front moves.
+// count is the colour segment size of the check launch.
 hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t parity, hipStream_t stream);
 
 // ---- many systems per launch -------------------------------------------------

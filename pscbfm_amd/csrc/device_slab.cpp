@@ -16,15 +16,12 @@ SlabLayout slab_layout(const HostSystem &s)
     L.bytes[kPartImage] = sizeof(int32_t) * 3 * size_t(s.n_pad);
     L.bytes[kPartNbr] = sizeof(uint32_t) * kMaxConnectivity * s.n_pad;
     L.bytes[kPartLattice] = sizeof(uint32_t) * s.lattice_words;
-    L.bytes[kPartClaim] = sizeof(uint32_t) * s.lattice_words;
-    L.bytes[kPartColl0] = sizeof(uint32_t) * s.lattice_words;
-    L.bytes[kPartColl1] = sizeof(uint32_t) * s.lattice_words;
-    L.bytes[kPartClist0] = sizeof(uint32_t) * kCollSlots * blocks;
-    L.bytes[kPartClist1] = sizeof(uint32_t) * kCollSlots * blocks;
-    L.bytes[kPartCcount0] = sizeof(uint32_t) * kWavesPerBlock * blocks;
-    L.bytes[kPartCcount1] = sizeof(uint32_t) * kWavesPerBlock * blocks;
-    L.bytes[kPartSurv] = sizeof(uint2) * kBlock * blocks;
-    L.bytes[kPartScount] = sizeof(uint32_t) * kWavesPerBlock * blocks;
+    L.bytes[kPartClaim0] = sizeof(uint32_t) * s.lattice_words;
+    L.bytes[kPartClaim1] = sizeof(uint32_t) * s.lattice_words;
+    L.bytes[kPartSurv0] = sizeof(uint2) * kBlock * blocks;
+    L.bytes[kPartSurv1] = sizeof(uint2) * kBlock * blocks;
+    L.bytes[kPartScount0] = sizeof(uint32_t) * kWavesPerBlock * blocks;
+    L.bytes[kPartScount1] = sizeof(uint32_t) * kWavesPerBlock * blocks;
     L.bytes[kPartAccepted] = sizeof(unsigned long long) * kStatSlots;
     L.bytes[kPartProposals] = sizeof(unsigned long long) * kStatSlots;
     L.bytes[kPartTables] = sizeof(Tables);
@@ -61,15 +58,12 @@ SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, u
     b.nbr = static_cast<const uint32_t *>(slab_part(slab, L, kPartNbr));
     b.n_pad = s.n_pad;
     b.lattice = static_cast<uint32_t *>(slab_part(slab, L, kPartLattice));
-    b.claim = static_cast<uint32_t *>(slab_part(slab, L, kPartClaim));
-    b.coll0 = static_cast<uint32_t *>(slab_part(slab, L, kPartColl0));
-    b.coll1 = static_cast<uint32_t *>(slab_part(slab, L, kPartColl1));
-    b.clist0 = static_cast<uint32_t *>(slab_part(slab, L, kPartClist0));
-    b.clist1 = static_cast<uint32_t *>(slab_part(slab, L, kPartClist1));
-    b.ccount0 = static_cast<uint32_t *>(slab_part(slab, L, kPartCcount0));
-    b.ccount1 = static_cast<uint32_t *>(slab_part(slab, L, kPartCcount1));
-    b.surv = static_cast<uint2 *>(slab_part(slab, L, kPartSurv));
-    b.scount = static_cast<uint32_t *>(slab_part(slab, L, kPartScount));
+    b.claim0 = static_cast<uint32_t *>(slab_part(slab, L, kPartClaim0));
+    b.claim1 = static_cast<uint32_t *>(slab_part(slab, L, kPartClaim1));
+    b.surv0 = static_cast<uint2 *>(slab_part(slab, L, kPartSurv0));
+    b.surv1 = static_cast<uint2 *>(slab_part(slab, L, kPartSurv1));
+    b.scount0 = static_cast<uint32_t *>(slab_part(slab, L, kPartScount0));
+    b.scount1 = static_cast<uint32_t *>(slab_part(slab, L, kPartScount1));
     b.accepted = static_cast<unsigned long long *>(slab_part(slab, L, kPartAccepted));
     b.proposals = static_cast<unsigned long long *>(slab_part(slab, L, kPartProposals));
     b.tables = static_cast<const Tables *>(slab_part(slab, L, kPartTables));

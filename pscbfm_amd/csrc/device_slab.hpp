@@ -17,15 +17,12 @@ enum SlabPart {
     kPartImage,
     kPartNbr,
     kPartLattice,
-    kPartClaim,
-    kPartColl0,
-    kPartColl1,
-    kPartClist0,
-    kPartClist1,
-    kPartCcount0,
-    kPartCcount1,
-    kPartSurv,
-    kPartScount,
+    kPartClaim0,
+    kPartClaim1,
+    kPartSurv0,
+    kPartSurv1,
+    kPartScount0,
+    kPartScount1,
     kPartAccepted,
     kPartProposals,
     kPartTables,

@@ -196,24 +196,34 @@ void plan(HostSystem &s)
                 }
             }
 
-    // occupancy lattice; overlap check (checkSystem, UpdaterGPUScBFM_AB_Type.cu:2808-2851)
+    // overlap check on the full cubes (checkSystem, UpdaterGPUScBFM_AB_Type.cu:
+    // 2808-2851), then the device lattice: one bit per monomer at its lower
+    // corner (the reference's mLatticeOut, initializeLattices :2142-2176)
+    {
+        std::vector<uint32_t> cubes(s.lattice_words, 0u);
+        for (uint32_t i = 0; i < s.n; ++i) {
+            const int32_t *r = &s.xyz[3 * size_t(i)];
+            for (int dz = 0; dz < 2; ++dz)
+                for (int dy = 0; dy < 2; ++dy)
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const uint32_t site = host_site(s.geom, s.tables, r[0] + dx, r[1] + dy, r[2] + dz);
+                        uint32_t &w = cubes[site >> 5];
+                        const uint32_t bit = 1u << (site & 31u);
+                        if (w & bit) {
+                            std::ostringstream msg;
+                            msg << "Occupation count in lattice is wrong: monomer " << i << " at (" << r[0] << ","
+                                << r[1] << "," << r[2] << ") overlaps another monomer";
+                            throw std::runtime_error(msg.str());
+                        }
+                        w |= bit;
+                    }
+        }
+    }
     s.lattice.assign(s.lattice_words, 0u);
     for (uint32_t i = 0; i < s.n; ++i) {
         const int32_t *r = &s.xyz[3 * size_t(i)];
-        for (int dz = 0; dz < 2; ++dz)
-            for (int dy = 0; dy < 2; ++dy)
-                for (int dx = 0; dx < 2; ++dx) {
-                    const uint32_t site = host_site(s.geom, s.tables, r[0] + dx, r[1] + dy, r[2] + dz);
-                    uint32_t &w = s.lattice[site >> 5];
-                    const uint32_t bit = 1u << (site & 31u);
-                    if (w & bit) {
-                        std::ostringstream msg;
-                        msg << "Occupation count in lattice is wrong: monomer " << i << " at (" << r[0] << ","
-                            << r[1] << "," << r[2] << ") overlaps another monomer";
-                        throw std::runtime_error(msg.str());
-                    }
-                    w |= bit;
-                }
+        const uint32_t site = host_site(s.geom, s.tables, r[0], r[1], r[2]);
+        s.lattice[site >> 5] |= 1u << (site & 31u);
     }
 
     // bonds (UpdaterGPUScBFM_AB_Type.cu:2853-2899)

@@ -188,3 +188,16 @@ def test_member_bytes_cover_the_layout():
     n = len(s["xyz"])
     assert b >= 16 * n + 4 * 8 * n + 4 * 64 ** 3 // 8   # positions, partner slots, four bit lattices
     assert b < 200 * n + 8 * 64 ** 3
+
+
+def test_front_sites_arithmetic():
+    """The kernels' 9-site front (at most 4 lattice words and their bit masks,
+    plus the old and moved corner sites) equals a direct enumeration with the
+    lattice's site function, exhaustively for boxes up to 16384 sites and on
+    random corners of larger and non-cubic boxes (tests/cpp/front_sites_check.cpp)."""
+    import subprocess
+    exe = Path(__file__).resolve().parent / "cpp" / "front_sites_check"
+    assert exe.exists(), "built by __graft_entry__.build()"
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.startswith("ok ")
