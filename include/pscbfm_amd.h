@@ -36,10 +36,15 @@ enum {
     BFM_E_STATE = -6
 };
 
-/* Random number generators; value 1 matches Rng::IntHash's slot of the
- * reference enum (UpdaterGPUScBFM_AB_Type.h:413-420) but selects this
- * engine's counter RNG, Philox4x32-10 keyed on (seed, MCS, monomer id). */
-enum { BFM_RNG_PHILOX = 1 };
+/* Random number generators, numbered as the reference's Rng enum
+ * (UpdaterGPUScBFM_AB_Type.h:413-420):
+ *   BFM_RNG_INT_HASH (default, as .cu:1407): rn = H(H(id) ^ s), H the
+ *     integer hash of rngs/Hash.h:39-44, s a 32-bit seed per substep drawn
+ *     from Philox4x32-10(seed, (substep, MCS lo, MCS hi, 3));
+ *   BFM_RNG_PHILOX: rn = word 0 of Philox4x32-10(seed, (id, MCS lo, MCS hi, 0)).
+ * Both are keyed on the original monomer id and the MCS, never on the
+ * layout, so trajectories do not depend on sorting or batching. */
+enum { BFM_RNG_INT_HASH = 1, BFM_RNG_PHILOX = 3 };
 
 typedef struct bfm_updater bfm_updater;
 
@@ -81,12 +86,13 @@ int bfm_set_periodicity(bfm_updater *h, int periodic_x, int periodic_y, int peri
 int bfm_set_age(bfm_updater *h, int64_t age);
 int bfm_get_age(const bfm_updater *h, int64_t *age);
 
-/* setRng, UpdaterGPUScBFM_AB_Type.h:506 (only BFM_RNG_PHILOX). */
+/* setRng, UpdaterGPUScBFM_AB_Type.h:506: BFM_RNG_INT_HASH or BFM_RNG_PHILOX
+ * (Saru, PCG and XORWOW are not provided). */
 int bfm_set_rng(bfm_updater *h, int rng);
 
-/* Seed of the counter RNG.  The reference seeds LeMonADE's r250 from a seed
- * file (src/pscbfm/mainCUDASimulatorGPUScBFM_AB_Type.cpp:123-134); here the
- * 64-bit seed is the Philox key. */
+/* Seed of the RNGs.  The reference seeds LeMonADE's r250 from a seed file
+ * (src/pscbfm/mainCUDASimulatorGPUScBFM_AB_Type.cpp:123-134); here the 64-bit
+ * seed is the Philox key (which also draws the integer hash's substep seeds). */
 int bfm_set_seed(bfm_updater *h, uint64_t seed);
 
 /* setStepsBetweenSortings, UpdaterGPUScBFM_AB_Type.h:510-513.  Accepted for

@@ -49,6 +49,32 @@ uint32_t orc_move_word(uint64_t seed, uint64_t age, uint32_t id)
     return w[0];
 }
 
+uint32_t orc_int_hash(uint32_t a)
+{
+    /* six add/xor-shift steps with the published constants */
+    a = (a + 0x7ed55d16u) + (a << 12);
+    a = (a ^ 0xc761c23cu) ^ (a >> 19);
+    a = (a + 0x165667b1u) + (a << 5);
+    a = (a + 0xd3a2646cu) ^ (a << 9);
+    a = (a + 0xfd7046c5u) + (a << 3);
+    a = (a ^ 0xb55a4f09u) ^ (a >> 16);
+    return a;
+}
+
+uint32_t orc_substep_seed(uint64_t seed, uint64_t age, uint32_t substep)
+{
+    uint32_t w[4];
+    philox_words(seed, substep, age, 3u, w);
+    return w[0];
+}
+
+uint32_t orc_direction_word(const orc_system *s, uint64_t age, uint32_t substep, uint32_t id)
+{
+    if (s->rng == ORC_RNG_PHILOX)
+        return orc_move_word(s->seed, age, id);
+    return orc_int_hash(orc_int_hash(id) ^ orc_substep_seed(s->seed, age, substep));
+}
+
 void orc_color_order(uint64_t seed, uint64_t age, uint32_t n_colors, uint32_t *order)
 {
     for (uint32_t i = 0; i < n_colors; ++i)
@@ -304,7 +330,8 @@ int orc_sweep(const orc_system *s, int32_t *pos, uint64_t age0, uint32_t n_mcs,
               int form, uint64_t *accepted, uint64_t *proposals)
 {
     geom_t g;
-    if (make_geom(s, &g) != 0 || form < 0 || form > 2 || s->n_colors == 0)
+    if (make_geom(s, &g) != 0 || form < 0 || form > 2 || s->n_colors == 0 ||
+        (s->rng != ORC_RNG_INT_HASH && s->rng != ORC_RNG_PHILOX))
         return -1;
     uint64_t n_acc = 0, n_prop = 0;
 
@@ -360,7 +387,7 @@ int orc_sweep(const orc_system *s, int32_t *pos, uint64_t age0, uint32_t n_mcs,
             /* phase 1: propose and check against the state before the substep */
             for (uint32_t t = cnt[c]; t < cnt[c + 1]; ++t) {
                 uint32_t i = members[t];
-                int dir = (int)(orc_move_word(s->seed, age, i) % 6u);
+                int dir = (int)(orc_direction_word(s, age, sub, i) % 6u);
                 const int32_t *r0 = pos + 3 * (size_t)i;
                 int32_t d[3], r1[3];
                 dir_delta(dir, d);

@@ -17,6 +17,7 @@
  *   direction table   src/pscbfm/UpdaterGPUScBFM_AB_Type.cu:1537-1541
  *   invariants        src/pscbfm/UpdaterGPUScBFM_AB_Type.cu:2789-2900
  *   colouring         src/pscbfm/graphColoring.tpp:131-294
+ *   integer-hash RNG  src/pscbfm/rngs/Hash.h:39-44,64-68 (default, .cu:1407)
  *
  * Parity status of each function is stated next to its declaration.
  */
@@ -39,7 +40,11 @@ typedef struct orc_system {
     uint32_t n_colors;
     const uint8_t *forbidden;  /* 512 entries, index (dx&7)|(dy&7)<<3|(dz&7)<<6 */
     uint64_t seed;             /* Philox key                                    */
+    uint32_t rng;              /* ORC_RNG_INT_HASH or ORC_RNG_PHILOX            */
 } orc_system;
+
+/* Values of the reference's Rng enum (UpdaterGPUScBFM_AB_Type.h:413-420). */
+enum { ORC_RNG_INT_HASH = 1, ORC_RNG_PHILOX = 3 };
 
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
  * Pinned by the published Random123 known-answer vectors (tests/test_oracle.py)
@@ -50,6 +55,24 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
  * word 0 of Philox(key = seed, ctr = (id, age_lo, age_hi, 0)).
  * Direction = word % 6 as in UpdaterGPUScBFM_AB_Type.cu:966. */
 uint32_t orc_move_word(uint64_t seed, uint64_t age, uint32_t id);
+
+/* The 32-bit integer mix of the reference's default RNG (Rngs::Hash,
+ * rngs/Hash.h:39-44; Robert Jenkins' six-step integer hash).  Pinned by
+ * vectors of the reference header itself, built from its source into
+ * oracle/_ref (tests/golden/int_hash_vectors.json), and by being a bijection. */
+uint32_t orc_int_hash(uint32_t a);
+
+/* Seed of substep `substep` (0-based slot in the MCS) of MCS `age`: word 0 of
+ * Philox(seed, (substep, age_lo, age_hi, 3)).  The reference draws this
+ * 32-bit value from LeMonADE's r250 on the host (.cu:3066). */
+uint32_t orc_substep_seed(uint64_t seed, uint64_t age, uint32_t substep);
+
+/* Direction word of monomer `id` in substep `substep` of MCS `age`:
+ * ORC_RNG_INT_HASH: H(H(id) ^ orc_substep_seed(...)), the reference's
+ *   Hash::rng32 with subsequence = monomer (Hash.h:64-68, .cu:961-964; keyed
+ *   here on the original id instead of the sorted index);
+ * ORC_RNG_PHILOX: orc_move_word(seed, age, id). */
+uint32_t orc_direction_word(const orc_system *sys, uint64_t age, uint32_t substep, uint32_t id);
 
 /* Colour order of MCS `age`: Fisher-Yates over 0..n-1, draw for slot i is
  * word 0 of Philox(seed, (i, age_lo, age_hi, 1)) % (i+1).  Every colour is

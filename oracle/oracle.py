@@ -28,7 +28,12 @@ class OrcSystem(ctypes.Structure):
         ("n_colors", c_uint32),
         ("forbidden", POINTER(c_uint8)),
         ("seed", c_uint64),
+        ("rng", c_uint32),
     ]
+
+
+RNG_INT_HASH = 1   # the reference's default Rngs::Hash (Rng::IntHash)
+RNG_PHILOX = 3     # Rng::Philox
 
 
 _lib = None
@@ -49,6 +54,12 @@ def load() -> ctypes.CDLL:
         lib.orc_philox4x32_10.restype = None
         lib.orc_move_word.argtypes = [c_uint64, c_uint64, c_uint32]
         lib.orc_move_word.restype = c_uint32
+        lib.orc_int_hash.argtypes = [c_uint32]
+        lib.orc_int_hash.restype = c_uint32
+        lib.orc_substep_seed.argtypes = [c_uint64, c_uint64, c_uint32]
+        lib.orc_substep_seed.restype = c_uint32
+        lib.orc_direction_word.argtypes = [POINTER(OrcSystem), c_uint64, c_uint32, c_uint32]
+        lib.orc_direction_word.restype = c_uint32
         lib.orc_color_order.argtypes = [c_uint64, c_uint64, c_uint32, c_void_p]
         lib.orc_color_order.restype = None
         lib.orc_standard_bondset.argtypes = [c_void_p]
@@ -79,6 +90,18 @@ def move_word(seed: int, age: int, mono_id: int) -> int:
     return int(load().orc_move_word(seed, age, mono_id))
 
 
+def int_hash(a: int) -> int:
+    return int(load().orc_int_hash(a & 0xFFFFFFFF))
+
+
+def substep_seed(seed: int, age: int, substep: int) -> int:
+    return int(load().orc_substep_seed(seed & 0xFFFFFFFFFFFFFFFF, age, substep))
+
+
+def direction_word(sys: "System", age: int, substep: int, mono_id: int) -> int:
+    return int(load().orc_direction_word(ctypes.byref(sys.struct), age, substep, mono_id))
+
+
 def color_order(seed: int, age: int, n_colors: int) -> list[int]:
     out = np.zeros(n_colors, dtype=np.uint32)
     load().orc_color_order(seed, age, n_colors, out.ctypes.data)
@@ -107,7 +130,7 @@ def csr(n: int, bonds: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
 class System:
     """Holds the arrays behind an OrcSystem so they outlive the struct."""
 
-    def __init__(self, system: dict, seed: int, colors: np.ndarray | None = None):
+    def __init__(self, system: dict, seed: int, colors: np.ndarray | None = None, rng: int = RNG_INT_HASH):
         self.xyz0 = np.ascontiguousarray(system["xyz"], dtype=np.int32).reshape(-1, 3)
         n = self.xyz0.shape[0]
         self.off, self.idx = csr(n, system["bonds"])
@@ -129,6 +152,7 @@ class System:
         s.n_colors = nc
         s.forbidden = self.forbidden.ctypes.data_as(POINTER(c_uint8))
         s.seed = seed & 0xFFFFFFFFFFFFFFFF
+        s.rng = rng
         self.struct = s
         self.n_colors = nc
 

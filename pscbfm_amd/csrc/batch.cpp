@@ -196,7 +196,7 @@ void BatchEngine::runChunk(uint32_t mcs)
             colour_order(M.seed, age, nc, order.data());
             for (uint32_t k = 0; k < mMaxColours; ++k) {
                 SchedEntry &e = sched[size_t(m * mMaxColours + k) * n_sys + j];
-                e.pad = 0;
+                e.sub_seed = k < nc ? substep_seed(M.seed, age, k) : 0u;
                 e.age = age;
                 if (k < nc) {
                     const uint32_t c = order[k];

@@ -103,6 +103,32 @@ __host__ __device__ __forceinline__ uint32_t philox_word0(uint64_t seed, uint32_
 
 constexpr uint32_t kTagMove = 0;
 constexpr uint32_t kTagColour = 1;
+constexpr uint32_t kTagSubstep = 3;
+
+// Random number generators, numbered as the reference's Rng enum
+// (UpdaterGPUScBFM_AB_Type.h:413-420).
+constexpr uint32_t kRngIntHash = 1;   // the reference's default (.cu:1407)
+constexpr uint32_t kRngPhilox = 3;
+
+// 32-bit integer mix of the reference's Rngs::Hash (rngs/Hash.h:39-44):
+// Robert Jenkins' six-step integer hash with its published constants.
+__host__ __device__ __forceinline__ uint32_t int_hash(uint32_t a)
+{
+    a = (a + 0x7ed55d16u) + (a << 12);
+    a = (a ^ 0xc761c23cu) ^ (a >> 19);
+    a = (a + 0x165667b1u) + (a << 5);
+    a = (a + 0xd3a2646cu) ^ (a << 9);
+    a = (a + 0xfd7046c5u) + (a << 3);
+    a = (a ^ 0xb55a4f09u) ^ (a >> 16);
+    return a;
+}
+
+// Seed of substep `substep` of MCS `age` for the integer-hash generator (the
+// reference draws it from LeMonADE's r250 on the host, .cu:3066).
+__host__ __device__ __forceinline__ uint32_t substep_seed(uint64_t seed, uint64_t age, uint32_t substep)
+{
+    return philox_word0(seed, substep, age, kTagSubstep);
+}
 
 // Colour order of one MCS: Fisher-Yates, slot i swaps with word % (i+1).
 inline void colour_order(uint64_t seed, uint64_t age, uint32_t n, uint32_t *order)

@@ -90,7 +90,8 @@ using DepositFor = typename std::conditional<kCubic, DepositCubic, DepositTable>
 // system (other parity), whose claims this launch clears.
 template <bool kAllPeriodic, bool kCubic>
 __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
-                                           uint32_t parity, uint32_t prev_blocks, uint32_t blk, uint32_t nblk)
+                                           uint32_t sub_seed, uint32_t parity, uint32_t prev_blocks, uint32_t blk,
+                                           uint32_t nblk)
 {
     const DepositFor<kCubic> dep{};
     const Geometry &g = buf.geom;
@@ -128,7 +129,13 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
                 partner[j] = buf.nbr[size_t(j) * buf.n_pad + i];
 
         const int4 r0 = unpack_xyz(p0);
-        const int dir = int(philox_word0(buf.seed, id, age, kTagMove) % 6u);
+        // direction = rn % 6 (.cu:964-967); the generator is uniform per launch
+        uint32_t rn;
+        if (buf.rng == kRngPhilox)
+            rn = philox_word0(buf.seed, id, age, kTagMove);
+        else
+            rn = int_hash(int_hash(id) ^ sub_seed);
+        const int dir = int(rn % 6u);
         const int4 r1 = moved(r0, dir);
 
         bool ok = true;
@@ -199,14 +206,14 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
 
 template <bool kAllPeriodic, bool kCubic>
 __global__ __launch_bounds__(kBlock) void k_check(SweepBuffers buf, uint32_t begin, uint32_t count, uint64_t age,
-                                                  uint32_t parity, uint32_t prev_blocks)
+                                                  uint32_t sub_seed, uint32_t parity, uint32_t prev_blocks)
 {
     load_bond(buf.tables);
     if (!kCubic)
         load_dep(buf.tables);
     __syncthreads();
     const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
-    check_body<kAllPeriodic, kCubic>(buf, begin, count, age, parity, prev_blocks, blk, gridDim.x);
+    check_body<kAllPeriodic, kCubic>(buf, begin, count, age, sub_seed, parity, prev_blocks, blk, gridDim.x);
 }
 
 // Phase 2: every wave of perform block `blk` takes the survivors of
@@ -317,7 +324,8 @@ __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__res
     if (!kCubic)
         load_dep(S.buf.tables);
     __syncthreads();
-    check_body<kAllPeriodic, kCubic>(S.buf, S.seg_begin[e.colour], count, e.age, e.parity, e.prev_blocks, blk, nb);
+    check_body<kAllPeriodic, kCubic>(S.buf, S.seg_begin[e.colour], count, e.age, e.sub_seed, e.parity,
+                                     e.prev_blocks, blk, nb);
 }
 
 template <bool kCubic>
@@ -353,22 +361,22 @@ __global__ __launch_bounds__(kBlock) void k_copy(const uint4 *__restrict__ src, 
 
 }  // namespace
 
-hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t parity,
-                        uint32_t prev_blocks, hipStream_t stream)
+hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t sub_seed,
+                        uint32_t parity, uint32_t prev_blocks, hipStream_t stream)
 {
     const dim3 grid(check_blocks(count)), block(kBlock);
     const bool periodic = buf.geom.periodic == 7u, cubic = buf.geom.cubic != 0u;
     if (periodic && cubic)
-        hipLaunchKernelGGL((k_check<true, true>), grid, block, 0, stream, buf, begin, count, age, parity, prev_blocks);
+        hipLaunchKernelGGL((k_check<true, true>), grid, block, 0, stream, buf, begin, count, age, sub_seed, parity, prev_blocks);
     else if (periodic)
-        hipLaunchKernelGGL((k_check<true, false>), grid, block, 0, stream, buf, begin, count, age, parity,
-                           prev_blocks);
+        hipLaunchKernelGGL((k_check<true, false>), grid, block, 0, stream, buf, begin, count, age, sub_seed,
+                           parity, prev_blocks);
     else if (cubic)
-        hipLaunchKernelGGL((k_check<false, true>), grid, block, 0, stream, buf, begin, count, age, parity,
-                           prev_blocks);
+        hipLaunchKernelGGL((k_check<false, true>), grid, block, 0, stream, buf, begin, count, age, sub_seed,
+                           parity, prev_blocks);
     else
-        hipLaunchKernelGGL((k_check<false, false>), grid, block, 0, stream, buf, begin, count, age, parity,
-                           prev_blocks);
+        hipLaunchKernelGGL((k_check<false, false>), grid, block, 0, stream, buf, begin, count, age, sub_seed,
+                           parity, prev_blocks);
     return hipGetLastError();
 }
 

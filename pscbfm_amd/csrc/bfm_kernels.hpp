@@ -28,6 +28,7 @@ struct SweepBuffers {
     const Tables *tables;     // Morton tables (read only for non-cubic boxes)
     Geometry geom;
     uint64_t seed;
+    uint32_t rng;             // kRngIntHash or kRngPhilox
     uint32_t xcd_ranges;      // 1: logical blocks laid out as one contiguous range per XCD
 };
 
@@ -58,8 +59,9 @@ __host__ __device__ inline uint32_t perform_blocks(uint32_t cb) { return (cb + k
 // walls, bonds and the occupancy lattice; survivors mark their moved corner in
 // claim[parity] and are compacted per wave into surv[parity].  Also clears
 // the marks of the previous check launch (prev_blocks blocks, other parity).
-hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t parity,
-                        uint32_t prev_blocks, hipStream_t stream);
+// sub_seed: substep_seed() of this substep (used by the integer-hash RNG).
+hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t sub_seed,
+                        uint32_t parity, uint32_t prev_blocks, hipStream_t stream);
 
 // Phase 2: every survivor with no other moved corner in its front moves.
 // count is the colour segment size of the check launch.
@@ -84,7 +86,7 @@ struct SchedEntry {
     uint32_t colour;       // kNoColour: no substep for this system in this slot
     uint32_t parity;       // substep parity of this system
     uint32_t prev_blocks;  // grid of this system's previous check launch
-    uint32_t pad;
+    uint32_t sub_seed;     // substep_seed() of this substep
 };
 
 // Kernel variant class of a system: periodic on every axis, cubic box.

@@ -69,6 +69,7 @@ SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, u
     b.tables = static_cast<const Tables *>(slab_part(slab, L, kPartTables));
     b.geom = s.geom;
     b.seed = seed;
+    b.rng = s.rng;
     return b;
 }
 

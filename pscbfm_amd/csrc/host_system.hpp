@@ -25,6 +25,7 @@ struct HostSystem {
     std::vector<uint32_t> nbr;       // n * kMaxConnectivity, directed links as given
     uint8_t forbidden[512];          // 1 = forbidden (copyBondSet)
     bool spatial_order = true;       // lay colours out in Morton order of position
+    uint32_t rng = kRngIntHash;      // setRng
 
     // ---- derived by plan() --------------------------------------------------
     Geometry geom{};

@@ -146,11 +146,14 @@ void UpdaterGPUScBFM_AB_Type::setPeriodicity(bool px, bool py, bool pz)
 
 void UpdaterGPUScBFM_AB_Type::setRng(int rng)
 {
-    if (rng != 1) {
+    if (rng != int(kRngIntHash) && rng != int(kRngPhilox)) {
         std::ostringstream msg;
-        msg << "random number generator " << rng << " is not available; only 1 (Philox4x32-10) is";
+        msg << "random number generator " << rng << " is not available; only " << kRngIntHash
+            << " (IntHash) and " << kRngPhilox << " (Philox) are";
         throw std::invalid_argument(msg.str());
     }
+    mSys.rng = uint32_t(rng);
+    mBuf.rng = uint32_t(rng);
 }
 
 void UpdaterGPUScBFM_AB_Type::setStepsBetweenSortings(int64_t steps)
@@ -229,7 +232,8 @@ void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs)
                     x = takeEvent();
                 hip_check(hipEventRecord(ev[0], mStream), "hipEventRecord");
             }
-            hip_check(launch_check(mBuf, s.seg_begin[c], s.seg_count[c], age, parity, mPrevBlocks, mStream),
+            hip_check(launch_check(mBuf, s.seg_begin[c], s.seg_count[c], age, substep_seed(mSeed, age, k), parity,
+                                   mPrevBlocks, mStream),
                       "check kernel");
             mPrevBlocks = check_blocks(s.seg_count[c]);
             if (mTiming)

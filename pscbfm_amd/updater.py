@@ -11,7 +11,10 @@ import numpy as np
 
 from . import _capi
 
-Rng_Philox = 1
+# Random number generators, numbered as the reference's Rng enum
+# (UpdaterGPUScBFM_AB_Type.h:413-420).
+Rng_IntHash = 1   # the reference's default (.cu:1407)
+Rng_Philox = 3
 
 
 class UpdaterGPUScBFM_AB_Type:
@@ -148,7 +151,8 @@ class UpdaterGPUScBFM_AB_Type:
 
 
 def updater_from_system(system: dict, seed: int, gpu: int | None = 0, age: int = 0,
-                        validate_only: bool = False, spatial_sorting: bool = True) -> UpdaterGPUScBFM_AB_Type:
+                        validate_only: bool = False, spatial_sorting: bool = True,
+                        rng: int = Rng_IntHash) -> UpdaterGPUScBFM_AB_Type:
     """Builds an updater the way the LeMonADE adapter does
     (GPUScBFM_AB_Type.h:120-184): box, periodicity, age, positions, both
     directions of every bond, the 512-entry bond table, then initialize."""
@@ -176,6 +180,7 @@ def updater_from_system(system: dict, seed: int, gpu: int | None = 0, age: int =
                 idx = (dx & 7) | ((dy & 7) << 3) | ((dz & 7) << 6)
                 u.copyBondSet(dx, dy, dz, bool(forbidden[idx]))
     u.setSpatialSorting(spatial_sorting)
+    u.setRng(rng)
     if validate_only:
         u.validate()
     else:

@@ -14,25 +14,25 @@ from pscbfm_amd import systems, updater_from_system
 pytestmark = pytest.mark.gpu
 
 
-def run_pair(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True):
+def run_pair(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True, rng=orc.RNG_INT_HASH):
     n = len(system["xyz"])
     print(f"case: n={n} box={system['box']} periodic={system.get('periodic')} seed={seed:#x} "
-          f"chunks={chunks} age0={age0} xcd_ranges={xcd_ranges} spatial={spatial}", flush=True)
-    u = updater_from_system(system, seed=seed, gpu=0, age=age0, spatial_sorting=spatial)
+          f"chunks={chunks} age0={age0} xcd_ranges={xcd_ranges} spatial={spatial} rng={rng}", flush=True)
+    u = updater_from_system(system, seed=seed, gpu=0, age=age0, spatial_sorting=spatial, rng=rng)
     u.setXcdRanges(xcd_ranges)
     for c in chunks:
         u.runSimulationOnGPU(c)
     got = u.getMonomerPositions(n)
     colors, nc = u.getColors(n)
     stats = u.getStats()
-    osys = orc.System(system, seed, colors)
+    osys = orc.System(system, seed, colors, rng=rng)
     want, acc, prop = orc.sweep(osys, system["xyz"], age0, sum(chunks), 0)
     u.cleanup()
     return got, want, stats, acc, prop, osys
 
 
-def assert_parity(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True):
-    got, want, stats, acc, prop, osys = run_pair(system, seed, chunks, age0, xcd_ranges, spatial)
+def assert_parity(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True, rng=orc.RNG_INT_HASH):
+    got, want, stats, acc, prop, osys = run_pair(system, seed, chunks, age0, xcd_ranges, spatial, rng)
     diff = np.nonzero((got != want).any(axis=1))[0]
     assert diff.size == 0, f"{diff.size} monomers differ, first {diff[:5]}: gpu {got[diff[:3]]} oracle {want[diff[:3]]}"
     assert stats["accepted"] == acc
@@ -71,6 +71,12 @@ def test_resume_is_continuous(gpu_ready):
     # counter RNG keyed on the age: split runs equal one long run
     s = systems.linear_melt(32, 64, 16, seed=5)
     assert_parity(s, seed=99, chunks=[37, 63], age0=1000)
+
+
+def test_philox_generator(gpu_ready):
+    # the reference's Rng::Philox slot: Philox4x32-10 keyed on (seed; id, MCS)
+    s = systems.linear_melt(64, 512, 32, seed=23)
+    assert_parity(s, seed=0xC0FFEE, chunks=[200, 100], age0=77, rng=orc.RNG_PHILOX)
 
 
 def test_hard_walls(gpu_ready):

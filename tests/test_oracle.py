@@ -137,18 +137,24 @@ def test_three_forms_agree(case, periodic):
     assert orc.check_system(sys_, outs[0][0])[0] == 0
 
 
-def test_single_monomer_walk_closed_form():
+@pytest.mark.parametrize("rng", [orc.RNG_INT_HASH, orc.RNG_PHILOX])
+def test_single_monomer_walk_closed_form(rng):
     """One free monomer: every proposal is accepted, so the trajectory is the
     sum of the direction table entries (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z,
-    UpdaterGPUScBFM_AB_Type.cu:1537-1541) over the move words."""
+    UpdaterGPUScBFM_AB_Type.cu:1537-1541) over the move words: Philox word 0,
+    or H(H(id) ^ substep seed) with the hash pinned by the reference build."""
     s = {"box": (16, 16, 16), "periodic": (1, 1, 1), "xyz": np.array([[3, 4, 5]], dtype=np.int32),
          "bonds": np.zeros((0, 2), dtype=np.uint32), "forbidden": orc.standard_bondset()}
-    sys_ = orc.System(s, 2024)
+    sys_ = orc.System(s, 2024, rng=rng)
     x, acc, prop = orc.sweep(sys_, s["xyz"], 7, 200, 0)
     table = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
     want = np.array([3, 4, 5])
     for age in range(7, 207):
-        want += table[orc.move_word(2024, age, 0) % 6]
+        if rng == orc.RNG_PHILOX:
+            word = orc.move_word(2024, age, 0)
+        else:
+            word = orc.int_hash(orc.int_hash(0) ^ orc.substep_seed(2024, age, 0))
+        want += table[word % 6]
     assert acc == prop == 200
     assert np.array_equal(x[0], want)
 
@@ -226,3 +232,68 @@ def test_three_forms_agree_on_network():
         assert np.array_equal(x, outs[0][0])
         assert (acc, prop) == outs[0][1:]
     assert orc.check_system(sys_, outs[0][0])[0] == 0
+
+
+# ---- integer-hash generator (the reference's default Rngs::Hash) -------------
+
+def _int_hash_vectors():
+    import json
+    from pathlib import Path
+    p = Path(__file__).resolve().parent / "golden" / "int_hash_vectors.json"
+    return json.loads(p.read_text())["vectors"]
+
+
+def test_int_hash_matches_reference_build_vectors():
+    """First rng32() of the reference's Rngs::Hash (built from its own header
+    into oracle/_ref by oracle/ref.mk) for 64 (seed, subsequence) pairs."""
+    for v in _int_hash_vectors():
+        got = orc.int_hash(orc.int_hash(v["subsequence"] & 0xFFFFFFFF) ^ (v["seed"] & 0xFFFFFFFF))
+        assert got == v["rng32"], v
+
+
+def test_int_hash_against_reference_build_live():
+    """Where the reference tree is present, 4096 more random pairs straight
+    from the reference-built program."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parent.parent / "oracle" / "_ref" / "hash_ref"
+    if not exe.exists():
+        pytest.skip("oracle/_ref/hash_ref not built (reference tree absent)")
+    rng = np.random.default_rng(5)
+    pairs = list(zip(rng.integers(0, 2**32, 4096).tolist(), rng.integers(0, 2**32, 4096).tolist()))
+    out = subprocess.run([str(exe)], input="".join(f"{a} {b}\n" for a, b in pairs), capture_output=True,
+                         text=True, check=True).stdout.split()
+    for (a, b), v in zip(pairs, out):
+        assert orc.int_hash(orc.int_hash(b) ^ a) == int(v)
+
+
+def test_int_hash_is_a_bijection_on_a_block():
+    # each of its six steps is invertible, so distinct inputs stay distinct
+    vals = {orc.int_hash(x) for x in range(1 << 16)}
+    assert len(vals) == 1 << 16
+
+
+def test_int_hash_directions_are_uniform():
+    from pscbfm_amd import systems
+    s = systems.linear_melt(32, 128, 16, seed=3)
+    o = orc.System(s, 99)
+    counts = np.zeros(6)
+    for age in range(4):
+        for sub in range(2):
+            for i in range(0, 2048, 3):
+                counts[orc.direction_word(o, age, sub, i) % 6] += 1
+    expected = counts.sum() / 6
+    chi2 = float(((counts - expected) ** 2 / expected).sum())
+    assert chi2 < 20.5, counts          # 5 dof, p ~ 0.001
+
+
+def test_generators_give_different_but_valid_trajectories():
+    from pscbfm_amd import systems
+    s = systems.linear_melt(32, 64, 16, seed=8)
+    a = orc.System(s, 17, rng=orc.RNG_INT_HASH)
+    b = orc.System(s, 17, rng=orc.RNG_PHILOX)
+    xa, acc_a, _ = orc.sweep(a, s["xyz"], 0, 50, 1)
+    xb, acc_b, _ = orc.sweep(b, s["xyz"], 0, 50, 1)
+    assert not np.array_equal(xa, xb)
+    assert orc.check_system(a, xa)[0] == 0 and orc.check_system(b, xb)[0] == 0
+    assert 0.5 < acc_a / acc_b < 2.0
