@@ -197,18 +197,16 @@ void BatchEngine::runChunk(uint32_t mcs)
             for (uint32_t k = 0; k < mMaxColours; ++k) {
                 SchedEntry &e = sched[size_t(m * mMaxColours + k) * n_sys + j];
                 e.sub_seed = k < nc ? substep_seed(M.seed, age, k) : 0u;
+                e.pad = 0;
                 e.age = age;
                 if (k < nc) {
                     const uint32_t c = order[k];
                     e.colour = c;
                     e.parity = uint32_t(M.substeps & 1u);
-                    e.prev_blocks = M.prev_blocks;
                     ++M.substeps;
-                    M.prev_blocks = check_blocks(M.sys.seg_count[c]);
                 } else {
                     e.colour = kNoColour;
                     e.parity = 0;
-                    e.prev_blocks = 0;
                 }
             }
         }

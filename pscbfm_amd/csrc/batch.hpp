@@ -67,7 +67,6 @@ private:
         uint64_t seed = 0;
         int64_t age = 0;
         uint64_t substeps = 0;
-        uint32_t prev_blocks = 0;
         void *slab = nullptr;
         size_t bytes = 0;
         BatchSystem dev{};

@@ -20,7 +20,7 @@
 //    :1222-1228) and no global counter is contended; the only workgroup
 //    barrier is the one after the LDS tables are filled at kernel start;
 //  * no memset: the marks of a substep are cleared by the next check launch
-//    (claims and survivor lists are double-buffered by substep parity);
+//    (the claim lattice is double-buffered by substep parity);
 //  * an accepted move flips its 2 corner bits with one atomic XOR per word.
 #include "bfm_kernels.hpp"
 
@@ -85,28 +85,17 @@ template <bool kCubic>
 using DepositFor = typename std::conditional<kCubic, DepositCubic, DepositTable>::type;
 
 // Phase 1 for check block `blk` of `nblk` covering the colour segment
-// [begin, begin+count).  Every wavefront compacts its own survivors into
-// surv[parity]; prev_blocks is the grid of the previous check launch of this
-// system (other parity), whose claims this launch clears.
+// [begin, begin+count).  Every wavefront compacts its own survivors into surv.
 template <bool kAllPeriodic, bool kCubic>
 __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
-                                           uint32_t sub_seed, uint32_t parity, uint32_t prev_blocks, uint32_t blk,
-                                           uint32_t nblk)
+                                           uint32_t sub_seed, uint32_t parity, uint32_t blk, uint32_t nblk)
 {
     const DepositFor<kCubic> dep{};
     const Geometry &g = buf.geom;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t gwave = wave_uniform(blk * kWavesPerBlock + (threadIdx.x >> 6));
-    const uint32_t nwaves = nblk * kWavesPerBlock;
-    const uint32_t prev_waves = prev_blocks * kWavesPerBlock;
-
     uint32_t *claim_cur = pick(parity, buf.claim0, buf.claim1);
     uint32_t *claim_prev = pick(parity, buf.claim1, buf.claim0);
-    uint2 *surv_cur = pick(parity, buf.surv0, buf.surv1);
-    const uint2 *surv_prev = pick(parity, buf.surv1, buf.surv0);
-    const uint32_t *scount_prev = pick(parity, buf.scount1, buf.scount0);
-    // survivors of the previous launch's wave with this index, issued early
-    const uint32_t n_prev_own = gwave < prev_waves ? scount_prev[gwave] : 0u;
 
     const uint32_t t = blk * kBlock + threadIdx.x;
     bool pass = false;
@@ -184,36 +173,30 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
     // wave compaction of the survivors
     const uint64_t mball = __ballot(pass);
     if (pass)
-        surv_cur[size_t(gwave) * kWave + lanes_below(mball)] = entry;
+        buf.surv[size_t(gwave) * kWave + lanes_below(mball)] = entry;
     if (lane == 0)
-        pick(parity, buf.scount0, buf.scount1)[gwave] = uint32_t(__popcll(mball));
+        buf.scount[gwave] = uint32_t(__popcll(mball));
 
-    // claims of the previous check launch (the reference zeroes its tentative
+    // the previous substep's claims (the reference zeroes its tentative
     // lattice in a third kernel, :1240-1262): nobody reads claim[parity^1] in
-    // this kernel, and its words hold only those claims, so a plain store of
-    // zero clears them
-    for (uint32_t pw = gwave; pw < prev_waves; pw += nwaves) {
-        const uint32_t n = pw == gwave ? n_prev_own : scount_prev[pw];
-        for (uint32_t k = lane; k < n; k += kWave) {
-            const uint2 e = surv_prev[size_t(pw) * kWave + k];
-            const int4 r = moved(unpack_xyz(e.y), int(e.x >> kSortedIndexBits));
-            const int4 rw = make_int4(int32_t(uint32_t(r.x) & g.mask[0]), int32_t(uint32_t(r.y) & g.mask[1]),
-                                      int32_t(uint32_t(r.z) & g.mask[2]), 0);
-            claim_prev[corner_site(dep, rw) >> 5] = 0u;
-        }
-    }
+    // this kernel, so the grid clears all of it with coalesced 16-byte
+    // stores; at the densities of a melt a survivor's mark shares every
+    // 64-byte line with others, so clearing marks one by one writes more
+    uint4 *cp = reinterpret_cast<uint4 *>(claim_prev);
+    for (uint32_t k = t; k < buf.lattice_words / 4u; k += nblk * kBlock)
+        cp[k] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 template <bool kAllPeriodic, bool kCubic>
 __global__ __launch_bounds__(kBlock) void k_check(SweepBuffers buf, uint32_t begin, uint32_t count, uint64_t age,
-                                                  uint32_t sub_seed, uint32_t parity, uint32_t prev_blocks)
+                                                  uint32_t sub_seed, uint32_t parity)
 {
     load_bond(buf.tables);
     if (!kCubic)
         load_dep(buf.tables);
     __syncthreads();
     const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
-    check_body<kAllPeriodic, kCubic>(buf, begin, count, age, sub_seed, parity, prev_blocks, blk, gridDim.x);
+    check_body<kAllPeriodic, kCubic>(buf, begin, count, age, sub_seed, parity, blk, gridDim.x);
 }
 
 // Phase 2: every wave of perform block `blk` takes the survivors of
@@ -229,8 +212,8 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
     const uint32_t check_waves = check_blocks_ * kWavesPerBlock;
     const uint32_t cw0 = pwave * kPerformGroup;
     const uint32_t *claim = pick(parity, buf.claim0, buf.claim1);
-    const uint2 *surv = pick(parity, buf.surv0, buf.surv1);
-    const uint32_t *scount = pick(parity, buf.scount0, buf.scount1);
+    const uint2 *surv = buf.surv;
+    const uint32_t *scount = buf.scount;
 
     uint32_t c = 0;
     if (lane < kPerformGroup && cw0 + lane < check_waves)
@@ -324,8 +307,7 @@ __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__res
     if (!kCubic)
         load_dep(S.buf.tables);
     __syncthreads();
-    check_body<kAllPeriodic, kCubic>(S.buf, S.seg_begin[e.colour], count, e.age, e.sub_seed, e.parity,
-                                     e.prev_blocks, blk, nb);
+    check_body<kAllPeriodic, kCubic>(S.buf, S.seg_begin[e.colour], count, e.age, e.sub_seed, e.parity, blk, nb);
 }
 
 template <bool kCubic>
@@ -362,21 +344,21 @@ __global__ __launch_bounds__(kBlock) void k_copy(const uint4 *__restrict__ src, 
 }  // namespace
 
 hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t sub_seed,
-                        uint32_t parity, uint32_t prev_blocks, hipStream_t stream)
+                        uint32_t parity, hipStream_t stream)
 {
     const dim3 grid(check_blocks(count)), block(kBlock);
     const bool periodic = buf.geom.periodic == 7u, cubic = buf.geom.cubic != 0u;
     if (periodic && cubic)
-        hipLaunchKernelGGL((k_check<true, true>), grid, block, 0, stream, buf, begin, count, age, sub_seed, parity, prev_blocks);
+        hipLaunchKernelGGL((k_check<true, true>), grid, block, 0, stream, buf, begin, count, age, sub_seed, parity);
     else if (periodic)
         hipLaunchKernelGGL((k_check<true, false>), grid, block, 0, stream, buf, begin, count, age, sub_seed,
-                           parity, prev_blocks);
+                           parity);
     else if (cubic)
         hipLaunchKernelGGL((k_check<false, true>), grid, block, 0, stream, buf, begin, count, age, sub_seed,
-                           parity, prev_blocks);
+                           parity);
     else
         hipLaunchKernelGGL((k_check<false, false>), grid, block, 0, stream, buf, begin, count, age, sub_seed,
-                           parity, prev_blocks);
+                           parity);
     return hipGetLastError();
 }
 

@@ -18,11 +18,13 @@ struct SweepBuffers {
     const uint32_t *nbr;
     uint32_t n_pad;
     uint32_t *lattice;        // occupancy: 1 bit per monomer, at its lower corner
-    // double-buffered by substep parity; selected with pick(), never indexed
-    // (a runtime index into an argument struct forces it into scratch memory)
-    uint32_t *claim0, *claim1;    // moved corners of this substep's survivors
-    uint2 *surv0, *surv1;         // [check wave][kWave] compacted proposals
-    uint32_t *scount0, *scount1;  // [check wave] proposals
+    uint32_t lattice_words;   // words of lattice and of each claim lattice (a power of two >= 16)
+    // moved corners of this substep's survivors, double-buffered by substep
+    // parity; selected with pick(), never indexed (a runtime index into an
+    // argument struct forces it into scratch memory)
+    uint32_t *claim0, *claim1;
+    uint2 *surv;              // [check wave][kWave] compacted proposals
+    uint32_t *scount;         // [check wave] proposals
     unsigned long long *accepted;   // [kStatSlots]
     unsigned long long *proposals;  // [kStatSlots]
     const Tables *tables;     // Morton tables (read only for non-cubic boxes)
@@ -57,11 +59,11 @@ __host__ __device__ inline uint32_t perform_blocks(uint32_t cb) { return (cb + k
 // Phase 1 of a substep of parity `parity` for the colour segment
 // [begin, begin+count): every monomer proposes a move and checks it against
 // walls, bonds and the occupancy lattice; survivors mark their moved corner in
-// claim[parity] and are compacted per wave into surv[parity].  Also clears
-// the marks of the previous check launch (prev_blocks blocks, other parity).
+// claim[parity] and are compacted per wave into surv.  Also clears
+// claim[parity ^ 1], the marks of the previous substep.
 // sub_seed: substep_seed() of this substep (used by the integer-hash RNG).
 hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t sub_seed,
-                        uint32_t parity, uint32_t prev_blocks, hipStream_t stream);
+                        uint32_t parity, hipStream_t stream);
 
 // Phase 2: every survivor with no other moved corner in its front moves.
 // count is the colour segment size of the check launch.
@@ -85,8 +87,8 @@ struct SchedEntry {
     uint64_t age;          // MCS that keys the RNG
     uint32_t colour;       // kNoColour: no substep for this system in this slot
     uint32_t parity;       // substep parity of this system
-    uint32_t prev_blocks;  // grid of this system's previous check launch
     uint32_t sub_seed;     // substep_seed() of this substep
+    uint32_t pad;
 };
 
 // Kernel variant class of a system: periodic on every axis, cubic box.

@@ -18,10 +18,8 @@ SlabLayout slab_layout(const HostSystem &s)
     L.bytes[kPartLattice] = sizeof(uint32_t) * s.lattice_words;
     L.bytes[kPartClaim0] = sizeof(uint32_t) * s.lattice_words;
     L.bytes[kPartClaim1] = sizeof(uint32_t) * s.lattice_words;
-    L.bytes[kPartSurv0] = sizeof(uint2) * kBlock * blocks;
-    L.bytes[kPartSurv1] = sizeof(uint2) * kBlock * blocks;
-    L.bytes[kPartScount0] = sizeof(uint32_t) * kWavesPerBlock * blocks;
-    L.bytes[kPartScount1] = sizeof(uint32_t) * kWavesPerBlock * blocks;
+    L.bytes[kPartSurv] = sizeof(uint2) * kBlock * blocks;
+    L.bytes[kPartScount] = sizeof(uint32_t) * kWavesPerBlock * blocks;
     L.bytes[kPartAccepted] = sizeof(unsigned long long) * kStatSlots;
     L.bytes[kPartProposals] = sizeof(unsigned long long) * kStatSlots;
     L.bytes[kPartTables] = sizeof(Tables);
@@ -58,12 +56,11 @@ SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, u
     b.nbr = static_cast<const uint32_t *>(slab_part(slab, L, kPartNbr));
     b.n_pad = s.n_pad;
     b.lattice = static_cast<uint32_t *>(slab_part(slab, L, kPartLattice));
+    b.lattice_words = uint32_t(s.lattice_words);
     b.claim0 = static_cast<uint32_t *>(slab_part(slab, L, kPartClaim0));
     b.claim1 = static_cast<uint32_t *>(slab_part(slab, L, kPartClaim1));
-    b.surv0 = static_cast<uint2 *>(slab_part(slab, L, kPartSurv0));
-    b.surv1 = static_cast<uint2 *>(slab_part(slab, L, kPartSurv1));
-    b.scount0 = static_cast<uint32_t *>(slab_part(slab, L, kPartScount0));
-    b.scount1 = static_cast<uint32_t *>(slab_part(slab, L, kPartScount1));
+    b.surv = static_cast<uint2 *>(slab_part(slab, L, kPartSurv));
+    b.scount = static_cast<uint32_t *>(slab_part(slab, L, kPartScount));
     b.accepted = static_cast<unsigned long long *>(slab_part(slab, L, kPartAccepted));
     b.proposals = static_cast<unsigned long long *>(slab_part(slab, L, kPartProposals));
     b.tables = static_cast<const Tables *>(slab_part(slab, L, kPartTables));

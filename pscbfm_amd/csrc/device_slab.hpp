@@ -19,10 +19,8 @@ enum SlabPart {
     kPartLattice,
     kPartClaim0,
     kPartClaim1,
-    kPartSurv0,
-    kPartSurv1,
-    kPartScount0,
-    kPartScount1,
+    kPartSurv,
+    kPartScount,
     kPartAccepted,
     kPartProposals,
     kPartTables,

@@ -205,7 +205,6 @@ void UpdaterGPUScBFM_AB_Type::initialize()
         throw;
     }
     mSubsteps = 0;
-    mPrevBlocks = 0;
     mAttempts = 0;
     mCheckMs = mPerformMs = 0.0;
     mTimedLaunches = 0;
@@ -233,9 +232,8 @@ void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs)
                 hip_check(hipEventRecord(ev[0], mStream), "hipEventRecord");
             }
             hip_check(launch_check(mBuf, s.seg_begin[c], s.seg_count[c], age, substep_seed(mSeed, age, k), parity,
-                                   mPrevBlocks, mStream),
+                                   mStream),
                       "check kernel");
-            mPrevBlocks = check_blocks(s.seg_count[c]);
             if (mTiming)
                 hip_check(hipEventRecord(ev[1], mStream), "hipEventRecord");
             hip_check(launch_perform(mBuf, s.seg_count[c], parity, mStream), "perform kernel");

@@ -102,7 +102,6 @@ private:
     void *mDevSlab = nullptr;   // one allocation holds every device array
     size_t mDevBytes = 0;
     uint64_t mSubsteps = 0;     // substep counter, its parity selects buffers
-    uint32_t mPrevBlocks = 0;   // grid of the previous check launch
     uint64_t mAttempts = 0;
 
     bool mTiming = false;
