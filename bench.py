@@ -235,9 +235,15 @@ def main() -> int:
     p_acc = accepted / attempts
     kbar = 2.0 * len(system["bonds"]) / n
     per_launch = n / n_colors                       # attempts in one check launch
-    b_check = 12.5 + 16.0 * kbar                    # own xyz + partner ids/xyz + 4 front bits
+    p_prop = proposals / attempts
+    clear = (w["box"] ** 3 // 8) / per_launch       # claim lattice cleared per launch, per attempt
+    # own position + id word, per partner its index and position, the 9 front
+    # bits; per survivor its 8-byte entry and claim bit; the claim clear
+    b_check = 8.0 + 8.0 * kbar + 9.0 / 8.0 + p_prop * (8.0 + 1.0 / 8.0) + clear
     achieved = per_launch * b_check / (check_ms * 1e-3) / 1e9
-    b_attempt = 12.5 + 16.0 * kbar + 13.0 * p_acc   # whole substep, per attempt
+    # whole substep: check, then per survivor its entry + 9 claim bits, per
+    # accepted move 2 lattice bits and a 4-byte position
+    b_attempt = b_check + p_prop * (8.0 + 9.0 / 8.0) + p_acc * (2.0 / 8.0 + 4.0)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
                 "kernel": "k_check", "kernel_avg_ms": round(check_ms, 4),
@@ -260,7 +266,8 @@ def main() -> int:
         "data": "synthetic (seeded random-walk melt)",
         "config": {"workload": args.workload, "box": w["box"], "chains": w["n_chains"],
                    "chain_len": w["chain_len"], "monomers": n, "colours": n_colors, "mcs": args.steps,
-                   "p_acc": round(p_acc, 4), "p_proposal": round(proposals / attempts, 4)},
+                   "p_acc": round(p_acc, 4), "p_proposal": round(p_prop, 4),
+                   "rng": "int_hash (the reference's default generator)"},
         "roofline": roofline,
     }
     u.cleanup()

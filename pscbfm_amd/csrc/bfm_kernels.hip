@@ -215,18 +215,33 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
     const uint2 *surv = buf.surv;
     const uint32_t *scount = buf.scount;
 
-    uint32_t c = 0;
+    // survivor counts of the group's check waves and their inclusive prefix
+    // sums, held by lanes 0..kPerformGroup-1
+    uint32_t incl = 0;
     if (lane < kPerformGroup && cw0 + lane < check_waves)
-        c = scount[cw0 + lane];
-    const uint32_t p1 = __shfl(c, 0);
-    const uint32_t p2 = p1 + __shfl(c, 1);
-    const uint32_t p3 = p2 + __shfl(c, 2);
-    const uint32_t total = p3 + __shfl(c, 3);
+        incl = scount[cw0 + lane];
+#pragma unroll
+    for (uint32_t d = 1; d < kPerformGroup; d <<= 1) {
+        const uint32_t up = __shfl_up(incl, d);
+        if (lane >= d)
+            incl += up;
+    }
+    uint32_t ends[kPerformGroup];
+#pragma unroll
+    for (uint32_t k = 0; k < kPerformGroup; ++k)
+        ends[k] = wave_uniform(__shfl(incl, int(k)));
+    const uint32_t total = ends[kPerformGroup - 1];
 
     uint32_t acc = 0;
     for (uint32_t t = lane; t < total; t += kWave) {
-        const uint32_t k = (t >= p1 ? 1u : 0u) + (t >= p2 ? 1u : 0u) + (t >= p3 ? 1u : 0u);
-        const uint32_t base = k == 0 ? 0u : (k == 1 ? p1 : (k == 2 ? p2 : p3));
+        // check wave k of the group holds survivors [ends[k-1], ends[k])
+        uint32_t k = 0, base = 0;
+#pragma unroll
+        for (uint32_t q = 0; q + 1 < kPerformGroup; ++q)
+            if (t >= ends[q]) {
+                k = q + 1;
+                base = ends[q];
+            }
         const uint2 e = surv[size_t(cw0 + k) * kWave + (t - base)];
         const uint32_t i = e.x & kSortedIndexMask;
         const int dir = int(e.x >> kSortedIndexBits);

@@ -7,7 +7,7 @@
 
 namespace pscbfm {
 
-constexpr uint32_t kPerformGroup = 4;         // check waves whose survivors one perform wave takes
+constexpr uint32_t kPerformGroup = 16;        // check waves whose survivors one perform wave takes
 constexpr uint32_t kStatSlots = 256;           // spread of the accepted / proposed counters
 
 // Everything one substep of one system needs on the device.
