@@ -237,17 +237,21 @@ def main() -> int:
     per_launch = n / n_colors                       # attempts in one check launch
     p_prop = proposals / attempts
     clear = (w["box"] ** 3 // 8) / per_launch       # claim lattice cleared per launch, per attempt
-    # own position + id word, per partner its index and position, the 9 front
-    # bits; per survivor its 8-byte entry and claim bit; the claim clear
-    b_check = 8.0 + 8.0 * kbar + 9.0 / 8.0 + p_prop * (8.0 + 1.0 / 8.0) + clear
+    # canonical algorithmic bytes per attempt of the check phase: own int32
+    # xyz (12), per bond a 4-byte partner id + 12-byte partner xyz, 4 front bits
+    b_check = 12.5 + 16.0 * kbar
     achieved = per_launch * b_check / (check_ms * 1e-3) / 1e9
-    # whole substep: check, then per survivor its entry + 9 claim bits, per
-    # accepted move 2 lattice bits and a 4-byte position
-    b_attempt = b_check + p_prop * (8.0 + 9.0 / 8.0) + p_acc * (2.0 / 8.0 + 4.0)
+    # whole path adds 13 B per accepted move (12-byte xyz write + 8 bit flips)
+    b_attempt = b_check + 13.0 * p_acc
+    # what this layout needs instead: packed position + id word (8), per bond
+    # index + packed position (8), 9 front bits, per survivor its 8-byte entry
+    # and claim bit, and the clear of the other claim lattice
+    b_layout = 8.0 + 8.0 * kbar + 9.0 / 8.0 + p_prop * (8.0 + 1.0 / 8.0) + clear
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
                 "kernel": "k_check", "kernel_avg_ms": round(check_ms, 4),
                 "perform_avg_ms": round(perform_ms, 4), "bytes_per_attempt_check": round(b_check, 3),
+                "layout_bytes_per_attempt_check": round(b_layout, 3),
                 "path_frac": round(value * b_attempt / 1e9 / HBM_PEAK_GBS, 4),
                 "copy_kernel_gbs": None if copy_gbs is None else round(copy_gbs, 1)}
 
