@@ -171,11 +171,13 @@ def served_leg(args, capacity: float) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200, help="timed Monte-Carlo steps")
+    ap.add_argument("--steps", type=int, default=1000, help="timed Monte-Carlo steps")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large_melt")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--time-every", type=int, default=8,
+                    help="time the kernels of every n-th substep with HIP events")
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-mcs", type=int, default=1000)
     ap.add_argument("--plain-mcs", type=int, default=200)
@@ -215,7 +217,9 @@ def main() -> int:
     colors, n_colors = u.getColors(n)
     log(f"initialized in {time.perf_counter() - t:.1f} s, {n_colors} colours")
 
-    u.setKernelTiming(True)
+    # HIP events around the kernels of every 8th substep of the timed region:
+    # live kernel durations without slowing the launch pipeline of the rest
+    u.setKernelTiming(args.time_every)
     u.runResident(args.warmup)
     s0 = u.getStats()  # synchronizes the updater's stream
     t0 = time.perf_counter()

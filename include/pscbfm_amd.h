@@ -157,9 +157,12 @@ typedef struct bfm_stats {
 } bfm_stats;
 int bfm_get_stats(const bfm_updater *h, bfm_stats *out);
 
-/* When on, every check/perform launch is bracketed by HIP events on the
- * handle's stream and the durations are summed into bfm_stats. */
-int bfm_set_kernel_timing(bfm_updater *h, int on);
+/* every > 0: the check and perform launches of every `every`-th substep are
+ * bracketed by HIP events on the handle's stream and their durations summed
+ * into bfm_stats (timed_launches counts the timed substeps); 0: off.  Each
+ * event costs a few microseconds of launch pipeline, so sampling (e.g. 8)
+ * keeps the sweep rate undisturbed. */
+int bfm_set_kernel_timing(bfm_updater *h, int every);
 
 /* Block-to-XCD placement of the sweep kernels (no counterpart in the
  * reference).  On (the default), each of the 8 XCDs sweeps one contiguous

@@ -303,10 +303,12 @@ int bfm_get_stats(const bfm_updater *h, bfm_stats *out)
     });
 }
 
-int bfm_set_kernel_timing(bfm_updater *h, int on)
+int bfm_set_kernel_timing(bfm_updater *h, int every)
 {
     NEED_HANDLE(h);
-    return guarded([&] { h->u.setKernelTiming(on != 0); });
+    if (every < 0)
+        return fail(BFM_E_INVALID_ARGUMENT, "kernel timing interval must be >= 0");
+    return guarded([&] { h->u.setKernelTiming(uint32_t(every)); });
 }
 
 int bfm_set_xcd_ranges(bfm_updater *h, int on)

@@ -226,7 +226,8 @@ void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs)
             const uint32_t c = order[k];
             const uint32_t parity = uint32_t(mSubsteps & 1u);
             hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-            if (mTiming) {
+            const bool timed = mTimingEvery != 0 && mSubsteps % mTimingEvery == 0;
+            if (timed) {
                 for (auto &x : ev)
                     x = takeEvent();
                 hip_check(hipEventRecord(ev[0], mStream), "hipEventRecord");
@@ -234,10 +235,10 @@ void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs)
             hip_check(launch_check(mBuf, s.seg_begin[c], s.seg_count[c], age, substep_seed(mSeed, age, k), parity,
                                    mStream),
                       "check kernel");
-            if (mTiming)
+            if (timed)
                 hip_check(hipEventRecord(ev[1], mStream), "hipEventRecord");
             hip_check(launch_perform(mBuf, s.seg_count[c], parity, mStream), "perform kernel");
-            if (mTiming) {
+            if (timed) {
                 hip_check(hipEventRecord(ev[2], mStream), "hipEventRecord");
                 mEvents.insert(mEvents.end(), ev, ev + 3);
             }

@@ -70,7 +70,7 @@ public:
     // extensions
     void runResident(uint32_t n_mcs);
     void copyBack();
-    void setKernelTiming(bool on) { mTiming = on; }
+    void setKernelTiming(uint32_t every) { mTimingEvery = every; }
     void setXcdRanges(bool on) { mBuf.xcd_ranges = on ? 1u : 0u; mXcdRanges = on; }
     const HostSystem &system() const { return mSys; }
     bool initialized() const { return mInitialized; }
@@ -104,7 +104,7 @@ private:
     uint64_t mSubsteps = 0;     // substep counter, its parity selects buffers
     uint64_t mAttempts = 0;
 
-    bool mTiming = false;
+    uint32_t mTimingEvery = 0;   // time every n-th substep (0: off)
     bool mXcdRanges = true;
     int64_t mStepsBetweenSortings = 5000;   // mnStepsBetweenSortings, UpdaterGPUScBFM_AB_Type.cu:1379
     std::vector<hipEvent_t> mEventPool;          // reusable events

@@ -130,8 +130,9 @@ class UpdaterGPUScBFM_AB_Type:
         _capi.check(self._lib.bfm_get_stats(self._h, byref(st)))
         return {name: getattr(st, name) for name, _ in _capi.BfmStats._fields_}
 
-    def setKernelTiming(self, on: bool) -> None:
-        _capi.check(self._lib.bfm_set_kernel_timing(self._h, int(bool(on))))
+    def setKernelTiming(self, every: int) -> None:
+        """Time the kernels of every `every`-th substep with HIP events (0: off)."""
+        _capi.check(self._lib.bfm_set_kernel_timing(self._h, int(every)))
 
     def setSpatialSorting(self, on: bool) -> None:
         """Morton order of positions within each colour (default) or id order;

@@ -190,6 +190,13 @@ def test_member_bytes_cover_the_layout():
     assert b < 200 * n + 8 * 64 ** 3
 
 
+def test_kernel_timing_interval_must_not_be_negative():
+    u = pscbfm_amd.UpdaterGPUScBFM_AB_Type()
+    with pytest.raises(InvalidArgument):
+        u.setKernelTiming(-1)
+    u.setKernelTiming(8)
+
+
 def test_front_sites_arithmetic():
     """The kernels' 9-site front (at most 4 lattice words and their bit masks,
     plus the old and moved corner sites) equals a direct enumeration with the
