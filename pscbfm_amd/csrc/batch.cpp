@@ -33,8 +33,10 @@ BatchEngine::~BatchEngine()
             hipGraphExecDestroy(mGraph[b]);
         if (mDone[b])
             hipEventDestroy(mDone[b]);
-        if (mHostSched[b])
-            hipHostFree(mHostSched[b]);
+        if (mHostChunk[b])
+            hipHostFree(mHostChunk[b]);
+        if (mDevChunk[b])
+            hipFree(mDevChunk[b]);
         if (mDevSched[b])
             hipFree(mDevSched[b]);
     }
@@ -79,6 +81,7 @@ uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age)
     }
     m->dev.seg_begin = static_cast<const uint32_t *>(slab_part(m->slab, L, kPartSegBegin));
     m->dev.seg_count = static_cast<const uint32_t *>(slab_part(m->slab, L, kPartSegCount));
+    m->dev.n_colours = m->sys.n_colours;
     m->check_blocks_max = check_blocks(std::max<uint32_t>(m->sys.max_segment, 1));
     m->perform_blocks_max = perform_blocks(m->check_blocks_max);
     mBytes += L.total;
@@ -158,25 +161,49 @@ void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
 void BatchEngine::runChunk(uint32_t mcs)
 {
     const uint32_t n_sys = uint32_t(mCompIds.size());
-    const uint32_t slots = mcs * mMaxColours;
-    const size_t entries = size_t(slots) * n_sys;
     const int b = mBuf;
     mBuf ^= 1;
 
     // the host buffer b was last read by the copy of chunk k-2
     if (mDone[b])
-        hip_check(hipEventSynchronize(mDone[b]), "hipEventSynchronize (schedule)");
+        hip_check(hipEventSynchronize(mDone[b]), "hipEventSynchronize (chunk table)");
     else
         hip_check(hipEventCreateWithFlags(&mDone[b], hipEventDisableTiming), "hipEventCreate");
+    if (n_sys > mChunkCap[b]) {
+        if (mHostChunk[b])
+            hip_check(hipHostFree(mHostChunk[b]), "hipHostFree");
+        if (mDevChunk[b])
+            hip_check(hipFree(mDevChunk[b]), "hipFree");
+        mHostChunk[b] = nullptr;
+        mDevChunk[b] = nullptr;
+        hip_check(hipHostMalloc(&mHostChunk[b], sizeof(ChunkEntry) * n_sys, hipHostMallocDefault),
+                  "hipHostMalloc (chunk table)");
+        hip_check(hipMalloc(&mDevChunk[b], sizeof(ChunkEntry) * n_sys), "hipMalloc (chunk table)");
+        mChunkCap[b] = n_sys;
+        if (mGraph[b]) {
+            hip_check(hipGraphExecDestroy(mGraph[b]), "hipGraphExecDestroy");
+            mGraph[b] = nullptr;
+        }
+    }
+
+    // where every member starts this chunk; the kernels derive colour, parity
+    // and substep seed of each slot from it
+    ChunkEntry *chunk = mHostChunk[b];
+    for (uint32_t j = 0; j < n_sys; ++j) {
+        Member &M = *mMembers[mCompIds[j]];
+        chunk[j].age = uint64_t(M.age);
+        chunk[j].substeps = M.substeps;
+        M.age += mcs;
+        M.substeps += uint64_t(mcs) * M.sys.n_colours;
+    }
+    hip_check(hipMemcpyAsync(mDevChunk[b], chunk, sizeof(ChunkEntry) * n_sys, hipMemcpyHostToDevice, mStream),
+              "upload chunk table");
+    const uint32_t slots = mcs * mMaxColours;
+    const size_t entries = size_t(slots) * n_sys;
     if (entries > mSchedCap[b]) {
-        if (mHostSched[b])
-            hip_check(hipHostFree(mHostSched[b]), "hipHostFree");
         if (mDevSched[b])
             hip_check(hipFree(mDevSched[b]), "hipFree");
-        mHostSched[b] = nullptr;
         mDevSched[b] = nullptr;
-        hip_check(hipHostMalloc(&mHostSched[b], sizeof(SchedEntry) * entries, hipHostMallocDefault),
-                  "hipHostMalloc (schedule)");
         hip_check(hipMalloc(&mDevSched[b], sizeof(SchedEntry) * entries), "hipMalloc (schedule)");
         mSchedCap[b] = entries;
         if (mGraph[b]) {
@@ -185,37 +212,9 @@ void BatchEngine::runChunk(uint32_t mcs)
         }
     }
 
-    // schedule: slot (m, k) of system j sweeps colour order[k] of MCS age+m
-    SchedEntry *sched = mHostSched[b];
-    std::vector<uint32_t> order(std::max<uint32_t>(mMaxColours, 1));
-    for (uint32_t j = 0; j < n_sys; ++j) {
-        Member &M = *mMembers[mCompIds[j]];
-        const uint32_t nc = M.sys.n_colours;
-        for (uint32_t m = 0; m < mcs; ++m) {
-            const uint64_t age = uint64_t(M.age) + m;
-            colour_order(M.seed, age, nc, order.data());
-            for (uint32_t k = 0; k < mMaxColours; ++k) {
-                SchedEntry &e = sched[size_t(m * mMaxColours + k) * n_sys + j];
-                e.sub_seed = k < nc ? substep_seed(M.seed, age, k) : 0u;
-                e.pad = 0;
-                e.age = age;
-                if (k < nc) {
-                    const uint32_t c = order[k];
-                    e.colour = c;
-                    e.parity = uint32_t(M.substeps & 1u);
-                    ++M.substeps;
-                } else {
-                    e.colour = kNoColour;
-                    e.parity = 0;
-                }
-            }
-        }
-        M.age += mcs;
-    }
-    hip_check(hipMemcpyAsync(mDevSched[b], sched, sizeof(SchedEntry) * entries, hipMemcpyHostToDevice, mStream),
-              "upload schedule");
-
     auto launch_all = [&]() {
+        hip_check(launch_schedule(mDevSys, mDevChunk[b], n_sys, mcs, mMaxColours, mDevSched[b], mStream),
+                  "schedule kernel");
         for (uint32_t slot = 0; slot < slots; ++slot)
             for (uint32_t c = 0; c < kClasses; ++c) {
                 if (!mCheckBlocks[c])
@@ -267,7 +266,7 @@ void BatchEngine::runChunk(uint32_t mcs)
     uint32_t classes = 0;
     for (uint32_t c = 0; c < kClasses; ++c)
         classes += mCheckBlocks[c] ? 1u : 0u;
-    mKernelLaunches += 2ull * slots * classes;
+    mKernelLaunches += 2ull * slots * classes + 1;
     hip_check(hipEventRecord(mDone[b], mStream), "hipEventRecord");
 }
 

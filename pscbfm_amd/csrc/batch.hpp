@@ -3,11 +3,12 @@
 // Every member keeps its own device slab (positions, partners, lattices,
 // survivor and collision lists: the same layout as one plain updater).  A
 // substep slot of a chunk is one k_check_batch and one k_perform_batch launch
-// over all members at once; which colour each member sweeps in each slot, its
-// parity, its previous check grid and its age come from a schedule the host
-// writes before the chunk.  The launches of a chunk are captured once into a
-// hipGraph and replayed for every chunk with the same composition, so the
-// whole chunk costs one graph launch on the host.
+// over all members at once.  The host writes only where each member stands at
+// the start of the chunk (age, substeps); a small kernel at the head of the
+// chunk derives every member's colour, parity and substep seed for every slot
+// (one thread per member and MCS), which the sweep kernels read.  The launches of a
+// chunk are captured once into a hipGraph and replayed for every chunk with
+// the same composition, so the whole chunk costs one graph launch on the host.
 //
 // A member's trajectory depends only on its own seed and age (the RNG is keyed
 // on seed, MCS and monomer id), so it is the same alone, in any batch and in
@@ -92,11 +93,13 @@ private:
     uint32_t mCheckBlocks[kClasses] = {}, mPerformBlocks[kClasses] = {};
     size_t mCheckCap[kClasses] = {}, mPerformCap[kClasses] = {};
 
-    // schedules of the last two chunks (double-buffered so that the host can
-    // write chunk k+1's schedule while chunk k runs)
+    // chunk tables of the last two chunks (double-buffered so that the host
+    // can write chunk k+1's table while chunk k runs)
     int mBuf = 0;
-    SchedEntry *mHostSched[2] = {nullptr, nullptr};   // pinned
-    SchedEntry *mDevSched[2] = {nullptr, nullptr};
+    ChunkEntry *mHostChunk[2] = {nullptr, nullptr};   // pinned
+    ChunkEntry *mDevChunk[2] = {nullptr, nullptr};
+    size_t mChunkCap[2] = {0, 0};
+    SchedEntry *mDevSched[2] = {nullptr, nullptr};    // written by the schedule kernel
     size_t mSchedCap[2] = {0, 0};
     hipEvent_t mDone[2] = {nullptr, nullptr};
 

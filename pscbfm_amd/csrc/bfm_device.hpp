@@ -110,6 +110,19 @@ constexpr uint32_t kTagSubstep = 3;
 constexpr uint32_t kRngIntHash = 1;   // the reference's default (.cu:1407)
 constexpr uint32_t kRngPhilox = 3;
 
+// Entry k of colour_order(seed, age, n) without the array: the value that ends
+// at position k is found by following k back through the transpositions of
+// the shuffle (the last one applied, i = 1, first).
+__host__ __device__ __forceinline__ uint32_t colour_at(uint64_t seed, uint64_t age, uint32_t n, uint32_t k)
+{
+    uint32_t p = k;
+    for (uint32_t i = 1; i < n; ++i) {
+        const uint32_t j = philox_word0(seed, i, age, kTagColour) % (i + 1u);
+        p = p == i ? j : (p == j ? i : p);
+    }
+    return p;
+}
+
 // 32-bit integer mix of the reference's Rngs::Hash (rngs/Hash.h:39-44):
 // Robert Jenkins' six-step integer hash with its published constants.
 __host__ __device__ __forceinline__ uint32_t int_hash(uint32_t a)

@@ -38,6 +38,28 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
 
 __device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Pointers into global memory.  The kernel bodies access their buffers through
+// these: a batched kernel reads its member's pointers from memory, and without
+// the address space the compiler would use flat instructions for them (slower,
+// and waited for on both the vector and the scalar counter).
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(T *p)
+{
+    return (gptr<T>)p;
+}
+
+// Generic view of a global element, for class types (HIP vector types) whose
+// member operators expect a generic `this`; the compiler still sees that the
+// address is global.
+template <class T>
+__device__ __forceinline__ T &element(gptr<T> p)
+{
+    return *(T *)p;
+}
+
 __device__ __forceinline__ int4 moved(const int4 &r, int dir)
 {
     const int32_t s = (dir & 1) ? 1 : -1;
@@ -58,12 +80,12 @@ __shared__ uint32_t s_dep[3 * 64];
 __device__ __forceinline__ void load_bond(const Tables *t)
 {
     if (threadIdx.x < 16)
-        s_bond[threadIdx.x] = t->bond[threadIdx.x];
+        s_bond[threadIdx.x] = as_global(t)->bond[threadIdx.x];
 }
 
 __device__ __forceinline__ void load_dep(const Tables *t)
 {
-    const uint32_t *src = t->dep[0];
+    const gptr<const uint32_t> src = as_global(t)->dep[0];
     for (int k = threadIdx.x; k < 3 * 64; k += blockDim.x)
         s_dep[k] = src[k];
 }
@@ -94,16 +116,20 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
     const Geometry &g = buf.geom;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t gwave = wave_uniform(blk * kWavesPerBlock + (threadIdx.x >> 6));
-    uint32_t *claim_cur = pick(parity, buf.claim0, buf.claim1);
-    uint32_t *claim_prev = pick(parity, buf.claim1, buf.claim0);
+    const gptr<const uint32_t> ppos = as_global<const uint32_t>(buf.ppos);
+    const gptr<const uint32_t> ids = as_global(buf.ids);
+    const gptr<const uint32_t> nbr = as_global(buf.nbr);
+    const gptr<const uint32_t> lat = as_global<const uint32_t>(buf.lattice);
+    const gptr<uint32_t> claim_cur = as_global(pick(parity, buf.claim0, buf.claim1));
+    const gptr<uint32_t> claim_prev = as_global(pick(parity, buf.claim1, buf.claim0));
 
     const uint32_t t = blk * kBlock + threadIdx.x;
     bool pass = false;
     uint2 entry = make_uint2(0u, 0u);
     if (t < count) {
         const uint32_t i = begin + t;
-        const uint32_t p0 = buf.ppos[i];
-        const uint32_t idn = buf.ids[i];
+        const uint32_t p0 = ppos[i];
+        const uint32_t idn = ids[i];
         const uint32_t id = idn & kIdMask, nn = idn >> kIdBits;
         // partners of the wave's busiest lane; lanes with fewer read their
         // own slot instead (in bounds, and the result is unused)
@@ -115,7 +141,7 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
 #pragma unroll
         for (int j = 0; j < kMaxConnectivity; ++j)
             if (uint32_t(j) < nn_wave)
-                partner[j] = buf.nbr[size_t(j) * buf.n_pad + i];
+                partner[j] = nbr[size_t(j) * buf.n_pad + i];
 
         const int4 r0 = unpack_xyz(p0);
         // direction = rn % 6 (.cu:964-967); the generator is uniform per launch
@@ -138,7 +164,6 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
 
         // excluded volume: no corner in the 9-site front
         const Front f = front_sites(g, dep, r0, dir);
-        const uint32_t *lat = buf.lattice;
         uint32_t l[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -150,7 +175,7 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
 #pragma unroll
         for (int j = 0; j < kMaxConnectivity; ++j)
             if (uint32_t(j) < nn_wave)
-                qpos[j] = buf.ppos[uint32_t(j) < nn ? partner[j] : i];
+                qpos[j] = ppos[uint32_t(j) < nn ? partner[j] : i];
 #pragma unroll
         for (int j = 0; j < kMaxConnectivity; ++j)
             if (uint32_t(j) < nn_wave && uint32_t(j) < nn) {
@@ -164,7 +189,7 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
         if (ok) {
             // the reference's tentative lattice (dpLatticeTmp, :1012-1023),
             // one bit per moved corner
-            atomicOr(&claim_cur[f.new_site >> 5], 1u << (f.new_site & 31u));
+            atomicOr((uint32_t *)&claim_cur[f.new_site >> 5], 1u << (f.new_site & 31u));
             pass = true;
             entry = make_uint2(i | (uint32_t(dir) << kSortedIndexBits), p0);
         }
@@ -173,18 +198,18 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
     // wave compaction of the survivors
     const uint64_t mball = __ballot(pass);
     if (pass)
-        buf.surv[size_t(gwave) * kWave + lanes_below(mball)] = entry;
+        element(as_global(buf.surv) + size_t(gwave) * kWave + lanes_below(mball)) = entry;
     if (lane == 0)
-        buf.scount[gwave] = uint32_t(__popcll(mball));
+        as_global(buf.scount)[gwave] = uint32_t(__popcll(mball));
 
     // the previous substep's claims (the reference zeroes its tentative
     // lattice in a third kernel, :1240-1262): nobody reads claim[parity^1] in
     // this kernel, so the grid clears all of it with coalesced 16-byte
     // stores; at the densities of a melt a survivor's mark shares every
     // 64-byte line with others, so clearing marks one by one writes more
-    uint4 *cp = reinterpret_cast<uint4 *>(claim_prev);
+    const gptr<uint4> cp = reinterpret_cast<gptr<uint4>>(claim_prev);
     for (uint32_t k = t; k < buf.lattice_words / 4u; k += nblk * kBlock)
-        cp[k] = make_uint4(0u, 0u, 0u, 0u);
+        element(cp + k) = make_uint4(0u, 0u, 0u, 0u);
 }
 
 template <bool kAllPeriodic, bool kCubic>
@@ -211,9 +236,11 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
     const uint32_t pwave = wave_uniform(blk * kWavesPerBlock + (threadIdx.x >> 6));
     const uint32_t check_waves = check_blocks_ * kWavesPerBlock;
     const uint32_t cw0 = pwave * kPerformGroup;
-    const uint32_t *claim = pick(parity, buf.claim0, buf.claim1);
-    const uint2 *surv = buf.surv;
-    const uint32_t *scount = buf.scount;
+    const gptr<const uint32_t> claim = as_global<const uint32_t>(pick(parity, buf.claim0, buf.claim1));
+    const gptr<const uint2> surv = as_global<const uint2>(buf.surv);
+    const gptr<const uint32_t> scount = as_global<const uint32_t>(buf.scount);
+    const gptr<uint32_t> lattice = as_global(buf.lattice);
+    const gptr<uint32_t> ppos = as_global(buf.ppos);
 
     // survivor counts of the group's check waves and their inclusive prefix
     // sums, held by lanes 0..kPerformGroup-1
@@ -242,7 +269,7 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
                 k = q + 1;
                 base = ends[q];
             }
-        const uint2 e = surv[size_t(cw0 + k) * kWave + (t - base)];
+        const uint2 e = element(surv + size_t(cw0 + k) * kWave + (t - base));
         const uint32_t i = e.x & kSortedIndexMask;
         const int dir = int(e.x >> kSortedIndexBits);
         const int4 r0 = unpack_xyz(e.y);
@@ -261,10 +288,10 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
             const uint32_t w0 = f.old_site >> 5, w1 = f.new_site >> 5;
             const uint32_t b0 = 1u << (f.old_site & 31u), b1 = 1u << (f.new_site & 31u);
             if (w0 == w1) {
-                atomicXor(&buf.lattice[w0], b0 | b1);
+                atomicXor((uint32_t *)&lattice[w0], b0 | b1);
             } else {
-                atomicXor(&buf.lattice[w0], b0);
-                atomicXor(&buf.lattice[w1], b1);
+                atomicXor((uint32_t *)&lattice[w0], b0);
+                atomicXor((uint32_t *)&lattice[w1], b1);
             }
             // new wrapped coordinate on the move axis; crossing the box
             // boundary changes the periodic image (only on periodic axes:
@@ -273,15 +300,16 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
             const uint32_t am = sel3(g.mask, a);
             const uint32_t v = uint32_t(comp(r0, a) + ((dir & 1) ? 1 : -1)) & am;
             const uint32_t shift = kCoordBits * uint32_t(a);
-            buf.ppos[i] = (e.y & ~(kCoordMask << shift)) | (v << shift);
+            ppos[i] = (e.y & ~(kCoordMask << shift)) | (v << shift);
             if ((dir & 1) ? v == 0u : v == am)
-                buf.image[size_t(a) * buf.n_pad + i] += (dir & 1) ? 1 : -1;
+                as_global(buf.image)[size_t(a) * buf.n_pad + i] += (dir & 1) ? 1 : -1;
         }
         acc += uint32_t(__popcll(__ballot(!clash)));
     }
     if (lane == 0 && total) {
-        atomicAdd(&buf.accepted[pwave & (kStatSlots - 1)], (unsigned long long)acc);
-        atomicAdd(&buf.proposals[pwave & (kStatSlots - 1)], (unsigned long long)total);
+        atomicAdd((unsigned long long *)&as_global(buf.accepted)[pwave & (kStatSlots - 1)], (unsigned long long)acc);
+        atomicAdd((unsigned long long *)&as_global(buf.proposals)[pwave & (kStatSlots - 1)],
+                  (unsigned long long)total);
     }
 }
 
@@ -297,9 +325,40 @@ __global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t c
 }
 
 // ---- many systems per launch ------------------------------------------------
-// Block b of a batched launch belongs to system blk_sys[b]; the system's own
-// blocks start at its check_base / perform_base.  The schedule says which
-// colour (if any) each system sweeps in this substep slot.
+// One thread per (system, MCS of the chunk) writes that MCS's slots.
+__global__ __launch_bounds__(kBlock) void k_schedule(const BatchSystem *__restrict__ sys,
+                                                     const ChunkEntry *__restrict__ chunk, uint32_t n_sys,
+                                                     uint32_t mcs, uint32_t max_colours, SchedEntry *__restrict__ sched)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n_sys * mcs)
+        return;
+    const uint32_t j = t % n_sys, m = t / n_sys;
+    const BatchSystem &S = sys[j];
+    const ChunkEntry ce = chunk[j];
+    const uint32_t nc = S.n_colours;
+    const uint64_t age = ce.age + m;
+    for (uint32_t k = 0; k < max_colours; ++k) {
+        SchedEntry e;
+        e.age = age;
+        e.pad = 0;
+        if (k < nc) {
+            const uint32_t c = colour_at(S.buf.seed, age, nc, k);
+            e.colour = c;
+            e.parity = uint32_t(ce.substeps + uint64_t(m) * nc + k) & 1u;
+            e.sub_seed = substep_seed(S.buf.seed, age, k);
+            e.begin = as_global(S.seg_begin)[c];
+            e.count = as_global(S.seg_count)[c];
+        } else {
+            e.colour = kNoColour;
+            e.parity = 0;
+            e.sub_seed = 0;
+            e.begin = 0;
+            e.count = 0;
+        }
+        sched[size_t(m * max_colours + k) * n_sys + j] = e;
+    }
+}
 
 template <bool kAllPeriodic, bool kCubic>
 __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__restrict__ sys,
@@ -313,8 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__res
     if (e.colour == kNoColour)
         return;
     const BatchSystem &S = sys[s];
-    const uint32_t count = S.seg_count[e.colour];
-    const uint32_t nb = check_blocks(count);
+    const uint32_t nb = check_blocks(e.count);
     const uint32_t blk = lb - S.check_base;
     if (blk >= nb)
         return;
@@ -322,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__res
     if (!kCubic)
         load_dep(S.buf.tables);
     __syncthreads();
-    check_body<kAllPeriodic, kCubic>(S.buf, S.seg_begin[e.colour], count, e.age, e.sub_seed, e.parity, blk, nb);
+    check_body<kAllPeriodic, kCubic>(S.buf, e.begin, e.count, e.age, e.sub_seed, e.parity, blk, nb);
 }
 
 template <bool kCubic>
@@ -337,7 +395,7 @@ __global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__r
     if (e.colour == kNoColour)
         return;
     const BatchSystem &S = sys[s];
-    const uint32_t cb = check_blocks(S.seg_count[e.colour]);
+    const uint32_t cb = check_blocks(e.count);
     const uint32_t blk = lb - S.perform_base;
     if (blk >= perform_blocks(cb))
         return;
@@ -384,6 +442,15 @@ hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t pari
         hipLaunchKernelGGL((k_perform<true>), dim3(perform_blocks(cb)), dim3(kBlock), 0, stream, buf, cb, parity);
     else
         hipLaunchKernelGGL((k_perform<false>), dim3(perform_blocks(cb)), dim3(kBlock), 0, stream, buf, cb, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_schedule(const BatchSystem *sys, const ChunkEntry *chunk, uint32_t n_sys, uint32_t mcs,
+                           uint32_t max_colours, SchedEntry *sched, hipStream_t stream)
+{
+    const uint32_t n = n_sys * mcs;
+    hipLaunchKernelGGL(k_schedule, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, sys, chunk, n_sys, mcs,
+                       max_colours, sched);
     return hipGetLastError();
 }
 

@@ -78,16 +78,29 @@ struct BatchSystem {
     SweepBuffers buf;
     const uint32_t *seg_begin;   // per colour, sorted index of the first monomer
     const uint32_t *seg_count;   // per colour
+    uint32_t n_colours;
     uint32_t check_base;         // first block of this system in a batched check launch
     uint32_t perform_base;       // first block of this system in a batched perform launch
+    uint32_t pad;
 };
 
-// What one system does in one substep slot of a chunk.
+// Where a system stands at the start of a chunk; substep k of MCS m of the
+// chunk sweeps colour colour_at(seed, age + m, n_colours, k) with parity
+// (substeps + m * n_colours + k) & 1.
+struct ChunkEntry {
+    uint64_t age;
+    uint64_t substeps;
+};
+
+// What one system does in substep slot m * max_colours + k of a chunk,
+// written on the device by launch_schedule.
 struct SchedEntry {
-    uint64_t age;          // MCS that keys the RNG
-    uint32_t colour;       // kNoColour: no substep for this system in this slot
-    uint32_t parity;       // substep parity of this system
+    uint64_t age;          // MCS that keys the generator
+    uint32_t colour;       // kNoColour: the system idles in this slot
+    uint32_t parity;       // substep parity
     uint32_t sub_seed;     // substep_seed() of this substep
+    uint32_t begin;        // the colour's segment: first sorted index ...
+    uint32_t count;        // ... and size
     uint32_t pad;
 };
 
@@ -98,6 +111,11 @@ inline uint32_t kernel_class(const Geometry &g)
     const bool p = g.periodic == 7u, c = g.cubic != 0u;
     return p && c ? kClassPeriodicCubic : (p ? kClassPeriodic : (c ? kClassCubic : kClassOther));
 }
+
+// The schedule of a chunk of `mcs` MCS: sched[(m * max_colours + k) * n_sys
+// + j] for every system j, from chunk[j] and the systems' colour segments.
+hipError_t launch_schedule(const BatchSystem *sys, const ChunkEntry *chunk, uint32_t n_sys, uint32_t mcs,
+                           uint32_t max_colours, SchedEntry *sched, hipStream_t stream);
 
 // Phase 1 / phase 2 of one substep slot for every system of one class of a
 // batch.  n_blocks = sum over the class's systems of their reserved blocks;

@@ -1,9 +1,12 @@
-// Host check of the 9-site front arithmetic (front_sites in bfm_device.hpp)
-// against a direct enumeration with the lattice's site function: for every
-// box, corner and direction tested, the at most 4 (word, mask) pairs must hold
-// exactly the bits of the 9 front sites, and new_site / old_site must be the
-// sites of the moved and the old corner.  Exhaustive for small boxes, random
-// corners for large ones.  Prints "ok <cases>" and exits 0 on success.
+// Host checks of device helpers in bfm_device.hpp:
+//  * the 9-site front arithmetic (front_sites) against a direct enumeration
+//    with the lattice's site function: for every box, corner and direction
+//    tested, the at most 4 (word, mask) pairs must hold exactly the bits of
+//    the 9 front sites, and new_site / old_site must be the sites of the moved
+//    and the old corner (exhaustive for small boxes, random corners for large);
+//  * colour_at (one entry of the per-MCS colour permutation, computed without
+//    the array) against the full shuffle colour_order.
+// Prints "ok <cases>" and exits 0 on success.
 #include <cstdio>
 #include <cstdlib>
 #include <set>
@@ -103,6 +106,20 @@ int main()
             }
         }
     }
+    for (uint32_t n = 1; n <= 40; ++n)
+        for (uint64_t age = 0; age < 200; ++age) {
+            const uint64_t seed = 0x5EED000000000000ull + n * 7919u + age;
+            uint32_t order[40];
+            colour_order(seed, age, n, order);
+            for (uint32_t k = 0; k < n; ++k) {
+                if (colour_at(seed, age, n, k) != order[k]) {
+                    std::printf("FAIL colour_at n %u age %llu k %u: %u vs %u\n", n, (unsigned long long)age, k,
+                                colour_at(seed, age, n, k), order[k]);
+                    return 1;
+                }
+                ++cases;
+            }
+        }
     std::printf("ok %ld\n", cases);
     return 0;
 }

@@ -197,13 +197,14 @@ def test_kernel_timing_interval_must_not_be_negative():
     u.setKernelTiming(8)
 
 
-def test_front_sites_arithmetic():
+def test_device_helpers_on_host():
     """The kernels' 9-site front (at most 4 lattice words and their bit masks,
     plus the old and moved corner sites) equals a direct enumeration with the
     lattice's site function, exhaustively for boxes up to 16384 sites and on
-    random corners of larger and non-cubic boxes (tests/cpp/front_sites_check.cpp)."""
+    random corners of larger and non-cubic boxes; the batched kernels' colour
+    lookup equals the full colour permutation (tests/cpp/device_helpers_check.cpp)."""
     import subprocess
-    exe = Path(__file__).resolve().parent / "cpp" / "front_sites_check"
+    exe = Path(__file__).resolve().parent / "cpp" / "device_helpers_check"
     assert exe.exists(), "built by __graft_entry__.build()"
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
