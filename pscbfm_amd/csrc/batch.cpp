@@ -21,6 +21,16 @@ BatchEngine::BatchEngine(int gpu) : mGpu(gpu)
         throw std::invalid_argument("batch: GPU id out of range");
     hip_check(hipSetDevice(gpu), "hipSetDevice");
     hip_check(hipStreamCreate(&mStream), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&mTablesCopied, hipEventDisableTiming), "hipEventCreate");
+    // members come and go: allocate their slabs in stream order from the
+    // device's pool and keep freed memory in the pool for the next ones
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, gpu) == hipSuccess && pool) {
+        uint64_t keep = ~0ull;
+        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        mPooled = true;
+    }
+    hipGetLastError();
 }
 
 BatchEngine::~BatchEngine()
@@ -51,8 +61,29 @@ BatchEngine::~BatchEngine()
         if (mDevPerformBlk[c])
             hipFree(mDevPerformBlk[c]);
     }
+    if (mTablesCopied)
+        hipEventDestroy(mTablesCopied);
     if (mStream)
         hipStreamDestroy(mStream);
+}
+
+void *BatchEngine::allocate(size_t bytes, const char *what)
+{
+    void *p = nullptr;
+    hipError_t e = mPooled ? hipMallocAsync(&p, bytes, mStream) : hipMalloc(&p, bytes);
+    if (e != hipSuccess)
+        hip_check(e, what);
+    return p;
+}
+
+void BatchEngine::release(void *p)
+{
+    if (!p)
+        return;
+    if (mPooled)
+        hip_check(hipFreeAsync(p, mStream), "hipFreeAsync");
+    else
+        hip_check(hipFree(p), "hipFree");
 }
 
 size_t BatchEngine::bytesFor(const HostSystem &sys) { return slab_layout(sys).total; }
@@ -65,18 +96,13 @@ uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age)
     m->seed = seed;
     m->age = age;
     const SlabLayout L = slab_layout(m->sys);
-    hipError_t e = hipMalloc(&m->slab, L.total);
-    if (e != hipSuccess) {
-        m->slab = nullptr;
-        hip_check(e, "hipMalloc (batch member)");
-    }
+    m->slab = allocate(L.total, "allocation of a batch member");
     m->bytes = L.total;
     try {
+        // the uploads read m->sys, which the member keeps until it is removed
         m->dev.buf = upload_slab(m->sys, m->slab, L, seed, mStream);
-        // the host copies must outlive the asynchronous uploads
-        hip_check(hipStreamSynchronize(mStream), "hipStreamSynchronize (batch add)");
     } catch (...) {
-        hipFree(m->slab);
+        release(m->slab);
         throw;
     }
     m->dev.seg_begin = static_cast<const uint32_t *>(slab_part(m->slab, L, kPartSegBegin));
@@ -94,8 +120,8 @@ void BatchEngine::remove(uint32_t id)
     Member &m = *mMembers.at(id);
     if (!m.slab)
         return;
-    synchronize();
-    hipFree(m.slab);
+    // stream order: launches already queued finish before the memory is freed
+    release(m.slab);
     m.slab = nullptr;
     mBytes -= m.bytes;
     mCompIds.clear();   // forces a rebuild of the tables
@@ -108,22 +134,29 @@ void BatchEngine::synchronize()
 }
 
 template <class T>
-static void ensure_device(T *&p, size_t &cap, size_t n, const char *what)
+void BatchEngine::ensureDevice(T *&p, size_t &cap, size_t n, const char *what)
 {
     if (n <= cap && p)
         return;
-    if (p)
-        hip_check(hipFree(p), "hipFree");
+    release(p);
     p = nullptr;
     cap = std::max<size_t>(n, 1);
-    hip_check(hipMalloc(&p, sizeof(T) * cap), what);
+    p = static_cast<T *>(allocate(sizeof(T) * cap, what));
 }
 
 void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
 {
-    synchronize();   // tables may still be read by in-flight launches
-    std::vector<BatchSystem> sys;
-    std::vector<uint32_t> cblk[kClasses], pblk[kClasses];
+    // Launches already queued still read the device tables, but the copies
+    // below are queued behind them on the same stream; only the host staging
+    // vectors of the previous rebuild must have been copied before reuse.
+    hip_check(hipEventSynchronize(mTablesCopied), "hipEventSynchronize (tables)");
+    std::vector<BatchSystem> &sys = mHostSys;
+    std::vector<uint32_t> *cblk = mHostCheckBlk, *pblk = mHostPerformBlk;
+    sys.clear();
+    for (uint32_t c = 0; c < kClasses; ++c) {
+        cblk[c].clear();
+        pblk[c].clear();
+    }
     mMaxColours = 0;
     for (uint32_t j = 0; j < ids.size(); ++j) {
         Member &m = *mMembers.at(ids[j]);
@@ -138,22 +171,24 @@ void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
         sys.push_back(d);
         mMaxColours = std::max(mMaxColours, m.sys.n_colours);
     }
-    ensure_device(mDevSys, mSysCap, sys.size(), "hipMalloc (batch systems)");
-    hip_check(hipMemcpy(mDevSys, sys.data(), sizeof(BatchSystem) * sys.size(), hipMemcpyHostToDevice),
+    ensureDevice(mDevSys, mSysCap, sys.size(), "allocation of the batch systems");
+    hip_check(hipMemcpyAsync(mDevSys, sys.data(), sizeof(BatchSystem) * sys.size(), hipMemcpyHostToDevice, mStream),
               "upload batch systems");
     for (uint32_t c = 0; c < kClasses; ++c) {
         mCheckBlocks[c] = uint32_t(cblk[c].size());
         mPerformBlocks[c] = uint32_t(pblk[c].size());
         if (cblk[c].empty())
             continue;
-        ensure_device(mDevCheckBlk[c], mCheckCap[c], cblk[c].size(), "hipMalloc (batch blocks)");
-        ensure_device(mDevPerformBlk[c], mPerformCap[c], pblk[c].size(), "hipMalloc (batch blocks)");
-        hip_check(hipMemcpy(mDevCheckBlk[c], cblk[c].data(), sizeof(uint32_t) * cblk[c].size(), hipMemcpyHostToDevice),
+        ensureDevice(mDevCheckBlk[c], mCheckCap[c], cblk[c].size(), "allocation of the batch blocks");
+        ensureDevice(mDevPerformBlk[c], mPerformCap[c], pblk[c].size(), "allocation of the batch blocks");
+        hip_check(hipMemcpyAsync(mDevCheckBlk[c], cblk[c].data(), sizeof(uint32_t) * cblk[c].size(),
+                                 hipMemcpyHostToDevice, mStream),
                   "upload batch blocks");
-        hip_check(hipMemcpy(mDevPerformBlk[c], pblk[c].data(), sizeof(uint32_t) * pblk[c].size(),
-                            hipMemcpyHostToDevice),
+        hip_check(hipMemcpyAsync(mDevPerformBlk[c], pblk[c].data(), sizeof(uint32_t) * pblk[c].size(),
+                                 hipMemcpyHostToDevice, mStream),
                   "upload batch blocks");
     }
+    hip_check(hipEventRecord(mTablesCopied, mStream), "hipEventRecord");
     mCompIds = ids;
     ++mCompVersion;
 }
@@ -172,13 +207,12 @@ void BatchEngine::runChunk(uint32_t mcs)
     if (n_sys > mChunkCap[b]) {
         if (mHostChunk[b])
             hip_check(hipHostFree(mHostChunk[b]), "hipHostFree");
-        if (mDevChunk[b])
-            hip_check(hipFree(mDevChunk[b]), "hipFree");
+        release(mDevChunk[b]);
         mHostChunk[b] = nullptr;
         mDevChunk[b] = nullptr;
         hip_check(hipHostMalloc(&mHostChunk[b], sizeof(ChunkEntry) * n_sys, hipHostMallocDefault),
                   "hipHostMalloc (chunk table)");
-        hip_check(hipMalloc(&mDevChunk[b], sizeof(ChunkEntry) * n_sys), "hipMalloc (chunk table)");
+        mDevChunk[b] = static_cast<ChunkEntry *>(allocate(sizeof(ChunkEntry) * n_sys, "allocation of a chunk table"));
         mChunkCap[b] = n_sys;
         if (mGraph[b]) {
             hip_check(hipGraphExecDestroy(mGraph[b]), "hipGraphExecDestroy");
@@ -201,10 +235,8 @@ void BatchEngine::runChunk(uint32_t mcs)
     const uint32_t slots = mcs * mMaxColours;
     const size_t entries = size_t(slots) * n_sys;
     if (entries > mSchedCap[b]) {
-        if (mDevSched[b])
-            hip_check(hipFree(mDevSched[b]), "hipFree");
-        mDevSched[b] = nullptr;
-        hip_check(hipMalloc(&mDevSched[b], sizeof(SchedEntry) * entries), "hipMalloc (schedule)");
+        release(mDevSched[b]);
+        mDevSched[b] = static_cast<SchedEntry *>(allocate(sizeof(SchedEntry) * entries, "allocation of a schedule"));
         mSchedCap[b] = entries;
         if (mGraph[b]) {
             hip_check(hipGraphExecDestroy(mGraph[b]), "hipGraphExecDestroy");

@@ -77,6 +77,11 @@ private:
 
     void rebuildTables(const std::vector<uint32_t> &ids);
     void runChunk(uint32_t mcs);
+    // device memory in stream order (the pool of the device) when available
+    void *allocate(size_t bytes, const char *what);
+    void release(void *p);
+    template <class T>
+    void ensureDevice(T *&p, size_t &cap, size_t n, const char *what);
 
     int mGpu;
     hipStream_t mStream = nullptr;
@@ -85,6 +90,11 @@ private:
 
     // tables of the current composition
     std::vector<uint32_t> mCompIds;          // member ids in launch order
+    bool mPooled = false;                    // stream-ordered allocation available
+    // host copies of the tables, kept until their upload has run
+    std::vector<BatchSystem> mHostSys;
+    std::vector<uint32_t> mHostCheckBlk[kClasses], mHostPerformBlk[kClasses];
+    hipEvent_t mTablesCopied = nullptr;
     BatchSystem *mDevSys = nullptr;
     size_t mSysCap = 0;
     uint32_t mMaxColours = 0;
