@@ -95,9 +95,11 @@ int bfm_set_rng(bfm_updater *h, int rng);
  * seed is the Philox key (which also draws the integer hash's substep seeds). */
 int bfm_set_seed(bfm_updater *h, uint64_t seed);
 
-/* setStepsBetweenSortings, UpdaterGPUScBFM_AB_Type.h:510-513.  Accepted for
- * API compatibility; re-sorting never changes a trajectory here because the
- * RNG is keyed on the original monomer id. */
+/* setStepsBetweenSortings, UpdaterGPUScBFM_AB_Type.h:510-513 (default 5000):
+ * every `steps`-th MCS the monomers of each colour are re-sorted on the device
+ * by the lattice site of their position (doSpatialSorting, .cu:1836-1935).
+ * Re-sorting never changes a trajectory here because the generator is keyed
+ * on the original monomer id. */
 int bfm_set_steps_between_sortings(bfm_updater *h, int64_t steps);
 
 /* Layout of each colour segment (no counterpart in the reference's API; its
@@ -154,6 +156,7 @@ typedef struct bfm_stats {
     double   check_ms;       /* summed duration of check kernels (if timed)   */
     double   perform_ms;     /* summed duration of perform kernels (if timed) */
     uint64_t timed_launches; /* check launches covered by check_ms             */
+    uint64_t sorts;          /* on-device spatial re-sorts so far               */
 } bfm_stats;
 int bfm_get_stats(const bfm_updater *h, bfm_stats *out);
 

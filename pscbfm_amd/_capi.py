@@ -64,6 +64,7 @@ class BfmStats(ctypes.Structure):
         ("check_ms", c_double),
         ("perform_ms", c_double),
         ("timed_launches", c_uint64),
+        ("sorts", c_uint64),
     ]
 
 

@@ -19,6 +19,7 @@ ARCH = os.environ.get("PSCBFM_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = [
     "bfm_kernels.hip",
+    "resort.hip",
     "host_system.cpp",
     "device_slab.cpp",
     "updater.cpp",

@@ -300,6 +300,7 @@ int bfm_get_stats(const bfm_updater *h, bfm_stats *out)
         out->check_ms = h->u.checkMs();
         out->perform_ms = h->u.performMs();
         out->timed_launches = h->u.timedLaunches();
+        out->sorts = h->u.sorts();
     });
 }
 

@@ -1,0 +1,156 @@
+// resort.hip -- on-device spatial re-sort within colour segments (gfx950).
+#include "resort.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+#include "updater.hpp"
+
+namespace pscbfm {
+namespace {
+
+constexpr int kResortBlock = 256;
+
+// Site of the lower corner of every slot (padding slots included; they lie
+// outside the sorted ranges) and the identity permutation.
+__global__ __launch_bounds__(kResortBlock) void k_keys(const uint32_t *__restrict__ ppos,
+                                                       const Tables *__restrict__ tables, uint32_t n_pad,
+                                                       uint32_t *__restrict__ keys, uint32_t *__restrict__ ident,
+                                                       uint32_t *__restrict__ perm)
+{
+    const uint32_t t = blockIdx.x * kResortBlock + threadIdx.x;
+    if (t >= n_pad)
+        return;
+    const int4 r = unpack_xyz(ppos[t]);
+    uint32_t site = 0;
+    const int32_t c[3] = {r.x, r.y, r.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t v = uint32_t(c[a]);
+        site |= tables->dep[a][v & 31u] | tables->dep[a][32 + (v >> 5)];
+    }
+    keys[t] = site;
+    ident[t] = t;
+    perm[t] = t;
+}
+
+// inv[perm[t]] = t: where every old slot went.
+__global__ __launch_bounds__(kResortBlock) void k_inverse(const uint32_t *__restrict__ perm, uint32_t n_pad,
+                                                          uint32_t *__restrict__ inv)
+{
+    const uint32_t t = blockIdx.x * kResortBlock + threadIdx.x;
+    if (t < n_pad)
+        inv[perm[t]] = t;
+}
+
+// New slot t takes old slot perm[t]; partner indices move with their owners.
+__global__ __launch_bounds__(kResortBlock) void k_gather(const uint32_t *__restrict__ ppos,
+                                                         const uint32_t *__restrict__ ids,
+                                                         const int32_t *__restrict__ image,
+                                                         const uint32_t *__restrict__ nbr,
+                                                         const uint32_t *__restrict__ perm,
+                                                         const uint32_t *__restrict__ inv, uint32_t n_pad,
+                                                         uint32_t *__restrict__ ppos2, uint32_t *__restrict__ ids2,
+                                                         int32_t *__restrict__ image2, uint32_t *__restrict__ nbr2)
+{
+    const uint32_t t = blockIdx.x * kResortBlock + threadIdx.x;
+    if (t >= n_pad)
+        return;
+    const uint32_t o = perm[t];
+    const uint32_t idn = ids[o];
+    ppos2[t] = ppos[o];
+    ids2[t] = idn;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        image2[size_t(a) * n_pad + t] = image[size_t(a) * n_pad + o];
+    const uint32_t nn = idn == 0xFFFFFFFFu ? 0u : idn >> kIdBits;
+#pragma unroll
+    for (int j = 0; j < kMaxConnectivity; ++j)
+        nbr2[size_t(j) * n_pad + t] = uint32_t(j) < nn ? inv[nbr[size_t(j) * n_pad + o]] : 0u;
+}
+
+uint32_t site_bits(const HostSystem &s)
+{
+    uint32_t b = 0;
+    while ((uint64_t(1) << b) < uint64_t(s.lattice_words) * 32u)
+        ++b;
+    return b;
+}
+
+}  // namespace
+
+Resorter::~Resorter() { release(); }
+
+void Resorter::release()
+{
+    if (mWork)
+        hipFree(mWork);
+    if (mSortTemp)
+        hipFree(mSortTemp);
+    mWork = mSortTemp = nullptr;
+    mWorkBytes = mSortBytes = 0;
+}
+
+void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const SweepBuffers &buf, hipStream_t stream)
+{
+    const uint32_t n_pad = s.n_pad;
+    // the gathered arrays mirror the slab's [ppos, ids, image, nbr] range, so
+    // one device-to-device copy puts them back
+    const size_t first = L.offset[kPartPPos];
+    const size_t range = L.offset[kPartNbr] + L.bytes[kPartNbr] - first;
+    const size_t words = (size_t(n_pad) * sizeof(uint32_t) + 255) / 256 * 256;
+    const size_t need = range + 5 * words;
+    if (mWorkBytes < need) {
+        if (mWork)
+            hip_check(hipFree(mWork), "hipFree");
+        mWork = nullptr;
+        mWorkBytes = 0;
+        hip_check(hipMalloc(&mWork, need), "hipMalloc (re-sort workspace)");
+        mWorkBytes = need;
+    }
+    char *w = static_cast<char *>(mWork);
+    auto mirror = [&](SlabPart p) { return w + (L.offset[p] - first); };
+    uint32_t *keys = reinterpret_cast<uint32_t *>(w + range);
+    uint32_t *keys_sorted = reinterpret_cast<uint32_t *>(w + range + words);
+    uint32_t *ident = reinterpret_cast<uint32_t *>(w + range + 2 * words);
+    uint32_t *perm = reinterpret_cast<uint32_t *>(w + range + 3 * words);
+    uint32_t *inv = reinterpret_cast<uint32_t *>(w + range + 4 * words);
+
+    const uint32_t bits = site_bits(s);
+    size_t temp = 0;
+    hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, keys, keys_sorted, ident, perm,
+                                                 int(std::max<uint32_t>(s.max_segment, 1)), 0, int(bits), stream),
+              "radix sort (size query)");
+    if (mSortBytes < temp) {
+        if (mSortTemp)
+            hip_check(hipFree(mSortTemp), "hipFree");
+        mSortTemp = nullptr;
+        mSortBytes = 0;
+        hip_check(hipMalloc(&mSortTemp, temp), "hipMalloc (re-sort temporary)");
+        mSortBytes = temp;
+    }
+
+    const dim3 grid((n_pad + kResortBlock - 1) / kResortBlock), block(kResortBlock);
+    hipLaunchKernelGGL(k_keys, grid, block, 0, stream, buf.ppos, buf.tables, n_pad, keys, ident, perm);
+    hip_check(hipGetLastError(), "re-sort keys");
+    for (uint32_t c = 0; c < s.n_colours; ++c) {
+        const uint32_t b = s.seg_begin[c], n = s.seg_count[c];
+        if (n < 2)
+            continue;
+        size_t tb = mSortBytes;
+        hip_check(hipcub::DeviceRadixSort::SortPairs(mSortTemp, tb, keys + b, keys_sorted + b, ident + b, perm + b,
+                                                     int(n), 0, int(bits), stream),
+                  "radix sort");
+    }
+    hipLaunchKernelGGL(k_inverse, grid, block, 0, stream, perm, n_pad, inv);
+    hip_check(hipGetLastError(), "re-sort inverse");
+    hipLaunchKernelGGL(k_gather, grid, block, 0, stream, buf.ppos, buf.ids, buf.image, buf.nbr, perm, inv, n_pad,
+                       reinterpret_cast<uint32_t *>(mirror(kPartPPos)), reinterpret_cast<uint32_t *>(mirror(kPartIds)),
+                       reinterpret_cast<int32_t *>(mirror(kPartImage)), reinterpret_cast<uint32_t *>(mirror(kPartNbr)));
+    hip_check(hipGetLastError(), "re-sort gather");
+    hip_check(hipMemcpyAsync(static_cast<char *>(slab) + first, w, range, hipMemcpyDeviceToDevice, stream),
+              "re-sort copy");
+}
+
+}  // namespace pscbfm

@@ -189,6 +189,7 @@ void UpdaterGPUScBFM_AB_Type::initialize()
 
     const HostSystem &s = mSys;
     const SlabLayout L = slab_layout(s);
+    mLayout = L;
     hipError_t e = hipMalloc(&mDevSlab, L.total);
     if (e != hipSuccess) {
         mDevSlab = nullptr;
@@ -208,6 +209,9 @@ void UpdaterGPUScBFM_AB_Type::initialize()
     mAttempts = 0;
     mCheckMs = mPerformMs = 0.0;
     mTimedLaunches = 0;
+    mLastSortAge = mAge;   // plan() laid the colours out in spatial order
+    mLayoutMoved = false;
+    mSorts = 0;
     mInitialized = true;
 }
 
@@ -220,6 +224,15 @@ void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs)
     const HostSystem &s = mSys;
     std::vector<uint32_t> order(s.n_colours);
     for (uint32_t m = 0; m < n_mcs; ++m) {
+        // periodic re-sort at the start of every mnStepsBetweenSortings-th MCS
+        // (runSimulationOnGPU, UpdaterGPUScBFM_AB_Type.cu:3005-3012)
+        if (s.spatial_order && mStepsBetweenSortings > 0 && mAge % mStepsBetweenSortings == 0 &&
+            mAge != mLastSortAge) {
+            mResorter.run(s, mDevSlab, mLayout, mBuf, mStream);
+            mLastSortAge = mAge;
+            mLayoutMoved = true;
+            ++mSorts;
+        }
         const uint64_t age = uint64_t(mAge);
         colour_order(mSeed, age, s.n_colours, order.data());
         for (uint32_t k = 0; k < s.n_colours; ++k) {
@@ -293,8 +306,18 @@ void UpdaterGPUScBFM_AB_Type::copyBack()
     hip_check(hipMemcpyAsync(s.image_sorted.data(), mBuf.image, sizeof(int32_t) * 3 * size_t(s.n_pad),
                              hipMemcpyDeviceToHost, mStream),
               "download images");
+    if (mLayoutMoved)   // a re-sort moved the monomers: read where each one is now
+        hip_check(hipMemcpyAsync(s.ids_sorted.data(), mBuf.ids, sizeof(uint32_t) * s.n_pad, hipMemcpyDeviceToHost,
+                                 mStream),
+                  "download ids");
     hip_check(hipStreamSynchronize(mStream), "hipStreamSynchronize (copy-back)");
     collectTimings();
+    if (mLayoutMoved) {
+        for (uint32_t t = 0; t < s.n_pad; ++t)
+            if (s.ids_sorted[t] != 0xFFFFFFFFu)
+                s.to_sorted[s.ids_sorted[t] & kIdMask] = t;
+        mLayoutMoved = false;
+    }
     unpack_positions(s);
 }
 
@@ -346,6 +369,7 @@ void UpdaterGPUScBFM_AB_Type::freeDevice()
         hipStreamDestroy(mStream);
         mStream = nullptr;
     }
+    mResorter.release();
     if (mDevSlab) {
         hipFree(mDevSlab);
         mDevSlab = nullptr;

@@ -14,7 +14,9 @@
 #include <vector>
 
 #include "bfm_kernels.hpp"
+#include "device_slab.hpp"
 #include "host_system.hpp"
+#include "resort.hpp"
 
 namespace pscbfm {
 
@@ -83,6 +85,7 @@ public:
     double checkMs() const { return mCheckMs; }
     double performMs() const { return mPerformMs; }
     uint64_t timedLaunches() const { return mTimedLaunches; }
+    uint64_t sorts() const { return mSorts; }
 
 private:
     int32_t at(T_Id i, int a) const;
@@ -103,6 +106,12 @@ private:
     size_t mDevBytes = 0;
     uint64_t mSubsteps = 0;     // substep counter, its parity selects buffers
     uint64_t mAttempts = 0;
+
+    SlabLayout mLayout{};
+    Resorter mResorter;          // on-device spatial re-sort
+    int64_t mLastSortAge = 0;    // age of the last sort (initialize sorts)
+    bool mLayoutMoved = false;   // the device order differs from the host's to_sorted
+    uint64_t mSorts = 0;
 
     uint32_t mTimingEvery = 0;   // time every n-th substep (0: off)
     bool mXcdRanges = true;

@@ -153,7 +153,8 @@ class UpdaterGPUScBFM_AB_Type:
 
 def updater_from_system(system: dict, seed: int, gpu: int | None = 0, age: int = 0,
                         validate_only: bool = False, spatial_sorting: bool = True,
-                        rng: int = Rng_IntHash) -> UpdaterGPUScBFM_AB_Type:
+                        rng: int = Rng_IntHash,
+                        steps_between_sortings: int | None = None) -> UpdaterGPUScBFM_AB_Type:
     """Builds an updater the way the LeMonADE adapter does
     (GPUScBFM_AB_Type.h:120-184): box, periodicity, age, positions, both
     directions of every bond, the 512-entry bond table, then initialize."""
@@ -182,6 +183,8 @@ def updater_from_system(system: dict, seed: int, gpu: int | None = 0, age: int =
                 u.copyBondSet(dx, dy, dz, bool(forbidden[idx]))
     u.setSpatialSorting(spatial_sorting)
     u.setRng(rng)
+    if steps_between_sortings is not None:
+        u.setStepsBetweenSortings(steps_between_sortings)
     if validate_only:
         u.validate()
     else:

@@ -14,11 +14,13 @@ from pscbfm_amd import systems, updater_from_system
 pytestmark = pytest.mark.gpu
 
 
-def run_pair(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True, rng=orc.RNG_INT_HASH):
+def run_pair(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True, rng=orc.RNG_INT_HASH, sort_every=None):
     n = len(system["xyz"])
     print(f"case: n={n} box={system['box']} periodic={system.get('periodic')} seed={seed:#x} "
-          f"chunks={chunks} age0={age0} xcd_ranges={xcd_ranges} spatial={spatial} rng={rng}", flush=True)
-    u = updater_from_system(system, seed=seed, gpu=0, age=age0, spatial_sorting=spatial, rng=rng)
+          f"chunks={chunks} age0={age0} xcd_ranges={xcd_ranges} spatial={spatial} rng={rng} "
+          f"sort_every={sort_every}", flush=True)
+    u = updater_from_system(system, seed=seed, gpu=0, age=age0, spatial_sorting=spatial, rng=rng,
+                            steps_between_sortings=sort_every)
     u.setXcdRanges(xcd_ranges)
     for c in chunks:
         u.runSimulationOnGPU(c)
@@ -31,8 +33,9 @@ def run_pair(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True, rng=or
     return got, want, stats, acc, prop, osys
 
 
-def assert_parity(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True, rng=orc.RNG_INT_HASH):
-    got, want, stats, acc, prop, osys = run_pair(system, seed, chunks, age0, xcd_ranges, spatial, rng)
+def assert_parity(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True, rng=orc.RNG_INT_HASH,
+                  sort_every=None):
+    got, want, stats, acc, prop, osys = run_pair(system, seed, chunks, age0, xcd_ranges, spatial, rng, sort_every)
     diff = np.nonzero((got != want).any(axis=1))[0]
     assert diff.size == 0, f"{diff.size} monomers differ, first {diff[:5]}: gpu {got[diff[:3]]} oracle {want[diff[:3]]}"
     assert stats["accepted"] == acc
@@ -67,6 +70,24 @@ def test_id_order_layout(gpu_ready):
     assert_parity(s, seed=78, chunks=[300], spatial=False)
 
 
+def test_periodic_resort_keeps_the_trajectory(gpu_ready):
+    # on-device re-sort every 7 MCS (ages 7, 14, ..., 56) between and inside chunks
+    s = systems.linear_melt(64, 512, 32, seed=24)
+    st = assert_parity(s, seed=79, chunks=[25, 35], sort_every=7)
+    assert st["sorts"] == 8
+
+
+def test_periodic_resort_network_non_cubic(gpu_ready):
+    # three colours, branch points (4 partners) and a non-cubic box: every
+    # segment is re-sorted and every partner index remapped
+    s = systems.diamond_network(64, (2, 2, 8), 32, seed=43)
+    st = assert_parity(s, seed=44, chunks=[60], sort_every=5, age0=3)
+    assert st["sorts"] == 12
+    big = non_cubic_box_system()
+    st = assert_parity(big, seed=45, chunks=[40], sort_every=10)
+    assert st["sorts"] == 3
+
+
 def test_resume_is_continuous(gpu_ready):
     # counter RNG keyed on the age: split runs equal one long run
     s = systems.linear_melt(32, 64, 16, seed=5)
@@ -92,8 +113,8 @@ def test_low_density_and_long_chains(gpu_ready):
     assert_parity(s, seed=22, chunks=[400])
 
 
-def test_non_cubic_box(gpu_ready):
-    # compact Morton order for a 64 x 16 x 32 box
+def non_cubic_box_system():
+    # four 16^3 melts side by side in a 64 x 16 x 32 box
     s = systems.linear_melt(16, 16, 16, seed=31)
     xyz = s["xyz"].copy()
     big = {"box": (64, 16, 32), "periodic": (1, 1, 1), "forbidden": s["forbidden"]}
@@ -103,7 +124,12 @@ def test_non_cubic_box(gpu_ready):
         bonds.append(s["bonds"] + k * len(xyz))
     big["xyz"] = np.concatenate(parts)
     big["bonds"] = np.concatenate(bonds)
-    assert_parity(big, seed=32, chunks=[250])
+    return big
+
+
+def test_non_cubic_box(gpu_ready):
+    # compact Morton order for a 64 x 16 x 32 box
+    assert_parity(non_cubic_box_system(), seed=32, chunks=[250])
 
 
 def test_large_melt_full_size_10_mcs(gpu_ready):
