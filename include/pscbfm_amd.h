@@ -110,6 +110,17 @@ int bfm_set_steps_between_sortings(bfm_updater *h, int64_t steps);
  * initialize / bfm_validate only. */
 int bfm_set_spatial_sorting(bfm_updater *h, int on);
 
+/* LeMonADE BFM files (UpdaterReadBfmFile / AnalyzerWriteBfmFile around the
+ * reference's simulator, mainCUDASimulatorGPUScBFM_AB_Type.cpp:166-171; the
+ * format as mainColorBFM.cpp:135-276 parses it).  read: configures an updater
+ * that is not initialized yet from the file's header, bond-vector set,
+ * attributes, bonds and last conformation (its !mcs value becomes the age).
+ * write: append = 0 writes a new file (header, bond set, attributes, the bonds
+ * the chain encoding does not carry, the current conformation); append != 0
+ * appends the current conformation as "!mcs=<age>". */
+int bfm_read_bfm_file(bfm_updater *h, const char *path);
+int bfm_write_bfm_file(bfm_updater *h, const char *path, int append);
+
 /* The host half of initialize (no device work): the checks of checkSystem
  * (UpdaterGPUScBFM_AB_Type.cu:2789-2900) and the bond-set count (:1526), then
  * the colouring of initializeSpeciesSorting (:1561-1686).  Lets a caller

@@ -101,6 +101,8 @@ SIGNATURES = {
     "bfm_get_stats": (c_int, [_H, POINTER(BfmStats)]),
     "bfm_set_kernel_timing": (c_int, [_H, c_int]),
     "bfm_set_spatial_sorting": (c_int, [_H, c_int]),
+    "bfm_read_bfm_file": (c_int, [_H, c_char_p]),
+    "bfm_write_bfm_file": (c_int, [_H, c_char_p, c_int]),
     "bfm_set_xcd_ranges": (c_int, [_H, c_int]),
     "bfm_run_resident": (c_int, [_H, c_uint32]),
     "bfm_copy_back": (c_int, [_H]),

@@ -27,6 +27,7 @@ SOURCES = [
     "server.cpp",
     "capi.cpp",
     "generators.cpp",
+    "bfm_io.cpp",
 ]
 
 # Packed FP32 instructions stay off in every HIP object (there is no float

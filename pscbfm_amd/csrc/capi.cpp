@@ -1,11 +1,13 @@
 // capi.cpp -- extern "C" boundary: status codes and a thread-local message,
 // never an exception or exit() across the ABI.
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <new>
 #include <string>
 
 #include "../../include/pscbfm_amd.h"
+#include "bfm_io.hpp"
 #include "server.hpp"
 #include "updater.hpp"
 
@@ -230,6 +232,74 @@ int bfm_cleanup(bfm_updater *h)
 {
     NEED_HANDLE(h);
     return guarded([&] { h->u.cleanup(); });
+}
+
+int bfm_read_bfm_file(bfm_updater *h, const char *path)
+{
+    NEED_HANDLE(h);
+    if (!path)
+        return fail(BFM_E_INVALID_ARGUMENT, "null path");
+    return guarded([&] {
+        const pscbfm::BfmState f = pscbfm::read_bfm_file(path);
+        auto &u = h->u;
+        u.setLatticeSize(f.box[0], f.box[1], f.box[2]);
+        u.setPeriodicity(f.periodic[0], f.periodic[1], f.periodic[2]);
+        u.setNrOfAllMonomers(f.n);
+        for (uint32_t i = 0; i < f.n; ++i) {
+            u.setMonomerCoordinates(i, f.xyz[3 * size_t(i)], f.xyz[3 * size_t(i) + 1], f.xyz[3 * size_t(i) + 2]);
+            u.setAttribute(i, f.attributes[i]);
+        }
+        // both directions, as the LeMonADE adapter forwards every link of every
+        // monomer (GPUScBFM_AB_Type.h:159-161)
+        for (auto b : f.bonds) {
+            u.setConnectivity(b.first, b.second);
+            u.setConnectivity(b.second, b.first);
+        }
+        bool allowed[512] = {};
+        for (auto &v : f.bondset_ids)
+            allowed[v.first] = true;
+        for (int dx = -4; dx < 4; ++dx)
+            for (int dy = -4; dy < 4; ++dy)
+                for (int dz = -4; dz < 4; ++dz)
+                    u.copyBondSet(dx, dy, dz, !allowed[pscbfm::bond_table_index(dx, dy, dz)]);
+        u.setAge(f.age);
+    });
+}
+
+int bfm_write_bfm_file(bfm_updater *h, const char *path, int append)
+{
+    NEED_HANDLE(h);
+    if (!path)
+        return fail(BFM_E_INVALID_ARGUMENT, "null path");
+    return guarded([&] {
+        const pscbfm::HostSystem &sys = h->u.system();
+        if (sys.n == 0 || sys.xyz.size() != 3 * size_t(sys.n))
+            throw std::runtime_error("write_bfm_file: the updater holds no monomers");
+        pscbfm::BfmState f;
+        for (int a = 0; a < 3; ++a) {
+            f.box[a] = sys.box[a];
+            f.periodic[a] = sys.periodic[a];
+        }
+        f.n = sys.n;
+        f.xyz = sys.xyz;
+        for (uint32_t i = 0; i < sys.n; ++i)
+            for (uint32_t j = 0; j < sys.nnbr[i]; ++j) {
+                const uint32_t k = sys.nbr[size_t(i) * pscbfm::kMaxConnectivity + j];
+                if (i < k || std::find(sys.nbr.begin() + size_t(k) * pscbfm::kMaxConnectivity,
+                                       sys.nbr.begin() + size_t(k) * pscbfm::kMaxConnectivity + sys.nnbr[k],
+                                       i) == sys.nbr.begin() + size_t(k) * pscbfm::kMaxConnectivity + sys.nnbr[k])
+                    f.bonds.push_back({std::min(i, k), std::max(i, k)});
+            }
+        std::sort(f.bonds.begin(), f.bonds.end());
+        f.bonds.erase(std::unique(f.bonds.begin(), f.bonds.end()), f.bonds.end());
+        f.bondset_ids = pscbfm::bondset_ids_for(sys.forbidden);
+        f.attributes = sys.attribute;
+        f.age = h->u.getAge();
+        if (append)
+            pscbfm::append_bfm_conformation(path, f);
+        else
+            pscbfm::write_bfm_file(path, f);
+    });
 }
 
 int bfm_set_monomer_coordinates_bulk(bfm_updater *h, const int32_t *xyz, uint32_t n)

@@ -134,6 +134,17 @@ class UpdaterGPUScBFM_AB_Type:
         """Time the kernels of every `every`-th substep with HIP events (0: off)."""
         _capi.check(self._lib.bfm_set_kernel_timing(self._h, int(every)))
 
+    def readBfmFile(self, path: str) -> None:
+        """Configures the (not yet initialized) updater from a LeMonADE BFM
+        file: header, bond-vector set, attributes, bonds and the last
+        conformation, whose !mcs value becomes the age."""
+        _capi.check(self._lib.bfm_read_bfm_file(self._h, str(path).encode()))
+
+    def writeBfmFile(self, path: str, append: bool = False) -> None:
+        """Writes a new BFM file with the current state, or appends the
+        current conformation as !mcs=<age>."""
+        _capi.check(self._lib.bfm_write_bfm_file(self._h, str(path).encode(), int(bool(append))))
+
     def setSpatialSorting(self, on: bool) -> None:
         """Morton order of positions within each colour (default) or id order;
         the trajectory is the same either way.  Before initialize."""
