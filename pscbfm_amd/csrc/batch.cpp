@@ -349,10 +349,10 @@ void BatchEngine::counters(uint32_t id, uint64_t &accepted, uint64_t &proposals)
     if (!m.slab)
         return;
     synchronize();
-    unsigned long long slots[2 * kStatSlots];
-    hip_check(hipMemcpy(slots, m.dev.buf.accepted, sizeof(unsigned long long) * kStatSlots, hipMemcpyDeviceToHost),
+    std::vector<unsigned long long> slots(2 * size_t(kStatSlots));
+    hip_check(hipMemcpy(slots.data(), m.dev.buf.accepted, sizeof(unsigned long long) * kStatSlots, hipMemcpyDeviceToHost),
               "download counters");
-    hip_check(hipMemcpy(slots + kStatSlots, m.dev.buf.proposals, sizeof(unsigned long long) * kStatSlots,
+    hip_check(hipMemcpy(slots.data() + kStatSlots, m.dev.buf.proposals, sizeof(unsigned long long) * kStatSlots,
                         hipMemcpyDeviceToHost),
               "download counters");
     for (uint32_t k = 0; k < kStatSlots; ++k) {

@@ -332,12 +332,12 @@ void UpdaterGPUScBFM_AB_Type::deviceCounters(uint64_t &accepted, uint64_t &propo
     accepted = proposals = 0;
     if (!mInitialized)
         return;
-    unsigned long long slots[2 * kStatSlots];
+    std::vector<unsigned long long> slots(2 * size_t(kStatSlots));
     hip_check(hipSetDevice(mGpu), "hipSetDevice");
-    hip_check(hipMemcpyAsync(slots, mBuf.accepted, sizeof(unsigned long long) * kStatSlots, hipMemcpyDeviceToHost,
+    hip_check(hipMemcpyAsync(slots.data(), mBuf.accepted, sizeof(unsigned long long) * kStatSlots, hipMemcpyDeviceToHost,
                              mStream),
               "download counters");
-    hip_check(hipMemcpyAsync(slots + kStatSlots, mBuf.proposals, sizeof(unsigned long long) * kStatSlots,
+    hip_check(hipMemcpyAsync(slots.data() + kStatSlots, mBuf.proposals, sizeof(unsigned long long) * kStatSlots,
                              hipMemcpyDeviceToHost, mStream),
               "download counters");
     hip_check(hipStreamSynchronize(mStream), "hipStreamSynchronize (counters)");
