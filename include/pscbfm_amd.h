@@ -191,6 +191,66 @@ int bfm_run_resident(bfm_updater *h, uint32_t n_mcs);
 /* Copies the device state back to the host (the doCopyBack half). */
 int bfm_copy_back(bfm_updater *h);
 
+/* ---- analyzers on the device -------------------------------------------------
+ * The reference analyses its systems on the host after copying every position
+ * back (AnalyzerMsd, src/AnalyzerMsd.tpp:99-421, behind LeMonADE's
+ * TaskManager, mainCUDASimulatorGPUScBFM_AB_Type.cpp:166-177).  These reduce
+ * the resident state on the device and return only the results, so they work
+ * between bfm_run_resident calls without a copy-back.  Integer results are
+ * exact; floating-point means come from fixed-order device sums. */
+typedef struct bfm_observables {
+    uint64_t bonds;           /* undirected bonds                                         */
+    uint64_t bond_hist[512];  /* bonds per bond-table index (dx&7)|(dy&7)<<3|(dz&7)<<6,
+                                 each oriented from the lower to the higher id           */
+    uint32_t n_chains;        /* linear chains; 0 when the bonds are not linear chains    */
+    uint32_t reserved;
+    int64_t  re2_sum;         /* sum over chains of |r_last - r_first|^2 (exact)          */
+    double   re2;             /* mean squared end-to-end distance (NaN without chains)    */
+    double   rg2;             /* mean squared radius of gyration (NaN without chains)     */
+    double   box_com[3];      /* mean of the chains' centres of mass (AnalyzerMsd.tpp:346-349) */
+} bfm_observables;
+int bfm_observe(bfm_updater *h, bfm_observables *out);
+
+/* Linear chains as AnalyzerMsd::getLinearPolymerSystemInfo finds them
+ * (AnalyzerMsd.tpp:186-288): components in order of their lowest id, ends in
+ * the order of its stack walk, the middle monomer floor(n/2) links from the
+ * first end.  Host only (no device needed).  Writes up to `capacity` entries
+ * of first/middle/last ids (any may be NULL) and the chain count; fails with
+ * the reference analyzer's message if the bonds are not linear chains. */
+int bfm_chain_table(bfm_updater *h, uint32_t *n_chains, uint32_t *first, uint32_t *middle, uint32_t *last,
+                    uint32_t capacity);
+
+/* Per linear chain, relative to its first monomer (exact integers). */
+typedef struct bfm_chain_sums {
+    int64_t  s[3];            /* sum over members of r - r_first   */
+    int64_t  q;               /* sum over members of |r - r_first|^2 */
+    int32_t  first[3], middle[3], last[3];
+    uint32_t n;               /* members                             */
+} bfm_chain_sums;
+int bfm_get_chain_sums(bfm_updater *h, bfm_chain_sums *out, uint32_t capacity);
+
+/* AnalyzerMsd on the device: attach = its constructor (rnMcsToSkip,
+ * rDeltaTRealization, rDeltaTEvaluate, rnMaxSubsequences; :141-161) plus
+ * initialize() (:173-177), which evaluates at the current age; execute =
+ * AnalyzerMsd::execute at the current age (:303-421), to be called after each
+ * run as the TaskManager does; points = the rows cleanup() writes (:456-467):
+ * dt, samples, and for g1..g5 (first, middle, last monomer, chain COM, middle
+ * monomer - chain COM) the mean and the reference's "sg" column.  The
+ * reference leaves the time of the last realization uninitialised (:107);
+ * here the first evaluation starts the first subsequence. */
+typedef struct bfm_msd_point {
+    uint64_t dt;
+    double   n_values;
+    double   mean[5];
+    double   sg[5];
+} bfm_msd_point;
+int bfm_msd_attach(bfm_updater *h, uint64_t mcs_to_skip, uint64_t delta_t_realization, uint64_t delta_t_evaluate,
+                   uint64_t max_subsequences);
+int bfm_msd_execute(bfm_updater *h);
+int bfm_msd_get_points(bfm_updater *h, bfm_msd_point *out, uint32_t capacity, uint32_t *n_points);
+/* AnalyzerMsd::cleanup's file (:423-468). */
+int bfm_msd_write(bfm_updater *h, const char *path);
+
 /* ---- batched form: many independent systems per launch ----------------------
  * No counterpart in the reference, whose box and bond tables are module-global
  * __constant__ symbols (UpdaterGPUScBFM_AB_Type.cu:84-133), so one process

@@ -68,6 +68,39 @@ class BfmStats(ctypes.Structure):
     ]
 
 
+class BfmObservables(ctypes.Structure):
+    _fields_ = [
+        ("bonds", c_uint64),
+        ("bond_hist", c_uint64 * 512),
+        ("n_chains", c_uint32),
+        ("reserved", c_uint32),
+        ("re2_sum", c_int64),
+        ("re2", c_double),
+        ("rg2", c_double),
+        ("box_com", c_double * 3),
+    ]
+
+
+class BfmChainSums(ctypes.Structure):
+    _fields_ = [
+        ("s", c_int64 * 3),
+        ("q", c_int64),
+        ("first", c_int32 * 3),
+        ("middle", c_int32 * 3),
+        ("last", c_int32 * 3),
+        ("n", c_uint32),
+    ]
+
+
+class BfmMsdPoint(ctypes.Structure):
+    _fields_ = [
+        ("dt", c_uint64),
+        ("n_values", c_double),
+        ("mean", c_double * 5),
+        ("sg", c_double * 5),
+    ]
+
+
 # name -> (restype, argtypes); every symbol declared in include/*.h
 _H = c_void_p
 SIGNATURES = {
@@ -106,6 +139,13 @@ SIGNATURES = {
     "bfm_set_xcd_ranges": (c_int, [_H, c_int]),
     "bfm_run_resident": (c_int, [_H, c_uint32]),
     "bfm_copy_back": (c_int, [_H]),
+    "bfm_observe": (c_int, [_H, POINTER(BfmObservables)]),
+    "bfm_chain_table": (c_int, [_H, POINTER(c_uint32), c_void_p, c_void_p, c_void_p, c_uint32]),
+    "bfm_get_chain_sums": (c_int, [_H, c_void_p, c_uint32]),
+    "bfm_msd_attach": (c_int, [_H, c_uint64, c_uint64, c_uint64, c_uint64]),
+    "bfm_msd_execute": (c_int, [_H]),
+    "bfm_msd_get_points": (c_int, [_H, c_void_p, c_uint32, POINTER(c_uint32)]),
+    "bfm_msd_write": (c_int, [_H, c_char_p]),
     "bfm_batch_create": (c_int, [POINTER(c_void_p), c_int]),
     "bfm_batch_destroy": (c_int, [_H]),
     "bfm_batch_add": (c_int, [_H, _H, POINTER(c_uint32)]),

@@ -28,6 +28,8 @@ SOURCES = [
     "capi.cpp",
     "generators.cpp",
     "bfm_io.cpp",
+    "analyzers_host.cpp",
+    "analyzers.hip",
 ]
 
 # Packed FP32 instructions stay off in every HIP object (there is no float

@@ -5,6 +5,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/pscbfm_amd.h"
 #include "bfm_io.hpp"
@@ -386,6 +387,112 @@ int bfm_set_xcd_ranges(bfm_updater *h, int on)
 {
     NEED_HANDLE(h);
     return guarded([&] { h->u.setXcdRanges(on != 0); });
+}
+
+int bfm_observe(bfm_updater *h, bfm_observables *out)
+{
+    NEED_HANDLE(h);
+    if (!out)
+        return fail(BFM_E_INVALID_ARGUMENT, "null output");
+    return guarded([&] {
+        pscbfm::Observables o;
+        h->u.observe(o);
+        std::memset(out, 0, sizeof(*out));
+        out->bonds = o.bonds;
+        std::memcpy(out->bond_hist, o.hist, sizeof(out->bond_hist));
+        out->n_chains = o.n_chains;
+        out->re2_sum = o.re2_sum;
+        out->re2 = o.re2;
+        out->rg2 = o.rg2;
+        for (int a = 0; a < 3; ++a)
+            out->box_com[a] = o.box_com[a];
+    });
+}
+
+int bfm_chain_table(bfm_updater *h, uint32_t *n_chains, uint32_t *first, uint32_t *middle, uint32_t *last,
+                    uint32_t capacity)
+{
+    NEED_HANDLE(h);
+    if (!n_chains)
+        return fail(BFM_E_INVALID_ARGUMENT, "null output");
+    return guarded([&] {
+        const pscbfm::ChainTable t = pscbfm::find_linear_chains(h->u.system());
+        if (!t.msd_error.empty())
+            throw std::invalid_argument(t.msd_error);
+        *n_chains = t.n_chains;
+        for (uint32_t c = 0; c < std::min(capacity, t.n_chains); ++c) {
+            const uint32_t b = t.offsets[c], n = t.offsets[c + 1] - b;
+            if (first)
+                first[c] = t.members[b];
+            if (middle)
+                middle[c] = t.members[b + n / 2];
+            if (last)
+                last[c] = t.members[b + n - 1];
+        }
+    });
+}
+
+static_assert(sizeof(bfm_chain_sums) == sizeof(pscbfm::ChainSums), "chain sums layout");
+
+int bfm_get_chain_sums(bfm_updater *h, bfm_chain_sums *out, uint32_t capacity)
+{
+    NEED_HANDLE(h);
+    if (!out)
+        return fail(BFM_E_INVALID_ARGUMENT, "null output");
+    return guarded([&] {
+        std::vector<pscbfm::ChainSums> v;
+        h->u.chainSums(v);
+        if (capacity < v.size())
+            throw std::invalid_argument("output holds " + std::to_string(capacity) + " chains, the system has " +
+                                        std::to_string(v.size()));
+        std::memcpy(out, v.data(), sizeof(bfm_chain_sums) * v.size());
+    });
+}
+
+int bfm_msd_attach(bfm_updater *h, uint64_t mcs_to_skip, uint64_t delta_t_realization, uint64_t delta_t_evaluate,
+                   uint64_t max_subsequences)
+{
+    NEED_HANDLE(h);
+    return guarded([&] { h->u.msdAttach(mcs_to_skip, delta_t_realization, delta_t_evaluate, max_subsequences); });
+}
+
+int bfm_msd_execute(bfm_updater *h)
+{
+    NEED_HANDLE(h);
+    return guarded([&] { h->u.msdExecute(); });
+}
+
+int bfm_msd_get_points(bfm_updater *h, bfm_msd_point *out, uint32_t capacity, uint32_t *n_points)
+{
+    NEED_HANDLE(h);
+    if (!n_points)
+        return fail(BFM_E_INVALID_ARGUMENT, "null output");
+    return guarded([&] {
+        if (!h->u.analyzer().msdAttached())
+            throw pscbfm::StateError("no MSD analyzer attached");
+        const auto pts = h->u.analyzer().msdPoints();
+        *n_points = uint32_t(pts.size());
+        for (uint32_t k = 0; out && k < std::min<size_t>(capacity, pts.size()); ++k) {
+            out[k].dt = pts[k].dt;
+            out[k].n_values = pts[k].n_values;
+            for (int g = 0; g < 5; ++g) {
+                out[k].mean[g] = pts[k].mean[g];
+                out[k].sg[g] = pts[k].sg[g];
+            }
+        }
+    });
+}
+
+int bfm_msd_write(bfm_updater *h, const char *path)
+{
+    NEED_HANDLE(h);
+    if (!path)
+        return fail(BFM_E_INVALID_ARGUMENT, "null path");
+    return guarded([&] {
+        if (!h->u.analyzer().msdAttached())
+            throw pscbfm::StateError("no MSD analyzer attached");
+        h->u.analyzer().msdWrite(h->u.system(), path);
+    });
 }
 
 int bfm_batch_create(bfm_batch **out, int gpu)

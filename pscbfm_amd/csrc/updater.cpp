@@ -370,12 +370,47 @@ void UpdaterGPUScBFM_AB_Type::freeDevice()
         mStream = nullptr;
     }
     mResorter.release();
+    mAnalyzer.release();
     if (mDevSlab) {
         hipFree(mDevSlab);
         mDevSlab = nullptr;
     }
     mBuf = SweepBuffers{};
     mInitialized = false;
+}
+
+void UpdaterGPUScBFM_AB_Type::observe(Observables &out)
+{
+    if (!mInitialized)
+        throw StateError("observables before initialize");
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    mAnalyzer.observe(mSys, mBuf, mStream, out);
+}
+
+void UpdaterGPUScBFM_AB_Type::chainSums(std::vector<ChainSums> &out)
+{
+    if (!mInitialized)
+        throw StateError("chain sums before initialize");
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    mAnalyzer.chainSums(mSys, mBuf, mStream, out);
+}
+
+void UpdaterGPUScBFM_AB_Type::msdAttach(uint64_t mcs_to_skip, uint64_t delta_t_realization,
+                                        uint64_t delta_t_evaluate, uint64_t max_subsequences)
+{
+    if (!mInitialized)
+        throw StateError("MSD analyzer before initialize");
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    mAnalyzer.msdCreate(mSys, mBuf, mStream, uint64_t(mAge), mcs_to_skip, delta_t_realization, delta_t_evaluate,
+                        max_subsequences);
+}
+
+void UpdaterGPUScBFM_AB_Type::msdExecute()
+{
+    if (!mInitialized)
+        throw StateError("MSD analyzer before initialize");
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    mAnalyzer.msdExecute(mSys, mBuf, mStream, uint64_t(mAge));
 }
 
 void UpdaterGPUScBFM_AB_Type::cleanup()

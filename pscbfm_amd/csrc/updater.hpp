@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "analyzers.hpp"
 #include "bfm_kernels.hpp"
 #include "device_slab.hpp"
 #include "host_system.hpp"
@@ -87,6 +88,14 @@ public:
     uint64_t timedLaunches() const { return mTimedLaunches; }
     uint64_t sorts() const { return mSorts; }
 
+    // analyzers on the resident device state (no copy-back needed)
+    void observe(Observables &out);
+    void chainSums(std::vector<ChainSums> &out);
+    void msdAttach(uint64_t mcs_to_skip, uint64_t delta_t_realization, uint64_t delta_t_evaluate,
+                   uint64_t max_subsequences);
+    void msdExecute();
+    const DeviceAnalyzer &analyzer() const { return mAnalyzer; }
+
 private:
     int32_t at(T_Id i, int a) const;
     void freeDevice();
@@ -112,6 +121,7 @@ private:
     int64_t mLastSortAge = 0;    // age of the last sort (initialize sorts)
     bool mLayoutMoved = false;   // the device order differs from the host's to_sorted
     uint64_t mSorts = 0;
+    DeviceAnalyzer mAnalyzer;    // observables and AnalyzerMsd on the device
 
     uint32_t mTimingEvery = 0;   // time every n-th substep (0: off)
     bool mXcdRanges = true;

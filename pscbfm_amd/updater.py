@@ -161,6 +161,63 @@ class UpdaterGPUScBFM_AB_Type:
     def copyBack(self) -> None:
         _capi.check(self._lib.bfm_copy_back(self._h))
 
+    # ---- analyzers on the device (src/AnalyzerMsd.tpp:99-468) ----------------
+    def observe(self) -> dict:
+        """Bond-vector histogram (by bond-table index), Re^2, Rg^2 and the box
+        centre of mass of the resident state, reduced on the device."""
+        o = _capi.BfmObservables()
+        _capi.check(self._lib.bfm_observe(self._h, byref(o)))
+        return {
+            "bonds": o.bonds,
+            "bond_hist": np.ctypeslib.as_array(o.bond_hist).astype(np.uint64).copy(),
+            "n_chains": o.n_chains,
+            "re2_sum": o.re2_sum,
+            "re2": o.re2,
+            "rg2": o.rg2,
+            "box_com": np.array(o.box_com[:]),
+        }
+
+    def chainTable(self) -> dict:
+        """First, middle and last monomer of every linear chain, as the
+        reference's AnalyzerMsd finds them (host only)."""
+        n = c_uint32()
+        _capi.check(self._lib.bfm_chain_table(self._h, byref(n), None, None, None, 0))
+        out = {k: np.empty(n.value, dtype=np.uint32) for k in ("first", "middle", "last")}
+        _capi.check(self._lib.bfm_chain_table(self._h, byref(n), out["first"].ctypes.data,
+                                              out["middle"].ctypes.data, out["last"].ctypes.data, n.value))
+        return out
+
+    def chainSums(self, n_chains: int) -> np.ndarray:
+        """Exact per-chain sums of the resident state (structured array)."""
+        dt = np.dtype([("s", "<i8", 3), ("q", "<i8"), ("first", "<i4", 3), ("middle", "<i4", 3),
+                       ("last", "<i4", 3), ("n", "<u4")])
+        assert dt.itemsize == ctypes.sizeof(_capi.BfmChainSums)
+        out = np.zeros(n_chains, dtype=dt)
+        _capi.check(self._lib.bfm_get_chain_sums(self._h, out.ctypes.data, n_chains))
+        return out
+
+    def attachMsd(self, mcs_to_skip: int, delta_t_realization: int, delta_t_evaluate: int,
+                  max_subsequences: int) -> None:
+        """AnalyzerMsd(...) + initialize(): evaluates once at the current age."""
+        _capi.check(self._lib.bfm_msd_attach(self._h, mcs_to_skip, delta_t_realization, delta_t_evaluate,
+                                             max_subsequences))
+
+    def executeMsd(self) -> None:
+        """AnalyzerMsd::execute at the current age."""
+        _capi.check(self._lib.bfm_msd_execute(self._h))
+
+    def msdPoints(self) -> list[dict]:
+        n = c_uint32()
+        _capi.check(self._lib.bfm_msd_get_points(self._h, None, 0, byref(n)))
+        buf = (_capi.BfmMsdPoint * max(n.value, 1))()
+        _capi.check(self._lib.bfm_msd_get_points(self._h, buf, n.value, byref(n)))
+        return [{"dt": p.dt, "n_values": p.n_values, "mean": list(p.mean), "sg": list(p.sg)}
+                for p in buf[: n.value]]
+
+    def writeMsd(self, path: str) -> None:
+        """AnalyzerMsd::cleanup's output file."""
+        _capi.check(self._lib.bfm_msd_write(self._h, str(path).encode()))
+
 
 def updater_from_system(system: dict, seed: int, gpu: int | None = 0, age: int = 0,
                         validate_only: bool = False, spatial_sorting: bool = True,
