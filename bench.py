@@ -9,7 +9,8 @@ timed region.  Prints one JSON line.
 
 The cpu_baseline leg times the oracle's random-sequential updater (the
 LeMonADE-style CPU updater, one core) on a bounded number of MCS of the same
-system.
+system, with the serial colour-sweep replay (one core) and the OpenMP colour
+sweep (all threads) beside it.
 """
 from __future__ import annotations
 
@@ -46,19 +47,41 @@ def measured_traffic(workload: str):
 
 
 def cpu_baseline(system: dict, seed: int, target_s: float) -> dict:
+    """Three CPU legs on the same system, each filling about target_s / 3:
+    the random-sequential updater (the reference's CPU path is LeMonADE's
+    serial updater; this is its restatement, one core; the reported value),
+    the serial colour-sweep replay (one core) and the OpenMP colour sweep
+    (all threads OpenMP is given)."""
     from oracle import oracle as orc
     osys = orc.System(system, seed)
     n = len(system["xyz"])
-    t = time.perf_counter()
-    xyz, _ = orc.random_sequential(osys, system["xyz"], 0, 1)
-    one = time.perf_counter() - t
-    m = max(1, int(target_s / max(one, 1e-6)) - 1)
-    t = time.perf_counter()
-    orc.random_sequential(osys, xyz, 1, m)
-    dt = time.perf_counter() - t
-    return {"value": m * n / dt, "unit": "attempts/s", "cores": 1, "kind": "port",
-            "sample": f"oracle random-sequential updater, {m} MCS x {n} monomers of the same system "
-                      f"({dt:.1f} s on 1 host core)"}
+    share = target_s / 3.0
+
+    def timed(run):
+        t = time.perf_counter()
+        xyz = run(system["xyz"], 0, 1)
+        one = time.perf_counter() - t
+        m = max(1, int(share / max(one, 1e-6)) - 1)
+        t = time.perf_counter()
+        run(xyz, 1, m)
+        dt = time.perf_counter() - t
+        return m, dt, m * n / dt
+
+    m_rs, dt_rs, rs = timed(lambda x, a, k: orc.random_sequential(osys, x, a, k)[0])
+    m_sw, dt_sw, sw = timed(lambda x, a, k: orc.sweep(osys, x, a, k, 1)[0])
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    m_omp, dt_omp, omp = timed(lambda x, a, k: orc.sweep_parallel(osys, x, a, k, threads)[0])
+    log(f"CPU: random-sequential {rs:.3g}, sweep replay {sw:.3g} (1 core), OpenMP sweep {omp:.3g} "
+        f"({threads} threads) attempts/s")
+    return {"value": rs, "unit": "attempts/s", "cores": 1, "kind": "port",
+            "sample": f"oracle random-sequential updater, {m_rs} MCS x {n} monomers of the same system "
+                      f"({dt_rs:.1f} s on 1 host core)",
+            "legs": {
+                "sweep_replay": {"value": sw, "cores": 1,
+                                 "sample": f"{m_sw} MCS ({dt_sw:.1f} s), serial colour sweep, bit-exact oracle"},
+                "sweep_openmp": {"value": omp, "cores": threads,
+                                 "sample": f"{m_omp} MCS ({dt_omp:.1f} s), OpenMP colour sweep"},
+            }}
 
 
 def batched_leg(args) -> dict:
@@ -174,7 +197,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1000, help="timed Monte-Carlo steps")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large_melt")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=24.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--time-every", type=int, default=8,
                     help="time the kernels of every n-th substep with HIP events")

@@ -9,6 +9,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <omp.h>
+
 /* ------------------------------------------------------------------------ */
 /* Philox4x32-10, written from the round function published by Salmon et al. */
 
@@ -510,6 +512,128 @@ int orc_sweep(const orc_system *s, int32_t *pos, uint64_t age0, uint32_t n_mcs,
         }
     }
     free(cnt); free(members); free(prop_id); free(prop_dir); free(ok2); free(occ); free(tmp); free(order);
+    if (accepted) *accepted = n_acc;
+    if (proposals) *proposals = n_prop;
+    return 0;
+}
+
+int orc_sweep_parallel(const orc_system *s, int32_t *pos, uint64_t age0, uint32_t n_mcs,
+                       int threads, uint64_t *accepted, uint64_t *proposals)
+{
+    geom_t g;
+    if (make_geom(s, &g) != 0 || s->n_colors == 0 || threads < 0 ||
+        (s->rng != ORC_RNG_INT_HASH && s->rng != ORC_RNG_PHILOX))
+        return -1;
+    if (threads == 0)
+        threads = omp_get_max_threads();
+    const size_t n1 = s->n ? s->n : 1;
+    uint32_t *cnt = (uint32_t *)calloc(s->n_colors + 1, sizeof(uint32_t));
+    uint32_t *members = (uint32_t *)malloc(sizeof(uint32_t) * n1);
+    uint8_t *flag = (uint8_t *)calloc(n1, 1);         /* per member slot: 0, or direction + 1 */
+    size_t *claim = (size_t *)malloc(sizeof(size_t) * n1);   /* claimed corner site */
+    uint8_t *occ = (uint8_t *)calloc(g.volume, 1);
+    uint8_t *tmp = (uint8_t *)calloc(g.volume, 1);
+    uint32_t *order = (uint32_t *)malloc(sizeof(uint32_t) * s->n_colors);
+    if (!cnt || !members || !flag || !claim || !occ || !tmp || !order) {
+        free(cnt); free(members); free(flag); free(claim); free(occ); free(tmp); free(order);
+        return -2;
+    }
+    for (uint32_t i = 0; i < s->n; ++i) {
+        if (s->color[i] >= s->n_colors) {
+            free(cnt); free(members); free(flag); free(claim); free(occ); free(tmp); free(order);
+            return -3;
+        }
+        cnt[s->color[i] + 1]++;
+    }
+    for (uint32_t c = 0; c < s->n_colors; ++c)
+        cnt[c + 1] += cnt[c];
+    {
+        uint32_t *fill = (uint32_t *)malloc(sizeof(uint32_t) * s->n_colors);
+        memcpy(fill, cnt, sizeof(uint32_t) * s->n_colors);
+        for (uint32_t i = 0; i < s->n; ++i)
+            members[fill[s->color[i]]++] = i;
+        free(fill);
+    }
+    for (uint32_t i = 0; i < s->n; ++i) {
+        const int32_t *r = pos + 3 * (size_t)i;
+        occ[site(&g, r[0], r[1], r[2])] = 1;
+    }
+    unsigned long long n_acc = 0, n_prop = 0;
+    for (uint32_t m = 0; m < n_mcs; ++m) {
+        const uint64_t age = age0 + m;
+        orc_color_order(s->seed, age, s->n_colors, order);
+        for (uint32_t sub = 0; sub < s->n_colors; ++sub) {
+            const uint32_t c = order[sub];
+            const long b = (long)cnt[c], e = (long)cnt[c + 1];
+            unsigned long long np = 0, na = 0;
+            /* phase 1: propose and check against the state before the substep */
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : np)
+            for (long t = b; t < e; ++t) {
+                const uint32_t i = members[t];
+                const int dir = (int)(orc_direction_word(s, age, sub, i) % 6u);
+                const int32_t *r0 = pos + 3 * (size_t)i;
+                int32_t d[3], r1[3];
+                dir_delta(dir, d);
+                for (int a = 0; a < 3; ++a)
+                    r1[a] = r0[a] + d[a];
+                flag[t] = 0;
+                if (!walls_ok(s, &g, r1) || !bonds_ok(s, pos, i, r1))
+                    continue;
+                size_t f[9];
+                front9_sites(&g, r0, dir, f);
+                int blocked = 0;
+                for (int k = 0; k < 9; ++k)
+                    blocked |= occ[f[k]];
+                if (blocked)
+                    continue;
+                flag[t] = (uint8_t)(dir + 1);
+                claim[t] = site(&g, r1[0], r1[1], r1[2]);
+                ++np;
+            }
+            /* claims */
+#pragma omp parallel for num_threads(threads) schedule(static)
+            for (long t = b; t < e; ++t)
+                if (flag[t]) {
+#pragma omp atomic write
+                    tmp[claim[t]] = 1;
+                }
+            /* phase 2: no other claimed corner in the front */
+#pragma omp parallel for num_threads(threads) schedule(static)
+            for (long t = b; t < e; ++t) {
+                if (!flag[t])
+                    continue;
+                size_t f[9];
+                front9_sites(&g, pos + 3 * (size_t)members[t], flag[t] - 1, f);
+                for (int k = 0; k < 9; ++k)
+                    if (tmp[f[k]]) {
+                        flag[t] |= 0x80u;
+                        break;
+                    }
+            }
+            /* apply (accepted moves touch disjoint sites) */
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : na)
+            for (long t = b; t < e; ++t) {
+                if (!flag[t] || (flag[t] & 0x80u))
+                    continue;
+                int32_t *r = pos + 3 * (size_t)members[t];
+                int32_t d[3];
+                dir_delta(flag[t] - 1, d);
+                occ[site(&g, r[0], r[1], r[2])] = 0;
+                occ[claim[t]] = 1;
+                r[0] += d[0]; r[1] += d[1]; r[2] += d[2];
+                ++na;
+            }
+#pragma omp parallel for num_threads(threads) schedule(static)
+            for (long t = b; t < e; ++t)
+                if (flag[t]) {
+#pragma omp atomic write
+                    tmp[claim[t]] = 0;
+                }
+            n_prop += np;
+            n_acc += na;
+        }
+    }
+    free(cnt); free(members); free(flag); free(claim); free(occ); free(tmp); free(order);
     if (accepted) *accepted = n_acc;
     if (proposals) *proposals = n_prop;
     return 0;

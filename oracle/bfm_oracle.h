@@ -118,6 +118,15 @@ int orc_greedy_coloring(uint32_t n, const uint32_t *nbr_off, const uint32_t *nbr
 int orc_sweep(const orc_system *sys, int32_t *pos, uint64_t age0, uint32_t n_mcs,
               int form, uint64_t *accepted, uint64_t *proposals);
 
+/* The same sweep (form 1: corner lattice, 9-site fronts) with every phase
+ * of a substep run as an OpenMP loop over the colour's monomers on `threads`
+ * threads (0: the OpenMP default).  Each loop only reads what earlier loops
+ * wrote; the outcome depends only on the set of proposals, so the result is
+ * that of orc_sweep for any thread count (pinned by comparing the two,
+ * tests/test_oracle.py).  The all-cores CPU timing of the sweep. */
+int orc_sweep_parallel(const orc_system *sys, int32_t *pos, uint64_t age0, uint32_t n_mcs,
+                       int threads, uint64_t *accepted, uint64_t *proposals);
+
 /* LeMonADE-style random-sequential updater: per MCS, n attempts; attempt k of
  * MCS `age` draws Philox(seed, (k, age_lo, age_hi, 2)): monomer = w0 % n,
  * direction = w1 % 6; a legal move is applied at once.  This is the CPU

@@ -71,6 +71,9 @@ def load() -> ctypes.CDLL:
         lib.orc_sweep.argtypes = [POINTER(OrcSystem), c_void_p, c_uint64, c_uint32, c_int,
                                   POINTER(c_uint64), POINTER(c_uint64)]
         lib.orc_sweep.restype = c_int
+        lib.orc_sweep_parallel.argtypes = [POINTER(OrcSystem), c_void_p, c_uint64, c_uint32, c_int,
+                                           POINTER(c_uint64), POINTER(c_uint64)]
+        lib.orc_sweep_parallel.restype = c_int
         lib.orc_random_sequential.argtypes = [POINTER(OrcSystem), c_void_p, c_uint64, c_uint32,
                                               POINTER(c_uint64)]
         lib.orc_random_sequential.restype = c_int
@@ -181,6 +184,17 @@ def sweep(sys: System, xyz: np.ndarray, age0: int, n_mcs: int, form: int = 0):
                           ctypes.byref(acc), ctypes.byref(prop))
     if rc != 0:
         raise RuntimeError(f"orc_sweep failed with {rc}")
+    return a, acc.value, prop.value
+
+
+def sweep_parallel(sys: System, xyz: np.ndarray, age0: int, n_mcs: int, threads: int = 0):
+    """The corner-lattice sweep with OpenMP loops; returns (new xyz, accepted, proposals)."""
+    a = np.ascontiguousarray(xyz, dtype=np.int32).copy()
+    acc, prop = c_uint64(), c_uint64()
+    rc = load().orc_sweep_parallel(ctypes.byref(sys.struct), a.ctypes.data, age0, n_mcs, threads,
+                                   ctypes.byref(acc), ctypes.byref(prop))
+    if rc != 0:
+        raise RuntimeError(f"orc_sweep_parallel failed with {rc}")
     return a, acc.value, prop.value
 
 

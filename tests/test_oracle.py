@@ -234,6 +234,23 @@ def test_three_forms_agree_on_network():
     assert orc.check_system(sys_, outs[0][0])[0] == 0
 
 
+@pytest.mark.parametrize("threads", [1, 3, 8])
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 1, 0)])
+def test_parallel_sweep_equals_serial(threads, periodic):
+    """The OpenMP sweep (every phase a parallel loop over the colour) gives the
+    serial replay's trajectory and counts for any thread count."""
+    s = dict(melt(32, 128, 16, 8), periodic=periodic)
+    sys_ = orc.System(s, 0x5EED77)
+    want = orc.sweep(sys_, s["xyz"], 3, 40, 1)
+    got = orc.sweep_parallel(sys_, s["xyz"], 3, 40, threads)
+    assert np.array_equal(got[0], want[0])
+    assert got[1:] == want[1:]
+    net = __import__("pscbfm_amd").systems.diamond_network(64, (2, 2, 8), 32, seed=3)
+    nsys = orc.System(net, 19)
+    assert np.array_equal(orc.sweep_parallel(nsys, net["xyz"], 0, 20, threads)[0],
+                          orc.sweep(nsys, net["xyz"], 0, 20, 1)[0])
+
+
 # ---- integer-hash generator (the reference's default Rngs::Hash) -------------
 
 def _int_hash_vectors():
