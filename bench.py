@@ -47,17 +47,16 @@ def measured_traffic(workload: str):
 
 
 def cpu_baseline(system: dict, seed: int, target_s: float) -> dict:
-    """Three CPU legs on the same system, each filling about target_s / 3:
-    the random-sequential updater (the reference's CPU path is LeMonADE's
-    serial updater; this is its restatement, one core; the reported value),
-    the serial colour-sweep replay (one core) and the OpenMP colour sweep
-    (all threads OpenMP is given)."""
+    """Three CPU legs on the same system, filling about target_s together:
+    half for the random-sequential updater (the reference's CPU path is
+    LeMonADE's serial updater; this is its restatement, one core; the reported
+    value), a quarter each for the serial colour-sweep replay (one core) and
+    the OpenMP colour sweep (all threads OpenMP is given)."""
     from oracle import oracle as orc
     osys = orc.System(system, seed)
     n = len(system["xyz"])
-    share = target_s / 3.0
 
-    def timed(run):
+    def timed(run, share):
         t = time.perf_counter()
         xyz = run(system["xyz"], 0, 1)
         one = time.perf_counter() - t
@@ -67,10 +66,10 @@ def cpu_baseline(system: dict, seed: int, target_s: float) -> dict:
         dt = time.perf_counter() - t
         return m, dt, m * n / dt
 
-    m_rs, dt_rs, rs = timed(lambda x, a, k: orc.random_sequential(osys, x, a, k)[0])
-    m_sw, dt_sw, sw = timed(lambda x, a, k: orc.sweep(osys, x, a, k, 1)[0])
+    m_rs, dt_rs, rs = timed(lambda x, a, k: orc.random_sequential(osys, x, a, k)[0], target_s / 2)
+    m_sw, dt_sw, sw = timed(lambda x, a, k: orc.sweep(osys, x, a, k, 1)[0], target_s / 4)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    m_omp, dt_omp, omp = timed(lambda x, a, k: orc.sweep_parallel(osys, x, a, k, threads)[0])
+    m_omp, dt_omp, omp = timed(lambda x, a, k: orc.sweep_parallel(osys, x, a, k, threads)[0], target_s / 4)
     log(f"CPU: random-sequential {rs:.3g}, sweep replay {sw:.3g} (1 core), OpenMP sweep {omp:.3g} "
         f"({threads} threads) attempts/s")
     return {"value": rs, "unit": "attempts/s", "cores": 1, "kind": "port",

@@ -144,6 +144,14 @@ static size_t site(const geom_t *g, int32_t x, int32_t y, int32_t z)
 }
 
 /* Directions 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z (UpdaterGPUScBFM_AB_Type.cu:1537-1541). */
+/* Linear index of the aligned 2x2x2 block holding a site. */
+static size_t block_of(const geom_t *g, int32_t x, int32_t y, int32_t z)
+{
+    const size_t bx = (size_t)((x & g->M[0]) >> 1), by = (size_t)((y & g->M[1]) >> 1),
+                 bz = (size_t)((z & g->M[2]) >> 1);
+    return bx + (size_t)(g->L[0] / 2) * (by + (size_t)(g->L[1] / 2) * bz);
+}
+
 static void dir_delta(int dir, int32_t d[3])
 {
     d[0] = d[1] = d[2] = 0;
@@ -346,8 +354,11 @@ int orc_sweep(const orc_system *s, int32_t *pos, uint64_t age0, uint32_t n_mcs,
     uint8_t *occ = (uint8_t *)calloc(g.volume, 1);   /* form 0/2: 8-site occupancy; form 1: corners */
     uint8_t *tmp = (uint8_t *)calloc(g.volume, 1);   /* form 0: claim counts; form 1: tentative corners */
     uint32_t *order = (uint32_t *)malloc(sizeof(uint32_t) * s->n_colors);
-    if (!cnt || !members || !prop_id || !prop_dir || !ok2 || !occ || !tmp || !order) {
+    /* accepted moves per aligned 2x2x2 block of sites in the current substep */
+    uint8_t *touch = (uint8_t *)calloc(g.volume / 8 + 1, 1);
+    if (!cnt || !members || !prop_id || !prop_dir || !ok2 || !occ || !tmp || !order || !touch) {
         free(cnt); free(members); free(prop_id); free(prop_dir); free(ok2); free(occ); free(tmp); free(order);
+        free(touch);
         return -2;
     }
     for (uint32_t i = 0; i < s->n; ++i) {
@@ -486,6 +497,37 @@ int orc_sweep(const orc_system *s, int32_t *pos, uint64_t age0, uint32_t n_mcs,
                 }
             }
 
+            /* Invariant the product's byte-wise lattice update rests on: no two
+             * accepted moves of a substep touch the same aligned 2x2x2 block
+             * (old or new corner).  Violations return -4. */
+            int shared_block = 0;
+            for (uint32_t p = 0; p < np; ++p) {
+                if (!ok2[p])
+                    continue;
+                const int32_t *r = pos + 3 * (size_t)prop_id[p];
+                int32_t d[3];
+                dir_delta(prop_dir[p], d);
+                const size_t b0 = block_of(&g, r[0], r[1], r[2]);
+                const size_t b1 = block_of(&g, r[0] + d[0], r[1] + d[1], r[2] + d[2]);
+                shared_block |= touch[b0]++ != 0;
+                if (b1 != b0)
+                    shared_block |= touch[b1]++ != 0;
+            }
+            for (uint32_t p = 0; p < np; ++p) {
+                if (!ok2[p])
+                    continue;
+                const int32_t *r = pos + 3 * (size_t)prop_id[p];
+                int32_t d[3];
+                dir_delta(prop_dir[p], d);
+                touch[block_of(&g, r[0], r[1], r[2])] = 0;
+                touch[block_of(&g, r[0] + d[0], r[1] + d[1], r[2] + d[2])] = 0;
+            }
+            if (shared_block) {
+                free(cnt); free(members); free(prop_id); free(prop_dir); free(ok2); free(occ); free(tmp);
+                free(order); free(touch);
+                return -4;
+            }
+
             /* apply */
             for (uint32_t p = 0; p < np; ++p) {
                 if (!ok2[p])
@@ -512,6 +554,7 @@ int orc_sweep(const orc_system *s, int32_t *pos, uint64_t age0, uint32_t n_mcs,
         }
     }
     free(cnt); free(members); free(prop_id); free(prop_dir); free(ok2); free(occ); free(tmp); free(order);
+    free(touch);
     if (accepted) *accepted = n_acc;
     if (proposals) *proposals = n_prop;
     return 0;

@@ -21,7 +21,9 @@
 //    barrier is the one after the LDS tables are filled at kernel start;
 //  * no memset: the marks of a substep are cleared by the next check launch
 //    (the claim lattice is double-buffered by substep parity);
-//  * an accepted move flips its 2 corner bits with one atomic XOR per word.
+//  * an accepted move writes its old and new lattice bytes with plain byte
+//    stores: each byte is an aligned 2x2x2 block of sites, which holds at
+//    most one corner, and no two accepted moves of a substep share a block.
 #include "bfm_kernels.hpp"
 
 #include <algorithm>
@@ -283,16 +285,21 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
             k4[q] = f.m[q] ? claim[f.w[q]] : 0u;
         const bool clash = ((k4[0] & f.m[0]) | (k4[1] & f.m[1]) | (k4[2] & f.m[2]) | (k4[3] & f.m[3])) != 0u;
         if (!clash) {
-            // vacate the old corner, occupy the new one: distinct bits of every
-            // accepted move, so XOR is exact under any interleaving
-            const uint32_t w0 = f.old_site >> 5, w1 = f.new_site >> 5;
-            const uint32_t b0 = 1u << (f.old_site & 31u), b1 = 1u << (f.new_site & 31u);
-            if (w0 == w1) {
-                atomicXor((uint32_t *)&lattice[w0], b0 | b1);
-            } else {
-                atomicXor((uint32_t *)&lattice[w0], b0);
-                atomicXor((uint32_t *)&lattice[w1], b1);
-            }
+            // Vacate the old corner, occupy the new one.  Byte j of the
+            // lattice is the aligned 2x2x2 block of sites j*8 .. j*8+7 (the
+            // lowest Morton bits are x0, y0, z0 in every geometry), and two
+            // corners in one block would be overlapping cubes: a valid state
+            // has at most one corner per byte.  An accepted move's old block
+            // holds only its own corner, its new block holds none, and no
+            // other accepted move of the substep touches either (the front
+            // tests reject every pair whose cubes would overlap).  So each
+            // move owns its two bytes and writes their new values with plain
+            // byte stores: no atomics, no reads.
+            const gptr<uint8_t> blocks = (gptr<uint8_t>)lattice;
+            const uint32_t nb = f.new_site >> 3;
+            if ((f.old_site >> 3) != nb)
+                blocks[f.old_site >> 3] = uint8_t(0);
+            blocks[nb] = uint8_t(1u << (f.new_site & 7u));
             // new wrapped coordinate on the move axis; crossing the box
             // boundary changes the periodic image (only on periodic axes:
             // walls keep the others inside)
