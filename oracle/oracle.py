@@ -130,13 +130,28 @@ def csr(n: int, bonds: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     return off, dst.astype(np.uint32)
 
 
+def directed_csr(n: int, links: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Links as given (source -> target), grouped by source in the given order."""
+    b = np.asarray(links, dtype=np.int64).reshape(-1, 2)
+    order = np.argsort(b[:, 0], kind="stable")
+    src, dst = b[order, 0], b[order, 1]
+    off = np.zeros(n + 1, dtype=np.uint32)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off).astype(np.uint32)
+    return off, dst.astype(np.uint32)
+
+
 class System:
     """Holds the arrays behind an OrcSystem so they outlive the struct."""
 
     def __init__(self, system: dict, seed: int, colors: np.ndarray | None = None, rng: int = RNG_INT_HASH):
         self.xyz0 = np.ascontiguousarray(system["xyz"], dtype=np.int32).reshape(-1, 3)
         n = self.xyz0.shape[0]
-        self.off, self.idx = csr(n, system["bonds"])
+        if system.get("links") is not None:
+            # directed links: a bond is tested only by the monomer listing it
+            self.off, self.idx = directed_csr(n, system["links"])
+        else:
+            self.off, self.idx = csr(n, system["bonds"])
         if colors is None:
             colors, nc = greedy_coloring(n, self.off, self.idx)
         else:

@@ -7,7 +7,7 @@
 
 namespace pscbfm {
 
-constexpr uint32_t kPerformGroup = 16;        // check waves whose survivors one perform wave takes
+constexpr uint32_t kPerformGroup = 4;         // check waves whose survivors one perform wave takes (4: ~42 survivors, one round of 64 lanes)
 constexpr uint32_t kStatSlots = 4096;          // spread of the accepted / proposed counters (one per perform wave up to 4096 waves)
 
 // Everything one substep of one system needs on the device.

@@ -225,7 +225,8 @@ def updater_from_system(system: dict, seed: int, gpu: int | None = 0, age: int =
                         steps_between_sortings: int | None = None) -> UpdaterGPUScBFM_AB_Type:
     """Builds an updater the way the LeMonADE adapter does
     (GPUScBFM_AB_Type.h:120-184): box, periodicity, age, positions, both
-    directions of every bond, the 512-entry bond table, then initialize."""
+    directions of every bond (or, if the system has "links", exactly those
+    directed links), the 512-entry bond table, then initialize."""
     u = UpdaterGPUScBFM_AB_Type()
     if gpu is not None and not validate_only:
         u.setGpu(gpu)
@@ -237,12 +238,19 @@ def updater_from_system(system: dict, seed: int, gpu: int | None = 0, age: int =
     xyz = np.asarray(system["xyz"], dtype=np.int32).reshape(-1, 3)
     u.setNrOfAllMonomers(xyz.shape[0])
     u.setMonomerCoordinatesBulk(xyz)
-    bonds = np.asarray(system["bonds"], dtype=np.uint32).reshape(-1, 2)
-    if bonds.size:
-        both = np.empty((2 * bonds.shape[0], 2), dtype=np.uint32)
-        both[0::2] = bonds
-        both[1::2] = bonds[:, ::-1]
-        u.setConnectivityBulk(both)
+    if system.get("links") is not None:
+        # directed links as given (setConnectivity is directed: a bond is
+        # tested only by moves of the monomer that lists it)
+        links = np.asarray(system["links"], dtype=np.uint32).reshape(-1, 2)
+        if links.size:
+            u.setConnectivityBulk(links)
+    else:
+        bonds = np.asarray(system["bonds"], dtype=np.uint32).reshape(-1, 2)
+        if bonds.size:
+            both = np.empty((2 * bonds.shape[0], 2), dtype=np.uint32)
+            both[0::2] = bonds
+            both[1::2] = bonds[:, ::-1]
+            u.setConnectivityBulk(both)
     forbidden = np.asarray(system["forbidden"], dtype=np.uint8)
     for dx in range(-4, 4):
         for dy in range(-4, 4):

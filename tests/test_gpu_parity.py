@@ -144,6 +144,37 @@ def test_network_three_colours(gpu_ready):
     assert st["accepted"] > 0
 
 
+@pytest.mark.parametrize("pattern", ["forward", "alternating"])
+def test_one_way_links(gpu_ready, pattern):
+    """Links given one way only (setConnectivity is directed): only the listing
+    monomer tests the bond, the other side keeps a mirror entry current.
+    The oracle replays the same directed links with the product's colours;
+    re-sorts every 20 MCS move the entries."""
+    s = systems.linear_melt(32, 128, 16, seed=55)
+    b = np.asarray(s["bonds"], dtype=np.int64)
+    if pattern == "forward":
+        links = b
+    else:
+        links = np.where((np.arange(len(b)) % 2 == 0)[:, None], b, b[:, ::-1])
+    s = dict(s, links=links)
+    n = len(s["xyz"])
+    seed = 0x5EED0000000000B1
+    print(f"case: one-way links {pattern} n={n}", flush=True)
+    u = updater_from_system(s, seed=seed, gpu=0, steps_between_sortings=20)
+    u.runSimulationOnGPU(60)
+    u.runSimulationOnGPU(90)
+    got = u.getMonomerPositions(n)
+    colors, _ = u.getColors(n)
+    stats = u.getStats()
+    u.cleanup()
+    osys = orc.System(s, seed, colors)
+    want, acc, prop = orc.sweep(osys, s["xyz"], 0, 150, 1)
+    diff = np.nonzero((got != want).any(axis=1))[0]
+    assert diff.size == 0, f"{diff.size} monomers differ, first {diff[:5]}"
+    assert (stats["accepted"], stats["proposals"]) == (acc, prop)
+    assert stats["sorts"] > 0
+
+
 def test_network_full_size_20_mcs(gpu_ready):
     # the end-linked network workload (256^3, 16384 crosslinkers, 32768 strands of N=32)
     s = systems.diamond_network(256, (8, 8, 32), 32, seed=0x5EED000000000003)
