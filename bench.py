@@ -241,7 +241,9 @@ def main() -> int:
 
     # HIP events around the kernels of every 8th substep of the timed region:
     # live kernel durations without slowing the launch pipeline of the rest
-    u.setKernelTiming(args.time_every)
+    # (short runs time every substep, so that some launches are timed)
+    every = args.time_every if args.steps * n_colors >= 4 * args.time_every else 1
+    u.setKernelTiming(every)
     u.runResident(args.warmup)
     s0 = u.getStats()  # synchronizes the updater's stream
     t0 = time.perf_counter()
