@@ -10,6 +10,60 @@
 
 namespace pscbfm {
 
+PinnedPool::~PinnedPool()
+{
+    for (Buf &b : mBufs) {
+        if (b.done) {
+            hipEventSynchronize(b.done);
+            hipEventDestroy(b.done);
+        }
+        if (b.p)
+            hipHostFree(b.p);
+    }
+}
+
+size_t PinnedPool::take(size_t bytes)
+{
+    size_t best = mBufs.size();
+    for (size_t k = 0; k < mBufs.size(); ++k) {
+        const Buf &b = mBufs[k];
+        if (b.held || b.bytes < bytes || (best < mBufs.size() && mBufs[best].bytes <= b.bytes))
+            continue;
+        if (hipEventQuery(b.done) == hipSuccess)
+            best = k;
+    }
+    if (best == mBufs.size()) {
+        Buf b;
+        b.bytes = std::max<size_t>(size_t(1) << 16, bytes);
+        while (b.bytes & (b.bytes - 1))
+            b.bytes += b.bytes & ~(b.bytes - 1);   // next power of two
+        hip_check(hipHostMalloc(&b.p, b.bytes, hipHostMallocDefault), "hipHostMalloc (staging)");
+        hip_check(hipEventCreateWithFlags(&b.done, hipEventDisableTiming), "hipEventCreate (staging)");
+        mBufs.push_back(b);
+    }
+    mBufs[best].held = true;
+    return best;
+}
+
+void PinnedPool::record(size_t k, hipStream_t stream)
+{
+    hip_check(hipEventRecord(mBufs[k].done, stream), "hipEventRecord (staging)");
+}
+
+bool PinnedPool::ready(size_t k) const
+{
+    const hipError_t e = hipEventQuery(mBufs[k].done);
+    if (e == hipErrorNotReady)
+        return false;
+    hip_check(e, "hipEventQuery (staging)");
+    return true;
+}
+
+void PinnedPool::wait(size_t k) const
+{
+    hip_check(hipEventSynchronize(mBufs[k].done), "hipEventSynchronize (staging)");
+}
+
 BatchEngine::BatchEngine(int gpu) : mGpu(gpu)
 {
     int n = 0;
@@ -99,8 +153,12 @@ uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age)
     m->slab = allocate(L.total, "allocation of a batch member");
     m->bytes = L.total;
     try {
-        // the uploads read m->sys, which the member keeps until it is removed
-        m->dev.buf = upload_slab(m->sys, m->slab, L, seed, mStream);
+        // staged through pinned memory, so that admitting a member never
+        // waits for the chunks already queued
+        const size_t k = mPinned.take(L.total);
+        m->dev.buf = upload_slab(m->sys, m->slab, L, seed, mStream, mPinned.data(k));
+        mPinned.record(k, mStream);
+        mPinned.give(k);
     } catch (...) {
         release(m->slab);
         throw;
@@ -329,16 +387,38 @@ void BatchEngine::run(uint32_t n_mcs, uint32_t chunk)
 
 void BatchEngine::copyBack(uint32_t id)
 {
+    const Download d = download(id);
+    wait(d);
+    unpack(d);
+}
+
+BatchEngine::Download BatchEngine::download(uint32_t id)
+{
     Member &m = *mMembers.at(id);
     if (!m.slab)
         throw std::invalid_argument("batch: member was removed");
-    synchronize();
-    HostSystem &s = m.sys;
-    hip_check(hipMemcpy(s.ppos_sorted.data(), m.dev.buf.ppos, sizeof(uint32_t) * s.n_pad, hipMemcpyDeviceToHost),
-              "download positions");
-    hip_check(hipMemcpy(s.image_sorted.data(), m.dev.buf.image, sizeof(int32_t) * 3 * size_t(s.n_pad),
-                        hipMemcpyDeviceToHost),
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    const HostSystem &s = m.sys;
+    const size_t pos_bytes = sizeof(uint32_t) * s.n_pad, img_bytes = sizeof(int32_t) * 3 * size_t(s.n_pad);
+    Download d;
+    d.member = id;
+    d.buf = mPinned.take(pos_bytes + img_bytes);
+    char *dst = static_cast<char *>(mPinned.data(d.buf));
+    hip_check(hipMemcpyAsync(dst, m.dev.buf.ppos, pos_bytes, hipMemcpyDeviceToHost, mStream), "download positions");
+    hip_check(hipMemcpyAsync(dst + pos_bytes, m.dev.buf.image, img_bytes, hipMemcpyDeviceToHost, mStream),
               "download images");
+    mPinned.record(d.buf, mStream);
+    return d;
+}
+
+void BatchEngine::unpack(const Download &d)
+{
+    HostSystem &s = mMembers.at(d.member)->sys;
+    const size_t pos_bytes = sizeof(uint32_t) * s.n_pad, img_bytes = sizeof(int32_t) * 3 * size_t(s.n_pad);
+    const char *src = static_cast<const char *>(mPinned.data(d.buf));
+    std::memcpy(s.ppos_sorted.data(), src, pos_bytes);
+    std::memcpy(s.image_sorted.data(), src + pos_bytes, img_bytes);
+    mPinned.give(d.buf);
     unpack_positions(s);
 }
 

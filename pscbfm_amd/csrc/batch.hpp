@@ -26,6 +26,33 @@
 
 namespace pscbfm {
 
+// Pinned host buffers for asynchronous copies.  A buffer is held by its user
+// until given back; a given-back buffer is reused once the work recorded on
+// its event has run.
+class PinnedPool {
+public:
+    PinnedPool() = default;
+    ~PinnedPool();
+    PinnedPool(const PinnedPool &) = delete;
+    PinnedPool &operator=(const PinnedPool &) = delete;
+
+    size_t take(size_t bytes);   // index of a free buffer of at least `bytes`
+    void *data(size_t k) const { return mBufs[k].p; }
+    void record(size_t k, hipStream_t stream);   // marks the work queued so far
+    bool ready(size_t k) const;                  // that work has run
+    void wait(size_t k) const;
+    void give(size_t k) { mBufs[k].held = false; }
+
+private:
+    struct Buf {
+        void *p = nullptr;
+        size_t bytes = 0;
+        hipEvent_t done = nullptr;
+        bool held = false;
+    };
+    std::vector<Buf> mBufs;
+};
+
 class BatchEngine {
 public:
     explicit BatchEngine(int gpu);
@@ -46,6 +73,19 @@ public:
 
     // Copies a member's positions back into its HostSystem (unsorted, by id).
     void copyBack(uint32_t id);
+    // The same in two halves: download() queues the copy of the member's
+    // positions into pinned memory behind the work queued so far and returns
+    // at once; once finished(), unpack() fills the member's HostSystem.  The
+    // member may be removed in between (its memory is freed in stream order,
+    // after the copy).
+    struct Download {
+        size_t buf = 0;
+        uint32_t member = 0;
+    };
+    Download download(uint32_t id);
+    bool finished(const Download &d) const { return mPinned.ready(d.buf); }
+    void wait(const Download &d) const { mPinned.wait(d.buf); }
+    void unpack(const Download &d);
     // Frees a member's device memory; its id is not reused.
     void remove(uint32_t id);
 
@@ -85,6 +125,7 @@ private:
 
     int mGpu;
     hipStream_t mStream = nullptr;
+    PinnedPool mPinned;   // staging of member uploads and downloads
     std::vector<std::unique_ptr<Member>> mMembers;
     size_t mBytes = 0;
 

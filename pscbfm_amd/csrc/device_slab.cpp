@@ -2,6 +2,7 @@
 #include "device_slab.hpp"
 
 #include <algorithm>
+#include <cstring>
 
 #include "updater.hpp"
 
@@ -34,9 +35,15 @@ SlabLayout slab_layout(const HostSystem &s)
     return L;
 }
 
-SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, uint64_t seed, hipStream_t stream)
+SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, uint64_t seed, hipStream_t stream,
+                         void *staging)
 {
     auto up = [&](SlabPart p, const void *src, size_t bytes, const char *what) {
+        if (staging) {
+            void *dst = static_cast<char *>(staging) + L.offset[p];
+            std::memcpy(dst, src, bytes);
+            src = dst;
+        }
         hip_check(hipMemcpyAsync(slab_part(slab, L, p), src, bytes, hipMemcpyHostToDevice, stream), what);
     };
     hip_check(hipMemsetAsync(slab, 0, L.total, stream), "hipMemsetAsync (slab)");

@@ -41,7 +41,12 @@ SlabLayout slab_layout(const HostSystem &s);
 // Zeroes the slab, uploads positions, partners, lattice, tables and colour
 // segments on `stream` (asynchronously: the HostSystem must stay alive until
 // the stream has passed this point) and returns the kernels' view of it.
-SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, uint64_t seed, hipStream_t stream);
+// Clears the slab and uploads the system's arrays on `stream`.  With a pinned
+// `staging` buffer of L.total bytes the host arrays are first copied into it
+// (at their slab offsets), so that the uploads never wait for the stream;
+// the caller keeps the buffer until the uploads have run.
+SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, uint64_t seed, hipStream_t stream,
+                         void *staging = nullptr);
 
 // Pointer to one part of the slab.
 inline void *slab_part(void *slab, const SlabLayout &L, SlabPart p)

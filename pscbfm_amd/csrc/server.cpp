@@ -68,8 +68,28 @@ uint64_t Server::submit(const HostSystem &sys, uint64_t seed, int64_t age, uint3
     return id;
 }
 
+void Server::collect(bool block)
+{
+    std::vector<uint64_t> left;
+    for (size_t k = 0; k < mRetiring.size(); ++k) {
+        Job &j = job(mRetiring[k]);
+        if (block && k == 0)
+            mEngine.wait(j.download);
+        else if (!mEngine.finished(j.download)) {
+            left.push_back(mRetiring[k]);
+            continue;
+        }
+        mEngine.unpack(j.download);
+        j.sys.xyz = mEngine.system(j.member).xyz;
+        j.state = kDone;
+        j.t_finish = now();
+    }
+    mRetiring.swap(left);
+}
+
 uint32_t Server::step()
 {
+    collect(false);
     // admission in arrival order while the budget allows (no overtaking, so a
     // large job is not starved by small ones behind it)
     while (!mQueue.empty()) {
@@ -89,8 +109,13 @@ uint32_t Server::step()
         }
         mQueue.pop_front();
     }
-    if (mRunning.empty())
-        return uint32_t(mQueue.size());
+    if (mRunning.empty()) {
+        // nothing to run (so nothing to admit): wait for the results instead
+        // of returning to a caller that would spin
+        if (!mRetiring.empty())
+            collect(true);
+        return pending();
+    }
 
     // jobs with less than a chunk left run their remainder first
     std::vector<uint64_t> full;
@@ -116,19 +141,18 @@ uint32_t Server::step()
         j.done += ran;
         mAttempts += uint64_t(ran) * j.sys.n;
         if (j.done >= j.n_mcs) {
-            mEngine.copyBack(j.member);
-            j.sys.xyz = mEngine.system(j.member).xyz;
+            j.download = mEngine.download(j.member);
             j.age = mEngine.age(j.member);
-            mEngine.remove(j.member);
+            mEngine.remove(j.member);   // freed after the download, in stream order
             mInUse -= j.bytes;
-            j.state = kDone;
-            j.t_finish = now();
+            mRetiring.push_back(id);
         } else {
             still.push_back(id);
         }
     }
     mRunning.swap(still);
-    return uint32_t(mQueue.size() + mRunning.size());
+    collect(false);
+    return pending();
 }
 
 void Server::cancel(uint64_t id)
@@ -137,6 +161,8 @@ void Server::cancel(uint64_t id)
     if (j.state == kQueued) {
         mQueue.erase(std::find(mQueue.begin(), mQueue.end(), id));
     } else if (j.state == kRunning) {
+        if (std::find(mRetiring.begin(), mRetiring.end(), id) != mRetiring.end())
+            return;   // finished on the device; its result is on its way
         mEngine.remove(j.member);
         mInUse -= j.bytes;
         mRunning.erase(std::find(mRunning.begin(), mRunning.end(), id));

@@ -4,9 +4,12 @@
 // arrival order.  Each scheduling round admits queued jobs while their device
 // bytes fit the HBM budget, runs one chunk of MCS on every admitted job in one
 // batched launch sequence (their work overlaps on the device), and retires the
-// jobs that are done: their positions are copied back and their device memory
-// is freed, which lets the next round admit more.  A job's state stays
-// resident on the device between chunks.  A job's result is bit-identical to
+// jobs that are done: the copy of their positions into pinned memory is queued
+// behind their last chunk and their device memory is freed in stream order,
+// which lets the next round admit more.  No round waits for the device unless
+// nothing else can proceed: a retiring job is reported done by the first round
+// that finds its copy arrived.  A job's state stays resident on the device
+// between chunks.  A job's result is bit-identical to
 // running it alone (its trajectory depends only on its seed and age).
 #pragma once
 
@@ -42,7 +45,8 @@ public:
     void times(uint64_t job, double &submitted, double &admitted, double &finished) const;
 
     uint32_t queued() const { return uint32_t(mQueue.size()); }
-    uint32_t running() const { return uint32_t(mRunning.size()); }
+    // running jobs, including those whose result is still on its way
+    uint32_t running() const { return uint32_t(mRunning.size() + mRetiring.size()); }
     uint64_t budget() const { return mBudget; }
     uint64_t bytesInUse() const { return mInUse; }
     uint64_t attemptsDone() const { return mAttempts; }
@@ -58,11 +62,16 @@ private:
         uint64_t bytes = 0;
         State state = kQueued;
         uint32_t member = 0;
+        BatchEngine::Download download{};   // while retiring
         std::string error;
         double t_submit = 0, t_admit = -1, t_finish = -1;
     };
 
     double now() const;
+    // Reports the retiring jobs whose positions have arrived; with `block`,
+    // first waits for the oldest one.
+    void collect(bool block);
+    uint32_t pending() const { return uint32_t(mQueue.size() + mRunning.size() + mRetiring.size()); }
     Job &job(uint64_t id);
     const Job &job(uint64_t id) const;
 
@@ -75,6 +84,7 @@ private:
     std::map<uint64_t, std::unique_ptr<Job>> mJobs;
     std::deque<uint64_t> mQueue;
     std::vector<uint64_t> mRunning;
+    std::vector<uint64_t> mRetiring;   // done on the device, positions on their way
     std::chrono::steady_clock::time_point mT0;
 };
 
