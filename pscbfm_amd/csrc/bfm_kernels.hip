@@ -156,13 +156,13 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
         const int4 r1 = moved(r0, dir);
 
         bool ok = true;
-        if (!kAllPeriodic) {
-            const int a = dir >> 1;
-            const int32_t v = comp(r1, a);
-            const int32_t hi = a == 0 ? g.wall_hi[0] : (a == 1 ? g.wall_hi[1] : g.wall_hi[2]);
-            if (!((g.periodic >> a) & 1u) && (v < 0 || v > hi))
-                ok = false;
-        }
+        // walls: every axis against its walls (the axes not moved along are
+        // inside already), so that no per-lane index reaches the buffers struct
+        if (!kAllPeriodic &&
+            ((!(g.periodic & 1u) && (r1.x < 0 || r1.x > g.wall_hi[0])) ||
+             (!(g.periodic & 2u) && (r1.y < 0 || r1.y > g.wall_hi[1])) ||
+             (!(g.periodic & 4u) && (r1.z < 0 || r1.z > g.wall_hi[2]))))
+            ok = false;
 
         // excluded volume: no corner in the 9-site front
         const Front f = front_sites(g, dep, r0, dir);
@@ -378,16 +378,18 @@ __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__res
     const SchedEntry e = sched[size_t(slot) * n_sys + s];
     if (e.colour == kNoColour)
         return;
-    const BatchSystem &S = sys[s];
     const uint32_t nb = check_blocks(e.count);
-    const uint32_t blk = lb - S.check_base;
+    const uint32_t blk = lb - sys[s].check_base;
     if (blk >= nb)
         return;
-    load_bond(S.buf.tables);
+    // the system's buffers by value: the body's stores cannot alias them, so
+    // they stay in registers as a plain launch's arguments do
+    const SweepBuffers buf = sys[s].buf;
+    load_bond(buf.tables);
     if (!kCubic)
-        load_dep(S.buf.tables);
+        load_dep(buf.tables);
     __syncthreads();
-    check_body<kAllPeriodic, kCubic>(S.buf, e.begin, e.count, e.age, e.sub_seed, e.parity, blk, nb);
+    check_body<kAllPeriodic, kCubic>(buf, e.begin, e.count, e.age, e.sub_seed, e.parity, blk, nb);
 }
 
 template <bool kCubic>
@@ -401,16 +403,16 @@ __global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__r
     const SchedEntry e = sched[size_t(slot) * n_sys + s];
     if (e.colour == kNoColour)
         return;
-    const BatchSystem &S = sys[s];
     const uint32_t cb = check_blocks(e.count);
-    const uint32_t blk = lb - S.perform_base;
+    const uint32_t blk = lb - sys[s].perform_base;
     if (blk >= perform_blocks(cb))
         return;
+    const SweepBuffers buf = sys[s].buf;
     if (!kCubic) {
-        load_dep(S.buf.tables);
+        load_dep(buf.tables);
         __syncthreads();
     }
-    perform_body<kCubic>(S.buf, cb, e.parity, blk);
+    perform_body<kCubic>(buf, cb, e.parity, blk);
 }
 
 // Plain streaming copy, 16 bytes per lane per iteration: the reference point
