@@ -276,6 +276,9 @@ int bfm_batch_get_age(bfm_batch *b, uint32_t member, int64_t *age);
 /* attempts = 0 here (the batch does not track them); accepted and proposals
  * are the member's device counters. */
 int bfm_batch_get_stats(bfm_batch *b, uint32_t member, bfm_stats *out);
+/* bfm_observe for a member (its resident state; waits for the member's queued
+ * work). */
+int bfm_batch_observe(bfm_batch *b, uint32_t member, bfm_observables *out);
 int bfm_batch_remove(bfm_batch *b, uint32_t member);
 int bfm_batch_set_graphs(bfm_batch *b, int on);
 /* As bfm_set_xcd_ranges, over the batch's whole grid (each member's blocks

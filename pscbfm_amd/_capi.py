@@ -155,6 +155,7 @@ SIGNATURES = {
     "bfm_batch_get_positions": (c_int, [_H, c_uint32, c_void_p, c_uint32]),
     "bfm_batch_get_age": (c_int, [_H, c_uint32, POINTER(c_int64)]),
     "bfm_batch_get_stats": (c_int, [_H, c_uint32, POINTER(BfmStats)]),
+    "bfm_batch_observe": (c_int, [_H, c_uint32, POINTER(BfmObservables)]),
     "bfm_batch_remove": (c_int, [_H, c_uint32]),
     "bfm_batch_set_graphs": (c_int, [_H, c_int]),
     "bfm_batch_set_xcd_ranges": (c_int, [_H, c_int]),

@@ -56,6 +56,14 @@ class Batch:
         _capi.check(self._lib.bfm_batch_get_positions(self._h, member, out.ctypes.data, n))
         return out
 
+    def observe(self, member: int) -> dict:
+        """Device observables of a member's resident state (as
+        UpdaterGPUScBFM_AB_Type.observe)."""
+        from .updater import observables_dict
+        o = _capi.BfmObservables()
+        _capi.check(self._lib.bfm_batch_observe(self._h, member, ctypes.byref(o)))
+        return observables_dict(o)
+
     def age(self, member: int) -> int:
         v = c_int64()
         _capi.check(self._lib.bfm_batch_get_age(self._h, member, byref(v)))

@@ -181,8 +181,20 @@ void BatchEngine::remove(uint32_t id)
     // stream order: launches already queued finish before the memory is freed
     release(m.slab);
     m.slab = nullptr;
+    m.analyzer.reset();
     mBytes -= m.bytes;
     mCompIds.clear();   // forces a rebuild of the tables
+}
+
+void BatchEngine::observe(uint32_t id, Observables &out)
+{
+    Member &m = *mMembers.at(id);
+    if (!m.slab)
+        throw std::invalid_argument("batch: member was removed");
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    if (!m.analyzer)
+        m.analyzer = std::make_unique<DeviceAnalyzer>();
+    m.analyzer->observe(m.sys, m.dev.buf, mStream, out);
 }
 
 void BatchEngine::synchronize()

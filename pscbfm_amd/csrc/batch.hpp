@@ -21,6 +21,7 @@
 #include <memory>
 #include <vector>
 
+#include "analyzers.hpp"
 #include "bfm_kernels.hpp"
 #include "host_system.hpp"
 
@@ -89,6 +90,10 @@ public:
     // Frees a member's device memory; its id is not reused.
     void remove(uint32_t id);
 
+    // Observables of a member's resident state (the device analyzers of a
+    // plain updater, on the batch's stream; waits for the member's work).
+    void observe(uint32_t id, Observables &out);
+
     const HostSystem &system(uint32_t id) const { return mMembers.at(id)->sys; }
     int64_t age(uint32_t id) const { return mMembers.at(id)->age; }
     bool active(uint32_t id) const { return mMembers.at(id)->slab != nullptr; }
@@ -113,6 +118,7 @@ private:
         BatchSystem dev{};
         uint32_t check_blocks_max = 0;
         uint32_t perform_blocks_max = 0;
+        std::unique_ptr<DeviceAnalyzer> analyzer;   // made by the first observe()
     };
 
     void rebuildTables(const std::vector<uint32_t> &ids);

@@ -389,6 +389,21 @@ int bfm_set_xcd_ranges(bfm_updater *h, int on)
     return guarded([&] { h->u.setXcdRanges(on != 0); });
 }
 
+namespace {
+void export_observables(const pscbfm::Observables &o, bfm_observables *out)
+{
+    std::memset(out, 0, sizeof(*out));
+    out->bonds = o.bonds;
+    std::memcpy(out->bond_hist, o.hist, sizeof(out->bond_hist));
+    out->n_chains = o.n_chains;
+    out->re2_sum = o.re2_sum;
+    out->re2 = o.re2;
+    out->rg2 = o.rg2;
+    for (int a = 0; a < 3; ++a)
+        out->box_com[a] = o.box_com[a];
+}
+}  // namespace
+
 int bfm_observe(bfm_updater *h, bfm_observables *out)
 {
     NEED_HANDLE(h);
@@ -397,15 +412,7 @@ int bfm_observe(bfm_updater *h, bfm_observables *out)
     return guarded([&] {
         pscbfm::Observables o;
         h->u.observe(o);
-        std::memset(out, 0, sizeof(*out));
-        out->bonds = o.bonds;
-        std::memcpy(out->bond_hist, o.hist, sizeof(out->bond_hist));
-        out->n_chains = o.n_chains;
-        out->re2_sum = o.re2_sum;
-        out->re2 = o.re2;
-        out->rg2 = o.rg2;
-        for (int a = 0; a < 3; ++a)
-            out->box_com[a] = o.box_com[a];
+        export_observables(o, out);
     });
 }
 
@@ -585,6 +592,19 @@ int bfm_batch_get_age(bfm_batch *b, uint32_t member, int64_t *age)
         if (member >= b->e->size())
             throw std::invalid_argument("batch: no such member");
         *age = b->e->age(member);
+    });
+}
+
+int bfm_batch_observe(bfm_batch *b, uint32_t member, bfm_observables *out)
+{
+    NEED(b, "batch handle");
+    NEED(out, "output pointer");
+    return guarded([&] {
+        if (member >= b->e->size())
+            throw std::invalid_argument("batch: no such member");
+        pscbfm::Observables o;
+        b->e->observe(member, o);
+        export_observables(o, out);
     });
 }
 

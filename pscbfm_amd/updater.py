@@ -167,15 +167,7 @@ class UpdaterGPUScBFM_AB_Type:
         centre of mass of the resident state, reduced on the device."""
         o = _capi.BfmObservables()
         _capi.check(self._lib.bfm_observe(self._h, byref(o)))
-        return {
-            "bonds": o.bonds,
-            "bond_hist": np.ctypeslib.as_array(o.bond_hist).astype(np.uint64).copy(),
-            "n_chains": o.n_chains,
-            "re2_sum": o.re2_sum,
-            "re2": o.re2,
-            "rg2": o.rg2,
-            "box_com": np.array(o.box_com[:]),
-        }
+        return observables_dict(o)
 
     def chainTable(self) -> dict:
         """First, middle and last monomer of every linear chain, as the
@@ -217,6 +209,18 @@ class UpdaterGPUScBFM_AB_Type:
     def writeMsd(self, path: str) -> None:
         """AnalyzerMsd::cleanup's output file."""
         _capi.check(self._lib.bfm_msd_write(self._h, str(path).encode()))
+
+
+def observables_dict(o: "_capi.BfmObservables") -> dict:
+    return {
+        "bonds": o.bonds,
+        "bond_hist": np.ctypeslib.as_array(o.bond_hist).astype(np.uint64).copy(),
+        "n_chains": o.n_chains,
+        "re2_sum": o.re2_sum,
+        "re2": o.re2,
+        "rg2": o.rg2,
+        "box_com": np.array(o.box_com[:]),
+    }
 
 
 def updater_from_system(system: dict, seed: int, gpu: int | None = 0, age: int = 0,

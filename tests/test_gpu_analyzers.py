@@ -157,3 +157,28 @@ def test_observables_full_size_large_melt(gpu_ready):
     com = c.mean(axis=1, keepdims=True)
     rg2 = float(((c - com) ** 2).sum(axis=2).mean())
     assert math.isclose(got["rg2"], rg2, rel_tol=RTOL)
+
+
+def test_batch_member_observables_equal_plain(gpu_ready):
+    """A batch member's device observables are those of the same system run
+    alone: same state (the trajectories are bit-exact), same kernels, same
+    reduction order, so every field is equal, floating point included."""
+    from pscbfm_amd import Batch
+    cases = [(systems.linear_melt(32, 128, 16, seed=31), 0x5EED0000000000C1),
+             (systems.diamond_network(64, (2, 2, 8), 32, seed=41), 0x5EED0000000000C2),
+             (systems.linear_melt(64, 300, 32, seed=33), 0x5EED0000000000C3)]
+    print(f"case: batch of {len(cases)} members, 40 MCS", flush=True)
+    b = Batch(0)
+    ids = [b.add(updater_from_system(s, seed=seed, validate_only=True), len(s["xyz"])) for s, seed in cases]
+    b.run(40, 20)
+    for (s, seed), m in zip(cases, ids):
+        u = updater_from_system(s, seed=seed, gpu=0)
+        u.runResident(40)
+        want, got = u.observe(), b.observe(m)
+        u.cleanup()
+        np.testing.assert_array_equal(got["bond_hist"], want["bond_hist"])
+        for key in ("bonds", "n_chains", "re2_sum"):
+            assert got[key] == want[key], key
+        for key in ("re2", "rg2"):
+            assert (math.isnan(got[key]) and math.isnan(want[key])) or got[key] == want[key], key
+        np.testing.assert_array_equal(got["box_com"], want["box_com"])
