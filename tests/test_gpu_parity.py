@@ -127,6 +127,53 @@ def non_cubic_box_system():
     return big
 
 
+def tiled_system(box, periodic, axis, seed):
+    """16^3 melts (coordinates within [0, 14]) tiled along one axis of a
+    larger box, every tile its own seed."""
+    parts, bonds, off = [], [], 0
+    forbidden = None
+    for k in range(box[axis] // 16):
+        s = systems.linear_melt(16, 16, 16, seed=seed + k)
+        shift = np.zeros(3, dtype=np.int32)
+        shift[axis] = 16 * k
+        parts.append(s["xyz"] + shift)
+        bonds.append(s["bonds"] + off)
+        off += len(s["xyz"])
+        forbidden = s["forbidden"]
+    return {"box": box, "periodic": periodic, "xyz": np.concatenate(parts), "bonds": np.concatenate(bonds),
+            "forbidden": forbidden}
+
+
+def test_largest_box_edge_periodic(gpu_ready):
+    # 1024 sites along x, the largest edge (10-bit packed coordinates): chains
+    # wrap from 1023 to 0 and their periodic images change
+    s = tiled_system((1024, 16, 16), (1, 1, 1), 0, seed=61)
+    st = assert_parity(s, seed=0x5EED0000000000D1, chunks=[150, 150])
+    assert st["accepted"] > 0
+
+
+def test_largest_box_edge_walls(gpu_ready):
+    # walls on y with corners up to 1022 = L - 2 (the last legal corner)
+    s = tiled_system((16, 1024, 16), (1, 0, 1), 1, seed=71)
+    assert s["xyz"][:, 1].max() >= 1008   # the last tile touches the far wall
+    assert_parity(s, seed=0x5EED0000000000D2, chunks=[300])
+
+
+def test_smallest_box_and_unbonded_monomers(gpu_ready):
+    # one monomer in the smallest box (8^3), then a gas of monomers without
+    # bonds (no partner columns at all)
+    forbidden = systems.standard_forbidden()
+    one = {"box": (8, 8, 8), "periodic": (1, 1, 1), "xyz": np.array([[3, 4, 5]], dtype=np.int32),
+           "bonds": np.zeros((0, 2), dtype=np.int64), "forbidden": forbidden}
+    assert_parity(one, seed=0x5EED0000000000D3, chunks=[500])
+    grid = np.array([(x, y, z) for x in range(0, 32, 4) for y in range(0, 32, 4) for z in range(0, 32, 4)],
+                    dtype=np.int32)
+    gas = {"box": (32, 32, 32), "periodic": (1, 1, 1), "xyz": grid, "bonds": np.zeros((0, 2), dtype=np.int64),
+           "forbidden": forbidden}
+    st = assert_parity(gas, seed=0x5EED0000000000D4, chunks=[200])
+    assert st["accepted"] > 0
+
+
 def test_non_cubic_box(gpu_ready):
     # compact Morton order for a 64 x 16 x 32 box
     assert_parity(non_cubic_box_system(), seed=32, chunks=[250])
