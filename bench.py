@@ -172,6 +172,7 @@ def served_leg(args, capacity: float) -> dict:
                 if nxt == len(jobs):
                     break
                 time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+        graphs = srv.graph_stats()
         lat = [finish[j] - arrivals[j] for j in range(len(jobs))]
         makespan = max(finish) - arrivals[0]
         exact = 0
@@ -183,9 +184,10 @@ def served_leg(args, capacity: float) -> dict:
         lat = np.array(lat)
         pt = {"load_factor": factor, "rate_jobs_per_s": round(rate, 2), "jobs": len(jobs),
               "throughput": float(work.sum() / makespan), "p50_latency_s": float(np.percentile(lat, 50)),
-              "p99_latency_s": float(np.percentile(lat, 99)), "bit_exact_sampled": f"{exact}/{len(range(0, len(jobs), max(1, len(jobs) // 8)))}"}
+              "p99_latency_s": float(np.percentile(lat, 99)), "graphs": graphs, "bit_exact_sampled": f"{exact}/{len(range(0, len(jobs), max(1, len(jobs) // 8)))}"}
         log(f"served at {factor}x capacity ({rate:.1f} jobs/s): {pt['throughput']:.4g} attempts/s, "
-            f"p50 {pt['p50_latency_s']*1e3:.0f} ms, p99 {pt['p99_latency_s']*1e3:.0f} ms, exact {pt['bit_exact_sampled']}")
+            f"p50 {pt['p50_latency_s']*1e3:.0f} ms, p99 {pt['p99_latency_s']*1e3:.0f} ms, exact {pt['bit_exact_sampled']}, "
+            f"chunk graphs {graphs['launches']} ({graphs['updates']} updated, {graphs['instantiations']} instantiated)")
         points.append(pt)
     return {"unit": "attempts/s", "chunk": args.chunk, "mean_job_attempts": mean_work, "points": points}
 

@@ -312,6 +312,10 @@ int bfm_server_release(bfm_server *s, uint64_t job);
 /* Queued / running jobs, device bytes in use and budget, attempts done. */
 int bfm_server_info(bfm_server *s, uint32_t *queued, uint32_t *running, uint64_t *bytes_in_use,
                     uint64_t *budget, uint64_t *attempts);
+/* How the chunks were launched: graph launches, graphs written by an update of
+ * an existing executable graph, graphs instantiated anew (any may be NULL). */
+int bfm_server_graph_stats(bfm_server *s, uint64_t *launches, uint64_t *updates, uint64_t *instantiations);
+int bfm_batch_graph_stats(bfm_batch *b, uint64_t *launches, uint64_t *updates, uint64_t *instantiations);
 
 /* Measurement helper, no counterpart in the reference: times `repeats`
  * launches of a plain 16-byte-per-lane copy kernel over `bytes` on device

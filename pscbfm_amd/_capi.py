@@ -172,6 +172,8 @@ SIGNATURES = {
     "bfm_server_release": (c_int, [_H, c_uint64]),
     "bfm_server_info": (c_int, [_H, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint64),
                                 POINTER(c_uint64), POINTER(c_uint64)]),
+    "bfm_server_graph_stats": (c_int, [_H, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]),
+    "bfm_batch_graph_stats": (c_int, [_H, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]),
     "bfm_device_count": (c_int, [POINTER(c_int)]),
     "bfm_calibration_copy": (c_int, [c_int, c_uint64, c_int, POINTER(c_double)]),
     "bfm_gen_linear_melt": (c_int, [c_uint32, c_uint32, c_uint32, c_uint64, c_void_p, c_void_p]),

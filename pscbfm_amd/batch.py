@@ -149,3 +149,9 @@ class Server:
         _capi.check(self._lib.bfm_server_info(self._h, byref(q), byref(r), byref(used), byref(budget), byref(att)))
         return {"queued": q.value, "running": r.value, "bytes_in_use": used.value, "budget": budget.value,
                 "attempts": att.value}
+
+    def graph_stats(self) -> dict:
+        """Chunk graphs launched, updated in place, instantiated."""
+        v = [c_uint64() for _ in range(3)]
+        _capi.check(self._lib.bfm_server_graph_stats(self._h, *[byref(x) for x in v]))
+        return dict(zip(("launches", "updates", "instantiations"), (x.value for x in v)))

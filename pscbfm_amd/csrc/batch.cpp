@@ -329,19 +329,26 @@ void BatchEngine::runChunk(uint32_t mcs)
                           "batched perform kernel");
             }
     };
-    // capture a chunk only once its composition has run before: a server
-    // whose composition changes every round launches directly
-    const bool reuse = mRanVersion == mCompVersion && mRanMcs == mcs;
-    mRanVersion = mCompVersion;
-    mRanMcs = mcs;
-    const bool have = mGraph[b] && mGraphVersion[b] == mCompVersion && mGraphMcs[b] == mcs &&
+    // A composition that has a graph replays it.  A new composition whose
+    // launch sequence has the same shape (MCS, colour slots, kernel classes)
+    // as the graph of this buffer is captured and written into that graph by
+    // an update (new arguments and grid sizes), far cheaper than a new
+    // instantiation; a composition running for the second time is captured
+    // and instantiated; any other chunk is launched kernel by kernel, which
+    // starts it soonest.  The graph of buffer b last ran for chunk k-2, which
+    // the wait above has seen finish.
+    uint32_t class_mask = 0;
+    for (uint32_t c = 0; c < kClasses; ++c)
+        class_mask |= mCheckBlocks[c] ? 1u << c : 0u;
+    const uint64_t shape = (uint64_t(mcs) << 32) | (uint64_t(mMaxColours) << 8) | class_mask;
+    const bool have = mGraph[b] && mGraphVersion[b] == mCompVersion && mGraphShape[b] == shape &&
                       mGraphXcd[b] == mXcdRanges;
-    if (mUseGraphs && (have || reuse)) {
+    const bool same_shape = mGraph[b] && mGraphShape[b] == shape && mGraphXcd[b] == mXcdRanges;
+    const bool reuse = mRanVersion == mCompVersion && mRanShape == shape;
+    mRanVersion = mCompVersion;
+    mRanShape = shape;
+    if (mUseGraphs && (have || same_shape || reuse)) {
         if (!have) {
-            if (mGraph[b]) {
-                hip_check(hipGraphExecDestroy(mGraph[b]), "hipGraphExecDestroy");
-                mGraph[b] = nullptr;
-            }
             hipGraph_t g = nullptr;
             hip_check(hipStreamBeginCapture(mStream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             try {
@@ -353,11 +360,30 @@ void BatchEngine::runChunk(uint32_t mcs)
                 throw;
             }
             hip_check(hipStreamEndCapture(mStream, &g), "hipStreamEndCapture");
-            hipError_t e = hipGraphInstantiate(&mGraph[b], g, nullptr, nullptr, 0);
+            bool updated = false;
+            if (mGraph[b] && mGraphShape[b] == shape) {
+                hipGraphNode_t bad = nullptr;
+                hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+                updated = hipGraphExecUpdate(mGraph[b], g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+                hipGetLastError();   // a refused update is not an error of this call
+                if (updated)
+                    ++mGraphUpdates;
+            }
+            if (!updated) {
+                if (mGraph[b]) {
+                    hip_check(hipGraphExecDestroy(mGraph[b]), "hipGraphExecDestroy");
+                    mGraph[b] = nullptr;
+                }
+                const hipError_t e = hipGraphInstantiate(&mGraph[b], g, nullptr, nullptr, 0);
+                if (e != hipSuccess) {
+                    hipGraphDestroy(g);
+                    hip_check(e, "hipGraphInstantiate");
+                }
+                ++mGraphInstantiations;
+            }
             hipGraphDestroy(g);
-            hip_check(e, "hipGraphInstantiate");
             mGraphVersion[b] = mCompVersion;
-            mGraphMcs[b] = mcs;
+            mGraphShape[b] = shape;
             mGraphXcd[b] = mXcdRanges;
         }
         hip_check(hipGraphLaunch(mGraph[b], mStream), "hipGraphLaunch");

@@ -6,9 +6,11 @@
 // over all members at once.  The host writes only where each member stands at
 // the start of the chunk (age, substeps); a small kernel at the head of the
 // chunk derives every member's colour, parity and substep seed for every slot
-// (one thread per member and MCS), which the sweep kernels read.  The launches of a
-// chunk are captured once into a hipGraph and replayed for every chunk with
-// the same composition, so the whole chunk costs one graph launch on the host.
+// (one thread per member and MCS), which the sweep kernels read.  A chunk
+// runs as one hipGraph launch once its composition has a graph: a composition
+// that ran before replays or captures one, and a new composition is written
+// into the existing graph of the same shape by an update (new arguments and
+// grid sizes) instead of a new instantiation.
 //
 // A member's trajectory depends only on its own seed and age (the RNG is keyed
 // on seed, MCS and monomer id), so it is the same alone, in any batch and in
@@ -104,6 +106,8 @@ public:
     void setUseGraphs(bool on) { mUseGraphs = on; }
     void setXcdRanges(bool on) { mXcdRanges = on; }
     uint64_t graphLaunches() const { return mGraphLaunches; }
+    uint64_t graphUpdates() const { return mGraphUpdates; }
+    uint64_t graphInstantiations() const { return mGraphInstantiations; }
     uint64_t kernelLaunches() const { return mKernelLaunches; }
     void synchronize();
 
@@ -163,14 +167,14 @@ private:
     // captured chunks, one per schedule buffer
     bool mUseGraphs = true;
     hipGraphExec_t mGraph[2] = {nullptr, nullptr};
-    uint32_t mGraphMcs[2] = {0, 0};
+    uint64_t mGraphShape[2] = {~0ull, ~0ull};   // MCS, colour slots, classes of the graph
     uint64_t mGraphVersion[2] = {~0ull, ~0ull};
     bool mGraphXcd[2] = {false, false};
     bool mXcdRanges = true;
     uint64_t mGraphLaunches = 0, mKernelLaunches = 0;
+    uint64_t mGraphUpdates = 0, mGraphInstantiations = 0;
     uint64_t mCompVersion = 0;
-    uint64_t mRanVersion = ~0ull;
-    uint32_t mRanMcs = 0;
+    uint64_t mRanVersion = ~0ull, mRanShape = ~0ull;   // the chunk before
 };
 
 }  // namespace pscbfm

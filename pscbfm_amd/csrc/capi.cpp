@@ -765,6 +765,30 @@ int bfm_server_info(bfm_server *s, uint32_t *queued, uint32_t *running, uint64_t
     });
 }
 
+namespace {
+void graph_stats(pscbfm::BatchEngine &e, uint64_t *launches, uint64_t *updates, uint64_t *instantiations)
+{
+    if (launches)
+        *launches = e.graphLaunches();
+    if (updates)
+        *updates = e.graphUpdates();
+    if (instantiations)
+        *instantiations = e.graphInstantiations();
+}
+}  // namespace
+
+int bfm_server_graph_stats(bfm_server *s, uint64_t *launches, uint64_t *updates, uint64_t *instantiations)
+{
+    NEED(s, "server handle");
+    return guarded([&] { graph_stats(s->s->engine(), launches, updates, instantiations); });
+}
+
+int bfm_batch_graph_stats(bfm_batch *b, uint64_t *launches, uint64_t *updates, uint64_t *instantiations)
+{
+    NEED(b, "batch handle");
+    return guarded([&] { graph_stats(*b->e, launches, updates, instantiations); });
+}
+
 int bfm_calibration_copy(int gpu, uint64_t bytes, int repeats, double *avg_ms)
 {
     if (!avg_ms || repeats <= 0 || bytes == 0 || bytes % 16)
