@@ -185,6 +185,15 @@ int bfm_set_kernel_timing(bfm_updater *h, int every);
  * is the same either way. */
 int bfm_set_xcd_ranges(bfm_updater *h, int on);
 
+/* Launch configuration (no counterpart in the reference; the trajectory is the
+ * same under every one).  perform group: check waves whose survivors one
+ * perform wave takes, 2, 4 (default) or 8.  bfm_autotune runs probe_mcs MCS
+ * under each of the 6 configurations (perform group x XCD placement), timed
+ * with HIP events, keeps the fastest and reports it; the system advances by
+ * 6 * probe_mcs MCS meanwhile.  After initialize. */
+int bfm_set_perform_group(bfm_updater *h, uint32_t group);
+int bfm_autotune(bfm_updater *h, uint32_t probe_mcs, uint32_t *group, int *xcd_ranges);
+
 /* Runs n_mcs steps without the copy-back of runSimulationOnGPU (positions
  * stay on the device; the age still advances).  For timing loops. */
 int bfm_run_resident(bfm_updater *h, uint32_t n_mcs);
@@ -284,6 +293,10 @@ int bfm_batch_set_graphs(bfm_batch *b, int on);
 /* As bfm_set_xcd_ranges, over the batch's whole grid (each member's blocks
  * stay on one or two XCDs). */
 int bfm_batch_set_xcd_ranges(bfm_batch *b, int on);
+/* As bfm_set_perform_group / bfm_autotune for the whole batch (probes run in
+ * chunks of `chunk` MCS on every active member). */
+int bfm_batch_set_perform_group(bfm_batch *b, uint32_t group);
+int bfm_batch_autotune(bfm_batch *b, uint32_t probe_mcs, uint32_t chunk, uint32_t *group, int *xcd_ranges);
 int bfm_batch_synchronize(bfm_batch *b);
 
 /* ---- served form: continuous batching of simulation jobs --------------------

@@ -134,6 +134,10 @@ SIGNATURES = {
     "bfm_get_stats": (c_int, [_H, POINTER(BfmStats)]),
     "bfm_set_kernel_timing": (c_int, [_H, c_int]),
     "bfm_set_spatial_sorting": (c_int, [_H, c_int]),
+    "bfm_set_perform_group": (c_int, [_H, c_uint32]),
+    "bfm_autotune": (c_int, [_H, c_uint32, POINTER(c_uint32), POINTER(c_int)]),
+    "bfm_batch_set_perform_group": (c_int, [_H, c_uint32]),
+    "bfm_batch_autotune": (c_int, [_H, c_uint32, c_uint32, POINTER(c_uint32), POINTER(c_int)]),
     "bfm_read_bfm_file": (c_int, [_H, c_char_p]),
     "bfm_write_bfm_file": (c_int, [_H, c_char_p, c_int]),
     "bfm_set_xcd_ranges": (c_int, [_H, c_int]),

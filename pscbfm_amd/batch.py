@@ -56,6 +56,16 @@ class Batch:
         _capi.check(self._lib.bfm_batch_get_positions(self._h, member, out.ctypes.data, n))
         return out
 
+    def set_perform_group(self, group: int) -> None:
+        _capi.check(self._lib.bfm_batch_set_perform_group(self._h, int(group)))
+
+    def autotune(self, probe_mcs: int, chunk: int = 100) -> dict:
+        """Times probe_mcs MCS of every member under each launch configuration
+        and keeps the fastest (members advance by 6 * probe_mcs MCS)."""
+        g, x = c_uint32(), ctypes.c_int()
+        _capi.check(self._lib.bfm_batch_autotune(self._h, int(probe_mcs), int(chunk), byref(g), byref(x)))
+        return {"perform_group": g.value, "xcd_ranges": bool(x.value)}
+
     def observe(self, member: int) -> dict:
         """Device observables of a member's resident state (as
         UpdaterGPUScBFM_AB_Type.observe)."""

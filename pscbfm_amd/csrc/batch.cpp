@@ -167,7 +167,6 @@ uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age)
     m->dev.seg_count = static_cast<const uint32_t *>(slab_part(m->slab, L, kPartSegCount));
     m->dev.n_colours = m->sys.n_colours;
     m->check_blocks_max = check_blocks(std::max<uint32_t>(m->sys.max_segment, 1));
-    m->perform_blocks_max = perform_blocks(m->check_blocks_max);
     mBytes += L.total;
     mMembers.push_back(std::move(m));
     return uint32_t(mMembers.size() - 1);
@@ -195,6 +194,55 @@ void BatchEngine::observe(uint32_t id, Observables &out)
     if (!m.analyzer)
         m.analyzer = std::make_unique<DeviceAnalyzer>();
     m.analyzer->observe(m.sys, m.dev.buf, mStream, out);
+}
+
+void BatchEngine::setPerformGroup(uint32_t group)
+{
+    if (!valid_perform_group(group))
+        throw std::invalid_argument("perform group must be 2, 4 or 8");
+    if (group != mPerformGroup) {
+        mPerformGroup = group;
+        mCompIds.clear();   // the perform block tables depend on it
+    }
+}
+
+void BatchEngine::autotune(uint32_t probe_mcs, uint32_t chunk)
+{
+    if (probe_mcs == 0 || chunk == 0)
+        throw std::invalid_argument("autotune: probe_mcs and chunk must be positive");
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    float best = -1.0f;
+    uint32_t best_group = mPerformGroup;
+    bool best_xcd = mXcdRanges;
+    try {
+        for (const bool xcd : {true, false})
+            for (const uint32_t g : kPerformGroups) {
+                setXcdRanges(xcd);
+                setPerformGroup(g);
+                hip_check(hipEventRecord(e0, mStream), "hipEventRecord");
+                run(probe_mcs, chunk);
+                hip_check(hipEventRecord(e1, mStream), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0.0f;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                if (best < 0.0f || ms < best) {
+                    best = ms;
+                    best_group = g;
+                    best_xcd = xcd;
+                }
+            }
+    } catch (...) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        throw;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    setPerformGroup(best_group);
+    setXcdRanges(best_xcd);
 }
 
 void BatchEngine::synchronize()
@@ -237,7 +285,7 @@ void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
         d.check_base = uint32_t(cblk[cls].size());
         d.perform_base = uint32_t(pblk[cls].size());
         cblk[cls].insert(cblk[cls].end(), m.check_blocks_max, j);
-        pblk[cls].insert(pblk[cls].end(), m.perform_blocks_max, j);
+        pblk[cls].insert(pblk[cls].end(), perform_blocks(m.check_blocks_max, mPerformGroup), j);
         sys.push_back(d);
         mMaxColours = std::max(mMaxColours, m.sys.n_colours);
     }
@@ -325,7 +373,7 @@ void BatchEngine::runChunk(uint32_t mcs)
                                              mXcdRanges ? 1u : 0u, mStream),
                           "batched check kernel");
                 hip_check(launch_perform_batch(c, mDevSys, mDevPerformBlk[c], mPerformBlocks[c], mDevSched[b], n_sys,
-                                               slot, mXcdRanges ? 1u : 0u, mStream),
+                                               slot, mXcdRanges ? 1u : 0u, mStream, mPerformGroup),
                           "batched perform kernel");
             }
     };
@@ -340,7 +388,8 @@ void BatchEngine::runChunk(uint32_t mcs)
     uint32_t class_mask = 0;
     for (uint32_t c = 0; c < kClasses; ++c)
         class_mask |= mCheckBlocks[c] ? 1u << c : 0u;
-    const uint64_t shape = (uint64_t(mcs) << 32) | (uint64_t(mMaxColours) << 8) | class_mask;
+    const uint64_t shape = (uint64_t(mcs) << 32) | (uint64_t(mPerformGroup) << 16) | (uint64_t(mMaxColours) << 8) |
+                           class_mask;
     const bool have = mGraph[b] && mGraphVersion[b] == mCompVersion && mGraphShape[b] == shape &&
                       mGraphXcd[b] == mXcdRanges;
     const bool same_shape = mGraph[b] && mGraphShape[b] == shape && mGraphXcd[b] == mXcdRanges;

@@ -105,6 +105,15 @@ public:
     hipStream_t stream() const { return mStream; }
     void setUseGraphs(bool on) { mUseGraphs = on; }
     void setXcdRanges(bool on) { mXcdRanges = on; }
+    bool xcdRanges() const { return mXcdRanges; }
+    // check waves per perform wave (2, 4 or 8); takes effect at the next chunk
+    void setPerformGroup(uint32_t group);
+    uint32_t performGroup() const { return mPerformGroup; }
+    // Times probe_mcs MCS (in chunks of `chunk`) on every active member under
+    // every launch configuration (perform group x XCD placement) and keeps the
+    // fastest.  The members run on meanwhile (6 * probe_mcs MCS each); their
+    // trajectories are the same under every configuration.
+    void autotune(uint32_t probe_mcs, uint32_t chunk);
     uint64_t graphLaunches() const { return mGraphLaunches; }
     uint64_t graphUpdates() const { return mGraphUpdates; }
     uint64_t graphInstantiations() const { return mGraphInstantiations; }
@@ -121,7 +130,6 @@ private:
         size_t bytes = 0;
         BatchSystem dev{};
         uint32_t check_blocks_max = 0;
-        uint32_t perform_blocks_max = 0;
         std::unique_ptr<DeviceAnalyzer> analyzer;   // made by the first observe()
     };
 
@@ -171,6 +179,7 @@ private:
     uint64_t mGraphVersion[2] = {~0ull, ~0ull};
     bool mGraphXcd[2] = {false, false};
     bool mXcdRanges = true;
+    uint32_t mPerformGroup = kPerformGroup;
     uint64_t mGraphLaunches = 0, mKernelLaunches = 0;
     uint64_t mGraphUpdates = 0, mGraphInstantiations = 0;
     uint64_t mCompVersion = 0;

@@ -226,9 +226,9 @@ __global__ __launch_bounds__(kBlock) void k_check(SweepBuffers buf, uint32_t beg
     check_body<kAllPeriodic, kCubic>(buf, begin, count, age, sub_seed, parity, blk, gridDim.x);
 }
 
-// Phase 2: every wave of perform block `blk` takes the survivors of
-// kPerformGroup consecutive check waves of a check launch of check_blocks blocks.
-template <bool kCubic>
+// Phase 2: every wave of perform block `blk` takes the survivors of kGroup
+// consecutive check waves of a check launch of check_blocks blocks.
+template <bool kCubic, uint32_t kGroup>
 __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t check_blocks_, uint32_t parity,
                                              uint32_t blk)
 {
@@ -237,7 +237,7 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t pwave = wave_uniform(blk * kWavesPerBlock + (threadIdx.x >> 6));
     const uint32_t check_waves = check_blocks_ * kWavesPerBlock;
-    const uint32_t cw0 = pwave * kPerformGroup;
+    const uint32_t cw0 = pwave * kGroup;
     const gptr<const uint32_t> claim = as_global<const uint32_t>(pick(parity, buf.claim0, buf.claim1));
     const gptr<const uint2> surv = as_global<const uint2>(buf.surv);
     const gptr<const uint32_t> scount = as_global<const uint32_t>(buf.scount);
@@ -245,28 +245,28 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
     const gptr<uint32_t> ppos = as_global(buf.ppos);
 
     // survivor counts of the group's check waves and their inclusive prefix
-    // sums, held by lanes 0..kPerformGroup-1
+    // sums, held by lanes 0..kGroup-1
     uint32_t incl = 0;
-    if (lane < kPerformGroup && cw0 + lane < check_waves)
+    if (lane < kGroup && cw0 + lane < check_waves)
         incl = scount[cw0 + lane];
 #pragma unroll
-    for (uint32_t d = 1; d < kPerformGroup; d <<= 1) {
+    for (uint32_t d = 1; d < kGroup; d <<= 1) {
         const uint32_t up = __shfl_up(incl, d);
         if (lane >= d)
             incl += up;
     }
-    uint32_t ends[kPerformGroup];
+    uint32_t ends[kGroup];
 #pragma unroll
-    for (uint32_t k = 0; k < kPerformGroup; ++k)
+    for (uint32_t k = 0; k < kGroup; ++k)
         ends[k] = wave_uniform(__shfl(incl, int(k)));
-    const uint32_t total = ends[kPerformGroup - 1];
+    const uint32_t total = ends[kGroup - 1];
 
     uint32_t acc = 0;
     for (uint32_t t = lane; t < total; t += kWave) {
         // check wave k of the group holds survivors [ends[k-1], ends[k])
         uint32_t k = 0, base = 0;
 #pragma unroll
-        for (uint32_t q = 0; q + 1 < kPerformGroup; ++q)
+        for (uint32_t q = 0; q + 1 < kGroup; ++q)
             if (t >= ends[q]) {
                 k = q + 1;
                 base = ends[q];
@@ -320,7 +320,7 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
     }
 }
 
-template <bool kCubic>
+template <bool kCubic, uint32_t kGroup>
 __global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t check_blocks_, uint32_t parity)
 {
     if (!kCubic) {
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t c
         __syncthreads();
     }
     const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
-    perform_body<kCubic>(buf, check_blocks_, parity, blk);
+    perform_body<kCubic, kGroup>(buf, check_blocks_, parity, blk);
 }
 
 // ---- many systems per launch ------------------------------------------------
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__res
     check_body<kAllPeriodic, kCubic>(buf, e.begin, e.count, e.age, e.sub_seed, e.parity, blk, nb);
 }
 
-template <bool kCubic>
+template <bool kCubic, uint32_t kGroup>
 __global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__restrict__ sys,
                                                           const uint32_t *__restrict__ blk_sys,
                                                           const SchedEntry *__restrict__ sched, uint32_t n_sys,
@@ -405,14 +405,14 @@ __global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__r
         return;
     const uint32_t cb = check_blocks(e.count);
     const uint32_t blk = lb - sys[s].perform_base;
-    if (blk >= perform_blocks(cb))
+    if (blk >= perform_blocks(cb, kGroup))
         return;
     const SweepBuffers buf = sys[s].buf;
     if (!kCubic) {
         load_dep(buf.tables);
         __syncthreads();
     }
-    perform_body<kCubic>(buf, cb, e.parity, blk);
+    perform_body<kCubic, kGroup>(buf, cb, e.parity, blk);
 }
 
 // Plain streaming copy, 16 bytes per lane per iteration: the reference point
@@ -444,13 +444,25 @@ hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count,
     return hipGetLastError();
 }
 
-hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t parity, hipStream_t stream)
+template <uint32_t G>
+void launch_perform_group(const SweepBuffers &buf, uint32_t cb, uint32_t parity, hipStream_t stream)
+{
+    const dim3 grid(perform_blocks(cb, G)), block(kBlock);
+    if (buf.geom.cubic != 0u)
+        hipLaunchKernelGGL((k_perform<true, G>), grid, block, 0, stream, buf, cb, parity);
+    else
+        hipLaunchKernelGGL((k_perform<false, G>), grid, block, 0, stream, buf, cb, parity);
+}
+
+hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t parity, hipStream_t stream,
+                          uint32_t group)
 {
     const uint32_t cb = check_blocks(count);
-    if (buf.geom.cubic != 0u)
-        hipLaunchKernelGGL((k_perform<true>), dim3(perform_blocks(cb)), dim3(kBlock), 0, stream, buf, cb, parity);
-    else
-        hipLaunchKernelGGL((k_perform<false>), dim3(perform_blocks(cb)), dim3(kBlock), 0, stream, buf, cb, parity);
+    switch (group) {
+    case 2: launch_perform_group<2>(buf, cb, parity, stream); break;
+    case 8: launch_perform_group<8>(buf, cb, parity, stream); break;
+    default: launch_perform_group<4>(buf, cb, parity, stream); break;
+    }
     return hipGetLastError();
 }
 
@@ -489,17 +501,30 @@ hipError_t launch_check_batch(uint32_t cls, const BatchSystem *sys, const uint32
     return hipGetLastError();
 }
 
-hipError_t launch_perform_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
+template <uint32_t G>
+void launch_perform_batch_group(bool cubic, dim3 grid, const BatchSystem *sys, const uint32_t *blk_sys,
                                 const SchedEntry *sched, uint32_t n_sys, uint32_t slot, uint32_t xcd_ranges,
                                 hipStream_t stream)
 {
-    const dim3 grid(n_blocks), block(kBlock);
-    if (cls == kClassPeriodicCubic || cls == kClassCubic)
-        hipLaunchKernelGGL((k_perform_batch<true>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot,
-                           xcd_ranges);
+    if (cubic)
+        hipLaunchKernelGGL((k_perform_batch<true, G>), grid, dim3(kBlock), 0, stream, sys, blk_sys, sched, n_sys,
+                           slot, xcd_ranges);
     else
-        hipLaunchKernelGGL((k_perform_batch<false>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot,
-                           xcd_ranges);
+        hipLaunchKernelGGL((k_perform_batch<false, G>), grid, dim3(kBlock), 0, stream, sys, blk_sys, sched, n_sys,
+                           slot, xcd_ranges);
+}
+
+hipError_t launch_perform_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
+                                const SchedEntry *sched, uint32_t n_sys, uint32_t slot, uint32_t xcd_ranges,
+                                hipStream_t stream, uint32_t group)
+{
+    const bool cubic = cls == kClassPeriodicCubic || cls == kClassCubic;
+    const dim3 grid(n_blocks);
+    switch (group) {
+    case 2: launch_perform_batch_group<2>(cubic, grid, sys, blk_sys, sched, n_sys, slot, xcd_ranges, stream); break;
+    case 8: launch_perform_batch_group<8>(cubic, grid, sys, blk_sys, sched, n_sys, slot, xcd_ranges, stream); break;
+    default: launch_perform_batch_group<4>(cubic, grid, sys, blk_sys, sched, n_sys, slot, xcd_ranges, stream); break;
+    }
     return hipGetLastError();
 }
 

@@ -54,7 +54,14 @@ __host__ __device__ inline T pick(uint32_t parity, T even, T odd)
 
 __host__ __device__ inline uint32_t check_blocks(uint32_t count) { return count == 0 ? 1u : (count + kBlock - 1) / kBlock; }
 // perform blocks that cover the survivors of a check launch of cb blocks
-__host__ __device__ inline uint32_t perform_blocks(uint32_t cb) { return (cb + kPerformGroup - 1) / kPerformGroup; }
+__host__ __device__ inline uint32_t perform_blocks(uint32_t cb, uint32_t group = kPerformGroup)
+{
+    return (cb + group - 1) / group;
+}
+// Perform group sizes with a compiled kernel (a launch parameter the
+// autotuners choose among); kPerformGroup is the default.
+constexpr uint32_t kPerformGroups[] = {2, 4, 8};
+inline bool valid_perform_group(uint32_t g) { return g == 2 || g == 4 || g == 8; }
 
 // Phase 1 of a substep of parity `parity` for the colour segment
 // [begin, begin+count): every monomer proposes a move and checks it against
@@ -67,7 +74,8 @@ hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count,
 
 // Phase 2: every survivor with no other moved corner in its front moves.
 // count is the colour segment size of the check launch.
-hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t parity, hipStream_t stream);
+hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t parity, hipStream_t stream,
+                          uint32_t group = kPerformGroup);
 
 // ---- many systems per launch -------------------------------------------------
 
@@ -126,7 +134,8 @@ hipError_t launch_check_batch(uint32_t cls, const BatchSystem *sys, const uint32
                               hipStream_t stream);
 hipError_t launch_perform_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
                                 const SchedEntry *sched, uint32_t n_sys, uint32_t slot, uint32_t xcd_ranges,
-                                hipStream_t stream);
+                                hipStream_t stream,
+                                uint32_t group = kPerformGroup);
 
 // Streaming copy of `bytes` (a multiple of 16) from src to dst.
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);

@@ -502,6 +502,24 @@ int bfm_msd_write(bfm_updater *h, const char *path)
     });
 }
 
+int bfm_set_perform_group(bfm_updater *h, uint32_t group)
+{
+    NEED_HANDLE(h);
+    return guarded([&] { h->u.setPerformGroup(group); });
+}
+
+int bfm_autotune(bfm_updater *h, uint32_t probe_mcs, uint32_t *group, int *xcd_ranges)
+{
+    NEED_HANDLE(h);
+    return guarded([&] {
+        h->u.autotune(probe_mcs);
+        if (group)
+            *group = h->u.performGroup();
+        if (xcd_ranges)
+            *xcd_ranges = h->u.xcdRanges() ? 1 : 0;
+    });
+}
+
 int bfm_batch_create(bfm_batch **out, int gpu)
 {
     if (!out)
@@ -643,6 +661,24 @@ int bfm_batch_set_xcd_ranges(bfm_batch *b, int on)
 {
     NEED(b, "batch handle");
     return guarded([&] { b->e->setXcdRanges(on != 0); });
+}
+
+int bfm_batch_set_perform_group(bfm_batch *b, uint32_t group)
+{
+    NEED(b, "batch handle");
+    return guarded([&] { b->e->setPerformGroup(group); });
+}
+
+int bfm_batch_autotune(bfm_batch *b, uint32_t probe_mcs, uint32_t chunk, uint32_t *group, int *xcd_ranges)
+{
+    NEED(b, "batch handle");
+    return guarded([&] {
+        b->e->autotune(probe_mcs, chunk);
+        if (group)
+            *group = b->e->performGroup();
+        if (xcd_ranges)
+            *xcd_ranges = b->e->xcdRanges() ? 1 : 0;
+    });
 }
 
 int bfm_batch_synchronize(bfm_batch *b)

@@ -163,6 +163,54 @@ void UpdaterGPUScBFM_AB_Type::setStepsBetweenSortings(int64_t steps)
     mStepsBetweenSortings = steps;
 }
 
+void UpdaterGPUScBFM_AB_Type::setPerformGroup(uint32_t group)
+{
+    if (!valid_perform_group(group))
+        throw std::invalid_argument("perform group must be 2, 4 or 8");
+    mPerformGroup = group;
+}
+
+void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
+{
+    if (!mInitialized)
+        throw StateError("autotune before initialize");
+    if (probe_mcs == 0)
+        throw std::invalid_argument("autotune: probe_mcs must be positive");
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    float best = -1.0f;
+    uint32_t best_group = mPerformGroup;
+    bool best_xcd = mXcdRanges;
+    try {
+        for (const bool xcd : {true, false})
+            for (const uint32_t g : kPerformGroups) {
+                setXcdRanges(xcd);
+                mPerformGroup = g;
+                hip_check(hipEventRecord(e0, mStream), "hipEventRecord");
+                runResident(probe_mcs);
+                hip_check(hipEventRecord(e1, mStream), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0.0f;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                if (best < 0.0f || ms < best) {
+                    best = ms;
+                    best_group = g;
+                    best_xcd = xcd;
+                }
+            }
+    } catch (...) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        throw;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    mPerformGroup = best_group;
+    setXcdRanges(best_xcd);
+}
+
 void UpdaterGPUScBFM_AB_Type::setSpatialSorting(bool on)
 {
     if (mInitialized)
@@ -250,7 +298,7 @@ void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs)
                       "check kernel");
             if (timed)
                 hip_check(hipEventRecord(ev[1], mStream), "hipEventRecord");
-            hip_check(launch_perform(mBuf, s.seg_count[c], parity, mStream), "perform kernel");
+            hip_check(launch_perform(mBuf, s.seg_count[c], parity, mStream, mPerformGroup), "perform kernel");
             if (timed) {
                 hip_check(hipEventRecord(ev[2], mStream), "hipEventRecord");
                 mEvents.insert(mEvents.end(), ev, ev + 3);

@@ -75,6 +75,15 @@ public:
     void copyBack();
     void setKernelTiming(uint32_t every) { mTimingEvery = every; }
     void setXcdRanges(bool on) { mBuf.xcd_ranges = on ? 1u : 0u; mXcdRanges = on; }
+    bool xcdRanges() const { return mXcdRanges; }
+    // check waves per perform wave (2, 4 or 8)
+    void setPerformGroup(uint32_t group);
+    uint32_t performGroup() const { return mPerformGroup; }
+    // Times probe_mcs MCS under every launch configuration (perform group x
+    // XCD placement) and keeps the fastest.  The run goes on meanwhile (the
+    // age advances by 6 * probe_mcs); the trajectory is the same under every
+    // configuration.
+    void autotune(uint32_t probe_mcs);
     const HostSystem &system() const { return mSys; }
     bool initialized() const { return mInitialized; }
     bool validated() const { return mValidated; }
@@ -125,6 +134,7 @@ private:
 
     uint32_t mTimingEvery = 0;   // time every n-th substep (0: off)
     bool mXcdRanges = true;
+    uint32_t mPerformGroup = kPerformGroup;
     int64_t mStepsBetweenSortings = 5000;   // mnStepsBetweenSortings, UpdaterGPUScBFM_AB_Type.cu:1379
     std::vector<hipEvent_t> mEventPool;          // reusable events
     mutable std::vector<hipEvent_t> mEvents;     // pending (start, mid, end) triples

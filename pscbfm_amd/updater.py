@@ -150,6 +150,17 @@ class UpdaterGPUScBFM_AB_Type:
         the trajectory is the same either way.  Before initialize."""
         _capi.check(self._lib.bfm_set_spatial_sorting(self._h, int(bool(on))))
 
+    def setPerformGroup(self, group: int) -> None:
+        """Check waves per perform wave: 2, 4 (default) or 8."""
+        _capi.check(self._lib.bfm_set_perform_group(self._h, int(group)))
+
+    def autotune(self, probe_mcs: int) -> dict:
+        """Times probe_mcs MCS under each launch configuration and keeps the
+        fastest (the system advances by 6 * probe_mcs MCS)."""
+        g, x = c_uint32(), ctypes.c_int()
+        _capi.check(self._lib.bfm_autotune(self._h, int(probe_mcs), byref(g), byref(x)))
+        return {"perform_group": g.value, "xcd_ranges": bool(x.value)}
+
     def setXcdRanges(self, on: bool) -> None:
         """Contiguous block range per XCD (default) or round-robin placement;
         the trajectory is the same either way."""

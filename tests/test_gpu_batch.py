@@ -117,3 +117,23 @@ def test_server_cancel_and_oversized(gpu_ready):
     j = small.submit(updater_from_system(s, seed=seed, age=age, validate_only=True), 100, len(s["xyz"]))
     assert small.state(j)[0] == JOB_FAILED
     assert "budget" in small.error(j)
+
+
+def test_batch_autotune_keeps_the_trajectories(gpu_ready):
+    """Autotuning runs the members under every launch configuration; each one
+    ends where it would alone after the same number of MCS."""
+    cases = [(systems.linear_melt(32, 128, 16, seed=85), 0x5EED0000000000F4),
+             (systems.diamond_network(64, (2, 2, 8), 32, seed=41), 0x5EED0000000000F5),
+             (systems.linear_melt(64, 400, 32, seed=86), 0x5EED0000000000F6)]
+    print(f"case: batch autotune, {len(cases)} members", flush=True)
+    b = Batch(0)
+    ids = [b.add(updater_from_system(s, seed=seed, validate_only=True), len(s["xyz"])) for s, seed in cases]
+    cfg = b.autotune(5, 5)
+    assert cfg["perform_group"] in (2, 4, 8)
+    b.run(20, 10)
+    for (s, seed), m in zip(cases, ids):
+        assert b.age(m) == 50
+        u = updater_from_system(s, seed=seed, gpu=0)
+        u.runSimulationOnGPU(50)
+        assert np.array_equal(b.positions(m), u.getMonomerPositions(len(s["xyz"])))
+        u.cleanup()

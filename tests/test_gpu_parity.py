@@ -14,14 +14,17 @@ from pscbfm_amd import systems, updater_from_system
 pytestmark = pytest.mark.gpu
 
 
-def run_pair(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True, rng=orc.RNG_INT_HASH, sort_every=None):
+def run_pair(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True, rng=orc.RNG_INT_HASH, sort_every=None,
+             perform_group=None):
     n = len(system["xyz"])
     print(f"case: n={n} box={system['box']} periodic={system.get('periodic')} seed={seed:#x} "
           f"chunks={chunks} age0={age0} xcd_ranges={xcd_ranges} spatial={spatial} rng={rng} "
-          f"sort_every={sort_every}", flush=True)
+          f"sort_every={sort_every} perform_group={perform_group}", flush=True)
     u = updater_from_system(system, seed=seed, gpu=0, age=age0, spatial_sorting=spatial, rng=rng,
                             steps_between_sortings=sort_every)
     u.setXcdRanges(xcd_ranges)
+    if perform_group is not None:
+        u.setPerformGroup(perform_group)
     for c in chunks:
         u.runSimulationOnGPU(c)
     got = u.getMonomerPositions(n)
@@ -34,8 +37,9 @@ def run_pair(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True, rng=or
 
 
 def assert_parity(system, seed, chunks, age0=0, xcd_ranges=True, spatial=True, rng=orc.RNG_INT_HASH,
-                  sort_every=None):
-    got, want, stats, acc, prop, osys = run_pair(system, seed, chunks, age0, xcd_ranges, spatial, rng, sort_every)
+                  sort_every=None, perform_group=None):
+    got, want, stats, acc, prop, osys = run_pair(system, seed, chunks, age0, xcd_ranges, spatial, rng, sort_every,
+                                                 perform_group)
     diff = np.nonzero((got != want).any(axis=1))[0]
     assert diff.size == 0, f"{diff.size} monomers differ, first {diff[:5]}: gpu {got[diff[:3]]} oracle {want[diff[:3]]}"
     assert stats["accepted"] == acc
@@ -172,6 +176,37 @@ def test_smallest_box_and_unbonded_monomers(gpu_ready):
            "forbidden": forbidden}
     st = assert_parity(gas, seed=0x5EED0000000000D4, chunks=[200])
     assert st["accepted"] > 0
+
+
+@pytest.mark.parametrize("group", [2, 8])
+def test_perform_groups(gpu_ready, group):
+    # check waves per perform wave: a launch parameter, never the trajectory
+    s = systems.linear_melt(32, 128, 16, seed=81)
+    assert_parity(s, seed=0x5EED0000000000F1, chunks=[150], perform_group=group)
+    assert_parity(diamond_small(), seed=0x5EED0000000000F2, chunks=[60], perform_group=group)
+
+
+def diamond_small():
+    return systems.diamond_network(64, (2, 2, 8), 32, seed=41)
+
+
+def test_autotune_keeps_the_trajectory(gpu_ready):
+    s = systems.linear_melt(64, 512, 32, seed=83)
+    n = len(s["xyz"])
+    seed = 0x5EED0000000000F3
+    print(f"case: autotune n={n}", flush=True)
+    u = updater_from_system(s, seed=seed, gpu=0)
+    cfg = u.autotune(5)                 # 6 configurations x 5 MCS
+    assert cfg["perform_group"] in (2, 4, 8)
+    assert u.getAge() == 30
+    u.runSimulationOnGPU(70)
+    got = u.getMonomerPositions(n)
+    colors, _ = u.getColors(n)
+    st = u.getStats()
+    u.cleanup()
+    want, acc, prop = orc.sweep(orc.System(s, seed, colors), s["xyz"], 0, 100, 0)
+    assert np.array_equal(got, want)
+    assert (st["accepted"], st["proposals"]) == (acc, prop)
 
 
 def test_non_cubic_box(gpu_ready):

@@ -209,3 +209,13 @@ def test_device_helpers_on_host():
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.startswith("ok ")
+
+
+def test_perform_group_must_be_compiled():
+    from pscbfm_amd import _capi, systems, updater_from_system
+    u = updater_from_system(systems.linear_melt(16, 8, 8, seed=1), seed=1, gpu=None, validate_only=True)
+    for g in (2, 4, 8):
+        u.setPerformGroup(g)
+    for g in (0, 1, 3, 16):
+        with pytest.raises(_capi.InvalidArgument):
+            u.setPerformGroup(g)
