@@ -1,11 +1,13 @@
 #!/usr/bin/env python3
 """Throughput of the plain colour sweep on one MI355X.
 
-Workload: the large linear melt (512^3 periodic box, 16384 chains of N=512,
-8,388,608 monomers, volume fraction 0.5, 2 colours, bit-packed compact-Morton
-lattice), synthetic seeded initial state.  A step is one Monte-Carlo step
-(every monomer attempts one move).  Inputs are resident in HBM before the
-timed region.  Prints one JSON line.
+Workload (default): the large linear melt (512^3 periodic box, 16384 chains of
+N=512, 8,388,608 monomers, volume fraction 0.5, 2 colours, bit-packed
+compact-Morton lattice), synthetic seeded initial state; --workload small_melt
+(64^3, 512 x 32) and --workload network (256^3 diamond network, 1,064,960
+monomers) time the other single-system cases the same way.  A step is one
+Monte-Carlo step (every monomer attempts one move).  Inputs are resident in
+HBM before the timed region.  Prints one JSON line.
 
 The cpu_baseline leg times the oracle's random-sequential updater (the
 LeMonADE-style CPU updater, one core) on a bounded number of MCS of the same
@@ -27,9 +29,27 @@ if str(ROOT) not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
-    "large_melt": dict(box=512, n_chains=16384, chain_len=512, seed=0x5EED000000000002),
-    "small_melt": dict(box=64, n_chains=512, chain_len=32, seed=0x5EED000000000001),
+    # 512^3 periodic melt, 16384 linear chains of 512 (8,388,608 monomers, 2 colours)
+    "large_melt": dict(kind="melt", box=512, n_chains=16384, chain_len=512, seed=0x5EED000000000002),
+    # 64^3 periodic melt, 512 linear chains of 32 (16,384 monomers, 2 colours)
+    "small_melt": dict(kind="melt", box=64, n_chains=512, chain_len=32, seed=0x5EED000000000001),
+    # 256^3 periodic end-linked diamond network: 8 x 8 x 32 unit cells, 16,384
+    # 4-functional crosslinkers joined by 32,768 strands of 32 (1,064,960 monomers)
+    "network": dict(kind="network", box=256, cells=(8, 8, 32), strand_len=32, seed=0x5EED000000000003),
 }
+
+
+def build_workload(w: dict) -> dict:
+    from pscbfm_amd import systems
+    if w["kind"] == "melt":
+        return systems.linear_melt(w["box"], w["n_chains"], w["chain_len"], w["seed"])
+    return systems.diamond_network(w["box"], w["cells"], w["strand_len"], w["seed"])
+
+
+def workload_config(name: str, w: dict) -> dict:
+    if w["kind"] == "melt":
+        return {"workload": name, "box": w["box"], "chains": w["n_chains"], "chain_len": w["chain_len"]}
+    return {"workload": name, "box": w["box"], "unit_cells": list(w["cells"]), "strand_len": w["strand_len"]}
 
 
 def log(msg: str) -> None:
@@ -218,7 +238,7 @@ def main() -> int:
               "use --gpus 1", file=sys.stderr)
         return 2
 
-    from pscbfm_amd import device_count, systems, updater_from_system
+    from pscbfm_amd import device_count, updater_from_system
     if device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
@@ -233,7 +253,7 @@ def main() -> int:
 
     w = WORKLOADS[args.workload]
     t = time.perf_counter()
-    system = systems.linear_melt(w["box"], w["n_chains"], w["chain_len"], w["seed"])
+    system = build_workload(w)
     n = len(system["xyz"])
     log(f"generated {args.workload}: {n} monomers in {time.perf_counter() - t:.1f} s")
     t = time.perf_counter()
@@ -297,9 +317,9 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
-        "data": "synthetic (seeded random-walk melt)",
-        "config": {"workload": args.workload, "box": w["box"], "chains": w["n_chains"],
-                   "chain_len": w["chain_len"], "monomers": n, "colours": n_colors, "mcs": args.steps,
+        "data": "synthetic (seeded random-walk melt)" if w["kind"] == "melt" else
+                "synthetic (seeded diamond network)",
+        "config": {**workload_config(args.workload, w), "monomers": n, "colours": n_colors, "mcs": args.steps,
                    "p_acc": round(p_acc, 4), "p_proposal": round(p_prop, 4),
                    "rng": "int_hash (the reference's default generator)"},
         "roofline": roofline,
