@@ -331,8 +331,9 @@ int bfm_server_graph_stats(bfm_server *s, uint64_t *launches, uint64_t *updates,
 int bfm_batch_graph_stats(bfm_batch *b, uint64_t *launches, uint64_t *updates, uint64_t *instantiations);
 
 /* Measurement helper, no counterpart in the reference: times `repeats`
- * launches of a plain 16-byte-per-lane copy kernel over `bytes` on device
- * `gpu` (after one warm-up launch).  A copy of B bytes reads B and writes B
+ * launches of a plain copy kernel (one 16-byte element per lane, one pass of
+ * the grid, at most 2^31 - 1 blocks of 256) over `bytes` on device `gpu`
+ * (after one warm-up launch).  A copy of B bytes reads B and writes B
  * bytes of HBM, which calibrates the FETCH_SIZE / WRITE_SIZE counters and
  * gives the achievable streaming bandwidth 2*B / avg_ms. */
 int bfm_calibration_copy(int gpu, uint64_t bytes, int repeats, double *avg_ms);

@@ -415,11 +415,14 @@ __global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__r
     perform_body<kCubic, kGroup>(buf, cb, e.parity, blk);
 }
 
-// Plain streaming copy, 16 bytes per lane per iteration: the reference point
-// for HBM counters (a copy of B bytes reads B and writes B).
+// Plain streaming copy, one 16-byte element per lane and one pass of the
+// grid: the reference point for HBM counters (a copy of B bytes reads B and
+// writes B) and for the achievable bandwidth.  On the MI355X this form
+// streams 6.2 TB/s; a grid-stride loop over 16384 blocks reached 4.6 TB/s.
 __global__ __launch_bounds__(kBlock) void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
 {
-    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock)
+    const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n)
         dst[i] = src[i];
 }
 
@@ -531,8 +534,10 @@ hipError_t launch_perform_batch(uint32_t cls, const BatchSystem *sys, const uint
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream)
 {
     const size_t n = bytes / sizeof(uint4);
-    const uint32_t blocks = uint32_t(std::min<size_t>((n + kBlock - 1) / kBlock, 256 * 64));
-    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(kBlock), 0, stream, static_cast<const uint4 *>(src),
+    const size_t blocks = (n + kBlock - 1) / kBlock;
+    if (blocks == 0 || blocks > 0x7fffffffu)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_copy, dim3(uint32_t(blocks)), dim3(kBlock), 0, stream, static_cast<const uint4 *>(src),
                        static_cast<uint4 *>(dst), n);
     return hipGetLastError();
 }
