@@ -190,7 +190,8 @@ int bfm_set_xcd_ranges(bfm_updater *h, int on);
  * perform wave takes, 2, 4 (default) or 8.  bfm_autotune runs probe_mcs MCS
  * under each of the 6 configurations (perform group x XCD placement), timed
  * with HIP events, keeps the fastest and reports it; the system advances by
- * 6 * probe_mcs MCS meanwhile.  After initialize. */
+ * 7 * probe_mcs MCS meanwhile (one untimed probe first).  On an error the
+ * configuration in force before the call stays.  After initialize. */
 int bfm_set_perform_group(bfm_updater *h, uint32_t group);
 int bfm_autotune(bfm_updater *h, uint32_t probe_mcs, uint32_t *group, int *xcd_ranges);
 

@@ -61,7 +61,8 @@ class Batch:
 
     def autotune(self, probe_mcs: int, chunk: int = 100) -> dict:
         """Times probe_mcs MCS of every member under each launch configuration
-        and keeps the fastest (members advance by 6 * probe_mcs MCS)."""
+        and keeps the fastest (members advance by 7 * probe_mcs MCS: one untimed
+        probe, then one per configuration)."""
         g, x = c_uint32(), ctypes.c_int()
         _capi.check(self._lib.bfm_batch_autotune(self._h, int(probe_mcs), int(chunk), byref(g), byref(x)))
         return {"perform_group": g.value, "xcd_ranges": bool(x.value)}

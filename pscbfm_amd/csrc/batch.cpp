@@ -214,10 +214,14 @@ void BatchEngine::autotune(uint32_t probe_mcs, uint32_t chunk)
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
+    const uint32_t group0 = mPerformGroup;
+    const bool xcd0 = mXcdRanges;
     float best = -1.0f;
-    uint32_t best_group = mPerformGroup;
-    bool best_xcd = mXcdRanges;
+    uint32_t best_group = group0;
+    bool best_xcd = xcd0;
     try {
+        // one untimed probe first, so that no configuration pays for cold caches
+        run(probe_mcs, chunk);
         for (const bool xcd : {true, false})
             for (const uint32_t g : kPerformGroups) {
                 setXcdRanges(xcd);
@@ -237,6 +241,8 @@ void BatchEngine::autotune(uint32_t probe_mcs, uint32_t chunk)
     } catch (...) {
         hipEventDestroy(e0);
         hipEventDestroy(e1);
+        setPerformGroup(group0);
+        setXcdRanges(xcd0);
         throw;
     }
     hipEventDestroy(e0);

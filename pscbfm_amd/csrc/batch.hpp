@@ -111,8 +111,9 @@ public:
     uint32_t performGroup() const { return mPerformGroup; }
     // Times probe_mcs MCS (in chunks of `chunk`) on every active member under
     // every launch configuration (perform group x XCD placement) and keeps the
-    // fastest.  The members run on meanwhile (6 * probe_mcs MCS each); their
-    // trajectories are the same under every configuration.
+    // fastest.  The members run on meanwhile (one untimed probe first, so
+    // 7 * probe_mcs MCS each); their trajectories are the same under every
+    // configuration.
     void autotune(uint32_t probe_mcs, uint32_t chunk);
     uint64_t graphLaunches() const { return mGraphLaunches; }
     uint64_t graphUpdates() const { return mGraphUpdates; }

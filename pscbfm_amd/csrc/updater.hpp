@@ -80,9 +80,9 @@ public:
     void setPerformGroup(uint32_t group);
     uint32_t performGroup() const { return mPerformGroup; }
     // Times probe_mcs MCS under every launch configuration (perform group x
-    // XCD placement) and keeps the fastest.  The run goes on meanwhile (the
-    // age advances by 6 * probe_mcs); the trajectory is the same under every
-    // configuration.
+    // XCD placement) and keeps the fastest.  The run goes on meanwhile (one
+    // untimed probe first, so the age advances by 7 * probe_mcs); the
+    // trajectory is the same under every configuration.
     void autotune(uint32_t probe_mcs);
     const HostSystem &system() const { return mSys; }
     bool initialized() const { return mInitialized; }

@@ -156,7 +156,8 @@ class UpdaterGPUScBFM_AB_Type:
 
     def autotune(self, probe_mcs: int) -> dict:
         """Times probe_mcs MCS under each launch configuration and keeps the
-        fastest (the system advances by 6 * probe_mcs MCS)."""
+        fastest (the system advances by 7 * probe_mcs MCS: one untimed probe, then
+        one per configuration)."""
         g, x = c_uint32(), ctypes.c_int()
         _capi.check(self._lib.bfm_autotune(self._h, int(probe_mcs), byref(g), byref(x)))
         return {"perform_group": g.value, "xcd_ranges": bool(x.value)}

@@ -132,8 +132,8 @@ def test_batch_autotune_keeps_the_trajectories(gpu_ready):
     assert cfg["perform_group"] in (2, 4, 8)
     b.run(20, 10)
     for (s, seed), m in zip(cases, ids):
-        assert b.age(m) == 50
+        assert b.age(m) == 55       # one untimed probe + 6 configurations, 5 MCS each, then 20
         u = updater_from_system(s, seed=seed, gpu=0)
-        u.runSimulationOnGPU(50)
+        u.runSimulationOnGPU(55)
         assert np.array_equal(b.positions(m), u.getMonomerPositions(len(s["xyz"])))
         u.cleanup()

@@ -196,10 +196,10 @@ def test_autotune_keeps_the_trajectory(gpu_ready):
     seed = 0x5EED0000000000F3
     print(f"case: autotune n={n}", flush=True)
     u = updater_from_system(s, seed=seed, gpu=0)
-    cfg = u.autotune(5)                 # 6 configurations x 5 MCS
+    cfg = u.autotune(5)                 # one untimed probe + 6 configurations, 5 MCS each
     assert cfg["perform_group"] in (2, 4, 8)
-    assert u.getAge() == 30
-    u.runSimulationOnGPU(70)
+    assert u.getAge() == 35
+    u.runSimulationOnGPU(65)
     got = u.getMonomerPositions(n)
     colors, _ = u.getColors(n)
     st = u.getStats()
