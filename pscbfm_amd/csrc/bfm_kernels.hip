@@ -73,17 +73,9 @@ __device__ __forceinline__ int4 moved(const int4 &r, int dir)
     return q;
 }
 
-// Per-system tables in LDS (file-scope __shared__ arrays keep the LDS address
-// space visible to the compiler): the 512-bit forbidden-bond mask, read by
-// every check workgroup, and the Morton tables of a non-cubic box.
-__shared__ uint32_t s_bond[16];
+// The Morton tables of a non-cubic box, in LDS (a file-scope __shared__ array
+// keeps the LDS address space visible to the compiler).
 __shared__ uint32_t s_dep[3 * 64];
-
-__device__ __forceinline__ void load_bond(const Tables *t)
-{
-    if (threadIdx.x < 16)
-        s_bond[threadIdx.x] = as_global(t)->bond[threadIdx.x];
-}
 
 __device__ __forceinline__ void load_dep(const Tables *t)
 {
@@ -92,11 +84,21 @@ __device__ __forceinline__ void load_dep(const Tables *t)
         s_dep[k] = src[k];
 }
 
-__device__ __forceinline__ bool bond_forbidden(int dx, int dy, int dz)
-{
-    const uint32_t b = bond_index(dx, dy, dz);
-    return (s_bond[b >> 5] >> (b & 31u)) & 1u;
-}
+// The 512-bit forbidden-bond mask held across a wavefront: lane k holds word
+// k & 15, and a lookup is one cross-lane read.  Unlike a table in LDS it needs
+// no workgroup barrier before the first data load, and its own load overlaps
+// the body's.  Every lane of the wave must be active at a lookup (an inactive
+// source lane supplies no data).
+struct WaveBondMask {
+    uint32_t word;
+    __device__ __forceinline__ explicit WaveBondMask(const Tables *t)
+        : word(as_global(t)->bond[threadIdx.x & 15u]) {}
+    __device__ __forceinline__ bool forbidden(int dx, int dy, int dz) const
+    {
+        const uint32_t b = bond_index(dx, dy, dz);
+        return (uint32_t(__shfl(int(word), int(b >> 5))) >> (b & 31u)) & 1u;
+    }
+};
 
 struct DepositTable {
     __device__ __forceinline__ uint32_t operator()(int a, uint32_t v) const
@@ -126,10 +128,15 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
     const gptr<uint32_t> claim_prev = as_global(pick(parity, buf.claim1, buf.claim0));
 
     const uint32_t t = blk * kBlock + threadIdx.x;
+    const WaveBondMask bonds(buf.tables);
     bool pass = false;
     uint2 entry = make_uint2(0u, 0u);
-    if (t < count) {
-        const uint32_t i = begin + t;
+    if (count != 0) {
+        // Every lane of the wave runs the body (lanes past the segment on its
+        // last monomer), so that the cross-lane bond lookups find every
+        // source lane active; only lanes inside the segment may pass.
+        const bool inside = t < count;
+        const uint32_t i = begin + (inside ? t : count - 1u);
         const uint32_t p0 = ppos[i];
         const uint32_t idn = ids[i];
         const uint32_t id = idn & kIdMask, nn = idn >> kIdBits;
@@ -178,17 +185,20 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
         for (int j = 0; j < kMaxConnectivity; ++j)
             if (uint32_t(j) < nn_wave)
                 qpos[j] = ppos[uint32_t(j) < nn ? partner[j] : i];
+        // (nn_wave is wave-uniform: the lookups run on every lane; a lane
+        // with fewer partners tests its own position and ignores the result)
 #pragma unroll
         for (int j = 0; j < kMaxConnectivity; ++j)
-            if (uint32_t(j) < nn_wave && uint32_t(j) < nn) {
+            if (uint32_t(j) < nn_wave) {
                 const int4 q = unpack_xyz(qpos[j]);
-                if (bond_forbidden(q.x - r1.x, q.y - r1.y, q.z - r1.z))
+                const bool bad = bonds.forbidden(q.x - r1.x, q.y - r1.y, q.z - r1.z);
+                if (uint32_t(j) < nn && bad)
                     ok = false;
             }
         if ((l[0] & f.m[0]) | (l[1] & f.m[1]) | (l[2] & f.m[2]) | (l[3] & f.m[3]))
             ok = false;
 
-        if (ok) {
+        if (ok && inside) {
             // the reference's tentative lattice (dpLatticeTmp, :1012-1023),
             // one bit per moved corner
             atomicOr((uint32_t *)&claim_cur[f.new_site >> 5], 1u << (f.new_site & 31u));
@@ -218,10 +228,10 @@ template <bool kAllPeriodic, bool kCubic>
 __global__ __launch_bounds__(kBlock) void k_check(SweepBuffers buf, uint32_t begin, uint32_t count, uint64_t age,
                                                   uint32_t sub_seed, uint32_t parity)
 {
-    load_bond(buf.tables);
-    if (!kCubic)
+    if (!kCubic) {
         load_dep(buf.tables);
-    __syncthreads();
+        __syncthreads();
+    }
     const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
     check_body<kAllPeriodic, kCubic>(buf, begin, count, age, sub_seed, parity, blk, gridDim.x);
 }
@@ -385,10 +395,10 @@ __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__res
     // the system's buffers by value: the body's stores cannot alias them, so
     // they stay in registers as a plain launch's arguments do
     const SweepBuffers buf = sys[s].buf;
-    load_bond(buf.tables);
-    if (!kCubic)
+    if (!kCubic) {
         load_dep(buf.tables);
-    __syncthreads();
+        __syncthreads();
+    }
     check_body<kAllPeriodic, kCubic>(buf, e.begin, e.count, e.age, e.sub_seed, e.parity, blk, nb);
 }
 
