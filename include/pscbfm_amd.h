@@ -172,10 +172,11 @@ typedef struct bfm_stats {
 int bfm_get_stats(const bfm_updater *h, bfm_stats *out);
 
 /* every > 0: the check and perform launches of every `every`-th substep are
- * bracketed by HIP events on the handle's stream and their durations summed
- * into bfm_stats (timed_launches counts the timed substeps); 0: off.  Each
- * event costs a few microseconds of launch pipeline, so sampling (e.g. 8)
- * keeps the sweep rate undisturbed. */
+ * timed by HIP events on the handle's stream that take each kernel's own
+ * dispatch timestamps (hipExtLaunchKernelGGL: the execution time a kernel
+ * trace reports), summed into bfm_stats (timed_launches counts the timed
+ * substeps); 0: off.  Timed launches cost some launch pipeline, so sampling
+ * (e.g. 8) keeps the sweep rate undisturbed. */
 int bfm_set_kernel_timing(bfm_updater *h, int every);
 
 /* Block-to-XCD placement of the sweep kernels (no counterpart in the

@@ -26,6 +26,8 @@
 //    most one corner, and no two accepted moves of a substep share a block.
 #include "bfm_kernels.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <type_traits>
 
@@ -438,43 +440,47 @@ __global__ __launch_bounds__(kBlock) void k_copy(const uint4 *__restrict__ src, 
 
 }  // namespace
 
-hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t sub_seed,
-                        uint32_t parity, hipStream_t stream)
+// Launches `kernel`; given a start and a stop event, the events take the
+// timestamps of the kernel's own dispatch (hipExtLaunchKernelGGL), the
+// interval a kernel trace reports, instead of the span between two markers
+// queued around it (which adds the dispatch and the end-of-kernel flush).
+template <class... P, class... A>
+void dispatch(void (*kernel)(P...), dim3 grid, hipStream_t stream, hipEvent_t start, hipEvent_t stop, A... args)
 {
-    const dim3 grid(check_blocks(count)), block(kBlock);
-    const bool periodic = buf.geom.periodic == 7u, cubic = buf.geom.cubic != 0u;
-    if (periodic && cubic)
-        hipLaunchKernelGGL((k_check<true, true>), grid, block, 0, stream, buf, begin, count, age, sub_seed, parity);
-    else if (periodic)
-        hipLaunchKernelGGL((k_check<true, false>), grid, block, 0, stream, buf, begin, count, age, sub_seed,
-                           parity);
-    else if (cubic)
-        hipLaunchKernelGGL((k_check<false, true>), grid, block, 0, stream, buf, begin, count, age, sub_seed,
-                           parity);
+    if (start && stop)
+        hipExtLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, stream, start, stop, 0, P(args)...);
     else
-        hipLaunchKernelGGL((k_check<false, false>), grid, block, 0, stream, buf, begin, count, age, sub_seed,
-                           parity);
+        hipLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, stream, P(args)...);
+}
+
+hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t sub_seed,
+                        uint32_t parity, hipStream_t stream, hipEvent_t start, hipEvent_t stop)
+{
+    const dim3 grid(check_blocks(count));
+    const bool periodic = buf.geom.periodic == 7u, cubic = buf.geom.cubic != 0u;
+    auto kernel = periodic ? (cubic ? k_check<true, true> : k_check<true, false>)
+                           : (cubic ? k_check<false, true> : k_check<false, false>);
+    dispatch(kernel, grid, stream, start, stop, buf, begin, count, age, sub_seed, parity);
     return hipGetLastError();
 }
 
 template <uint32_t G>
-void launch_perform_group(const SweepBuffers &buf, uint32_t cb, uint32_t parity, hipStream_t stream)
+void launch_perform_group(const SweepBuffers &buf, uint32_t cb, uint32_t parity, hipStream_t stream,
+                          hipEvent_t start, hipEvent_t stop)
 {
-    const dim3 grid(perform_blocks(cb, G)), block(kBlock);
-    if (buf.geom.cubic != 0u)
-        hipLaunchKernelGGL((k_perform<true, G>), grid, block, 0, stream, buf, cb, parity);
-    else
-        hipLaunchKernelGGL((k_perform<false, G>), grid, block, 0, stream, buf, cb, parity);
+    const dim3 grid(perform_blocks(cb, G));
+    auto kernel = buf.geom.cubic != 0u ? k_perform<true, G> : k_perform<false, G>;
+    dispatch(kernel, grid, stream, start, stop, buf, cb, parity);
 }
 
 hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t parity, hipStream_t stream,
-                          uint32_t group)
+                          uint32_t group, hipEvent_t start, hipEvent_t stop)
 {
     const uint32_t cb = check_blocks(count);
     switch (group) {
-    case 2: launch_perform_group<2>(buf, cb, parity, stream); break;
-    case 8: launch_perform_group<8>(buf, cb, parity, stream); break;
-    default: launch_perform_group<4>(buf, cb, parity, stream); break;
+    case 2: launch_perform_group<2>(buf, cb, parity, stream, start, stop); break;
+    case 8: launch_perform_group<8>(buf, cb, parity, stream, start, stop); break;
+    default: launch_perform_group<4>(buf, cb, parity, stream, start, stop); break;
     }
     return hipGetLastError();
 }

@@ -69,13 +69,15 @@ inline bool valid_perform_group(uint32_t g) { return g == 2 || g == 4 || g == 8;
 // claim[parity] and are compacted per wave into surv.  Also clears
 // claim[parity ^ 1], the marks of the previous substep.
 // sub_seed: substep_seed() of this substep (used by the integer-hash RNG).
+// With a start and a stop event, the events take the kernel's own dispatch
+// timestamps (its execution time, as a kernel trace reports it).
 hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t sub_seed,
-                        uint32_t parity, hipStream_t stream);
+                        uint32_t parity, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 
 // Phase 2: every survivor with no other moved corner in its front moves.
 // count is the colour segment size of the check launch.
 hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t parity, hipStream_t stream,
-                          uint32_t group = kPerformGroup);
+                          uint32_t group = kPerformGroup, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 
 // ---- many systems per launch -------------------------------------------------
 

@@ -292,28 +292,24 @@ void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs)
         for (uint32_t k = 0; k < s.n_colours; ++k) {
             const uint32_t c = order[k];
             const uint32_t parity = uint32_t(mSubsteps & 1u);
-            hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+            // timed substeps: start and stop events of each kernel's own dispatch
+            hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
             const bool timed = mTimingEvery != 0 && mSubsteps % mTimingEvery == 0;
-            if (timed) {
+            if (timed)
                 for (auto &x : ev)
                     x = takeEvent();
-                hip_check(hipEventRecord(ev[0], mStream), "hipEventRecord");
-            }
             hip_check(launch_check(mBuf, s.seg_begin[c], s.seg_count[c], age, substep_seed(mSeed, age, k), parity,
-                                   mStream),
+                                   mStream, ev[0], ev[1]),
                       "check kernel");
+            hip_check(launch_perform(mBuf, s.seg_count[c], parity, mStream, mPerformGroup, ev[2], ev[3]),
+                      "perform kernel");
             if (timed)
-                hip_check(hipEventRecord(ev[1], mStream), "hipEventRecord");
-            hip_check(launch_perform(mBuf, s.seg_count[c], parity, mStream, mPerformGroup), "perform kernel");
-            if (timed) {
-                hip_check(hipEventRecord(ev[2], mStream), "hipEventRecord");
-                mEvents.insert(mEvents.end(), ev, ev + 3);
-            }
+                mEvents.insert(mEvents.end(), ev, ev + 4);
             ++mSubsteps;
         }
         mAttempts += s.n;
         ++mAge;
-        if (mEvents.size() > 3 * 4096)
+        if (mEvents.size() > 4 * 4096)
             collectTimings();
     }
 }
@@ -335,10 +331,10 @@ void UpdaterGPUScBFM_AB_Type::collectTimings() const
     if (mEvents.empty())
         return;
     hip_check(hipEventSynchronize(mEvents.back()), "hipEventSynchronize");
-    for (size_t k = 0; k + 2 < mEvents.size(); k += 3) {
+    for (size_t k = 0; k + 3 < mEvents.size(); k += 4) {
         float a = 0.f, b = 0.f;
         hip_check(hipEventElapsedTime(&a, mEvents[k], mEvents[k + 1]), "hipEventElapsedTime");
-        hip_check(hipEventElapsedTime(&b, mEvents[k + 1], mEvents[k + 2]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&b, mEvents[k + 2], mEvents[k + 3]), "hipEventElapsedTime");
         mCheckMs += a;
         mPerformMs += b;
         ++mTimedLaunches;

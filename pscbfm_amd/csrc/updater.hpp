@@ -137,7 +137,7 @@ private:
     uint32_t mPerformGroup = kPerformGroup;
     int64_t mStepsBetweenSortings = 5000;   // mnStepsBetweenSortings, UpdaterGPUScBFM_AB_Type.cu:1379
     std::vector<hipEvent_t> mEventPool;          // reusable events
-    mutable std::vector<hipEvent_t> mEvents;     // pending (start, mid, end) triples
+    mutable std::vector<hipEvent_t> mEvents;     // pending (check start, stop, perform start, stop) quadruples
     mutable double mCheckMs = 0.0, mPerformMs = 0.0;
     mutable uint64_t mTimedLaunches = 0;
     hipEvent_t takeEvent();
