@@ -275,7 +275,7 @@ void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
     // vectors of the previous rebuild must have been copied before reuse.
     hip_check(hipEventSynchronize(mTablesCopied), "hipEventSynchronize (tables)");
     std::vector<BatchSystem> &sys = mHostSys;
-    std::vector<uint32_t> *cblk = mHostCheckBlk, *pblk = mHostPerformBlk;
+    std::vector<BlockEntry> *cblk = mHostCheckBlk, *pblk = mHostPerformBlk;
     sys.clear();
     for (uint32_t c = 0; c < kClasses; ++c) {
         cblk[c].clear();
@@ -287,12 +287,12 @@ void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
         if (!m.slab)
             throw std::invalid_argument("batch: member was removed");
         const uint32_t cls = kernel_class(m.sys.geom);
-        BatchSystem d = m.dev;
-        d.check_base = uint32_t(cblk[cls].size());
-        d.perform_base = uint32_t(pblk[cls].size());
-        cblk[cls].insert(cblk[cls].end(), m.check_blocks_max, j);
-        pblk[cls].insert(pblk[cls].end(), perform_blocks(m.check_blocks_max, mPerformGroup), j);
-        sys.push_back(d);
+        for (uint32_t b = 0; b < m.check_blocks_max; ++b)
+            cblk[cls].push_back(BlockEntry{j, b});
+        const uint32_t pb = perform_blocks(m.check_blocks_max, mPerformGroup);
+        for (uint32_t b = 0; b < pb; ++b)
+            pblk[cls].push_back(BlockEntry{j, b});
+        sys.push_back(m.dev);
         mMaxColours = std::max(mMaxColours, m.sys.n_colours);
     }
     ensureDevice(mDevSys, mSysCap, sys.size(), "allocation of the batch systems");
@@ -305,10 +305,10 @@ void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
             continue;
         ensureDevice(mDevCheckBlk[c], mCheckCap[c], cblk[c].size(), "allocation of the batch blocks");
         ensureDevice(mDevPerformBlk[c], mPerformCap[c], pblk[c].size(), "allocation of the batch blocks");
-        hip_check(hipMemcpyAsync(mDevCheckBlk[c], cblk[c].data(), sizeof(uint32_t) * cblk[c].size(),
+        hip_check(hipMemcpyAsync(mDevCheckBlk[c], cblk[c].data(), sizeof(BlockEntry) * cblk[c].size(),
                                  hipMemcpyHostToDevice, mStream),
                   "upload batch blocks");
-        hip_check(hipMemcpyAsync(mDevPerformBlk[c], pblk[c].data(), sizeof(uint32_t) * pblk[c].size(),
+        hip_check(hipMemcpyAsync(mDevPerformBlk[c], pblk[c].data(), sizeof(BlockEntry) * pblk[c].size(),
                                  hipMemcpyHostToDevice, mStream),
                   "upload batch blocks");
     }

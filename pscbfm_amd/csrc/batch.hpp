@@ -153,13 +153,13 @@ private:
     bool mPooled = false;                    // stream-ordered allocation available
     // host copies of the tables, kept until their upload has run
     std::vector<BatchSystem> mHostSys;
-    std::vector<uint32_t> mHostCheckBlk[kClasses], mHostPerformBlk[kClasses];
+    std::vector<BlockEntry> mHostCheckBlk[kClasses], mHostPerformBlk[kClasses];
     hipEvent_t mTablesCopied = nullptr;
     BatchSystem *mDevSys = nullptr;
     size_t mSysCap = 0;
     uint32_t mMaxColours = 0;
     // per kernel-variant class: block -> system tables
-    uint32_t *mDevCheckBlk[kClasses] = {}, *mDevPerformBlk[kClasses] = {};
+    BlockEntry *mDevCheckBlk[kClasses] = {}, *mDevPerformBlk[kClasses] = {};
     uint32_t mCheckBlocks[kClasses] = {}, mPerformBlocks[kClasses] = {};
     size_t mCheckCap[kClasses] = {}, mPerformCap[kClasses] = {};
 

@@ -379,24 +379,27 @@ __global__ __launch_bounds__(kBlock) void k_schedule(const BatchSystem *__restri
     }
 }
 
+// The prologue of a batched block is a chain of dependent loads (block entry,
+// then the system's slot entry and buffers) before the body's first data
+// load; the slot entry and the buffers depend on the system only, so both are
+// read before either exit test.  n_blocks is the grid size, passed with the
+// arguments rather than read from the launch's implicit arguments later.
 template <bool kAllPeriodic, bool kCubic>
 __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__restrict__ sys,
-                                                        const uint32_t *__restrict__ blk_sys,
+                                                        const BlockEntry *__restrict__ blk_tab,
                                                         const SchedEntry *__restrict__ sched, uint32_t n_sys,
-                                                        uint32_t slot, uint32_t xcd_ranges)
+                                                        uint32_t slot, uint32_t xcd_ranges, uint32_t n_blocks)
 {
-    const uint32_t lb = xcd_ranges ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
-    const uint32_t s = blk_sys[lb];
-    const SchedEntry e = sched[size_t(slot) * n_sys + s];
-    if (e.colour == kNoColour)
-        return;
-    const uint32_t nb = check_blocks(e.count);
-    const uint32_t blk = lb - sys[s].check_base;
-    if (blk >= nb)
-        return;
+    const uint32_t lb = xcd_ranges ? xcd_logical(blockIdx.x, n_blocks) : blockIdx.x;
+    const BlockEntry be = blk_tab[lb];
+    const SchedEntry e = sched[size_t(slot) * n_sys + be.sys];
     // the system's buffers by value: the body's stores cannot alias them, so
     // they stay in registers as a plain launch's arguments do
-    const SweepBuffers buf = sys[s].buf;
+    const SweepBuffers buf = sys[be.sys].buf;
+    const uint32_t nb = check_blocks(e.count);
+    if (e.colour == kNoColour || be.blk >= nb)
+        return;
+    const uint32_t blk = be.blk;
     if (!kCubic) {
         load_dep(buf.tables);
         __syncthreads();
@@ -406,20 +409,18 @@ __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__res
 
 template <bool kCubic, uint32_t kGroup>
 __global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__restrict__ sys,
-                                                          const uint32_t *__restrict__ blk_sys,
+                                                          const BlockEntry *__restrict__ blk_tab,
                                                           const SchedEntry *__restrict__ sched, uint32_t n_sys,
-                                                          uint32_t slot, uint32_t xcd_ranges)
+                                                          uint32_t slot, uint32_t xcd_ranges, uint32_t n_blocks)
 {
-    const uint32_t lb = xcd_ranges ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
-    const uint32_t s = blk_sys[lb];
-    const SchedEntry e = sched[size_t(slot) * n_sys + s];
-    if (e.colour == kNoColour)
-        return;
+    const uint32_t lb = xcd_ranges ? xcd_logical(blockIdx.x, n_blocks) : blockIdx.x;
+    const BlockEntry be = blk_tab[lb];
+    const SchedEntry e = sched[size_t(slot) * n_sys + be.sys];
+    const SweepBuffers buf = sys[be.sys].buf;
     const uint32_t cb = check_blocks(e.count);
-    const uint32_t blk = lb - sys[s].perform_base;
-    if (blk >= perform_blocks(cb, kGroup))
+    if (e.colour == kNoColour || be.blk >= perform_blocks(cb, kGroup))
         return;
-    const SweepBuffers buf = sys[s].buf;
+    const uint32_t blk = be.blk;
     if (!kCubic) {
         load_dep(buf.tables);
         __syncthreads();
@@ -494,55 +495,55 @@ hipError_t launch_schedule(const BatchSystem *sys, const ChunkEntry *chunk, uint
     return hipGetLastError();
 }
 
-hipError_t launch_check_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
+hipError_t launch_check_batch(uint32_t cls, const BatchSystem *sys, const BlockEntry *blk_tab, uint32_t n_blocks,
                               const SchedEntry *sched, uint32_t n_sys, uint32_t slot, uint32_t xcd_ranges,
                               hipStream_t stream)
 {
     const dim3 grid(n_blocks), block(kBlock);
     switch (cls) {
     case kClassPeriodicCubic:
-        hipLaunchKernelGGL((k_check_batch<true, true>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot,
-                           xcd_ranges);
+        hipLaunchKernelGGL((k_check_batch<true, true>), grid, block, 0, stream, sys, blk_tab, sched, n_sys, slot,
+                           xcd_ranges, n_blocks);
         break;
     case kClassPeriodic:
-        hipLaunchKernelGGL((k_check_batch<true, false>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot,
-                           xcd_ranges);
+        hipLaunchKernelGGL((k_check_batch<true, false>), grid, block, 0, stream, sys, blk_tab, sched, n_sys, slot,
+                           xcd_ranges, n_blocks);
         break;
     case kClassCubic:
-        hipLaunchKernelGGL((k_check_batch<false, true>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot,
-                           xcd_ranges);
+        hipLaunchKernelGGL((k_check_batch<false, true>), grid, block, 0, stream, sys, blk_tab, sched, n_sys, slot,
+                           xcd_ranges, n_blocks);
         break;
     default:
-        hipLaunchKernelGGL((k_check_batch<false, false>), grid, block, 0, stream, sys, blk_sys, sched, n_sys, slot,
-                           xcd_ranges);
+        hipLaunchKernelGGL((k_check_batch<false, false>), grid, block, 0, stream, sys, blk_tab, sched, n_sys, slot,
+                           xcd_ranges, n_blocks);
         break;
     }
     return hipGetLastError();
 }
 
 template <uint32_t G>
-void launch_perform_batch_group(bool cubic, dim3 grid, const BatchSystem *sys, const uint32_t *blk_sys,
+void launch_perform_batch_group(bool cubic, dim3 grid, const BatchSystem *sys, const BlockEntry *blk_tab,
                                 const SchedEntry *sched, uint32_t n_sys, uint32_t slot, uint32_t xcd_ranges,
                                 hipStream_t stream)
 {
     if (cubic)
-        hipLaunchKernelGGL((k_perform_batch<true, G>), grid, dim3(kBlock), 0, stream, sys, blk_sys, sched, n_sys,
-                           slot, xcd_ranges);
+        hipLaunchKernelGGL((k_perform_batch<true, G>), grid, dim3(kBlock), 0, stream, sys, blk_tab, sched, n_sys,
+                           slot, xcd_ranges, grid.x);
     else
-        hipLaunchKernelGGL((k_perform_batch<false, G>), grid, dim3(kBlock), 0, stream, sys, blk_sys, sched, n_sys,
-                           slot, xcd_ranges);
+        hipLaunchKernelGGL((k_perform_batch<false, G>), grid, dim3(kBlock), 0, stream, sys, blk_tab, sched, n_sys,
+                           slot, xcd_ranges, grid.x);
 }
 
-hipError_t launch_perform_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
+hipError_t launch_perform_batch(uint32_t cls, const BatchSystem *sys, const BlockEntry *blk_tab, uint32_t n_blocks,
                                 const SchedEntry *sched, uint32_t n_sys, uint32_t slot, uint32_t xcd_ranges,
                                 hipStream_t stream, uint32_t group)
 {
     const bool cubic = cls == kClassPeriodicCubic || cls == kClassCubic;
     const dim3 grid(n_blocks);
     switch (group) {
-    case 2: launch_perform_batch_group<2>(cubic, grid, sys, blk_sys, sched, n_sys, slot, xcd_ranges, stream); break;
-    case 8: launch_perform_batch_group<8>(cubic, grid, sys, blk_sys, sched, n_sys, slot, xcd_ranges, stream); break;
-    default: launch_perform_batch_group<4>(cubic, grid, sys, blk_sys, sched, n_sys, slot, xcd_ranges, stream); break;
+    case 2: launch_perform_batch_group<2>(cubic, grid, sys, blk_tab, sched, n_sys, slot, xcd_ranges, stream); break;
+    case 8: launch_perform_batch_group<8>(cubic, grid, sys, blk_tab, sched, n_sys, slot, xcd_ranges, stream); break;
+    default: launch_perform_batch_group<4>(cubic, grid, sys, blk_tab, sched, n_sys, slot, xcd_ranges, stream); break;
     }
     return hipGetLastError();
 }

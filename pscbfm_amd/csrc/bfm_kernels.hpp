@@ -89,9 +89,14 @@ struct BatchSystem {
     const uint32_t *seg_begin;   // per colour, sorted index of the first monomer
     const uint32_t *seg_count;   // per colour
     uint32_t n_colours;
-    uint32_t check_base;         // first block of this system in a batched check launch
-    uint32_t perform_base;       // first block of this system in a batched perform launch
     uint32_t pad;
+};
+
+// Block `blk` of system `sys` in a batched launch (one entry per block, so
+// that a block finds its system and its place in it with one load).
+struct BlockEntry {
+    uint32_t sys;
+    uint32_t blk;
 };
 
 // Where a system stands at the start of a chunk; substep k of MCS m of the
@@ -131,10 +136,10 @@ hipError_t launch_schedule(const BatchSystem *sys, const ChunkEntry *chunk, uint
 // batch.  n_blocks = sum over the class's systems of their reserved blocks;
 // blk_sys maps a block to its system (index into sys and into each slot's
 // n_sys schedule entries).
-hipError_t launch_check_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
+hipError_t launch_check_batch(uint32_t cls, const BatchSystem *sys, const BlockEntry *blk_tab, uint32_t n_blocks,
                               const SchedEntry *sched, uint32_t n_sys, uint32_t slot, uint32_t xcd_ranges,
                               hipStream_t stream);
-hipError_t launch_perform_batch(uint32_t cls, const BatchSystem *sys, const uint32_t *blk_sys, uint32_t n_blocks,
+hipError_t launch_perform_batch(uint32_t cls, const BatchSystem *sys, const BlockEntry *blk_tab, uint32_t n_blocks,
                                 const SchedEntry *sched, uint32_t n_sys, uint32_t slot, uint32_t xcd_ranges,
                                 hipStream_t stream,
                                 uint32_t group = kPerformGroup);
