@@ -234,8 +234,11 @@ __global__ __launch_bounds__(kBlock) void k_check(SweepBuffers buf, uint32_t beg
         load_dep(buf.tables);
         __syncthreads();
     }
-    const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
-    check_body<kAllPeriodic, kCubic>(buf, begin, count, age, sub_seed, parity, blk, gridDim.x);
+    // the grid size from the arguments (launch_check sizes the grid so), not
+    // from the launch's implicit arguments, which are read after a branch
+    const uint32_t nblk = check_blocks(count);
+    const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, nblk) : blockIdx.x;
+    check_body<kAllPeriodic, kCubic>(buf, begin, count, age, sub_seed, parity, blk, nblk);
 }
 
 // Phase 2: every wave of perform block `blk` takes the survivors of kGroup
@@ -339,7 +342,8 @@ __global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t c
         load_dep(buf.tables);
         __syncthreads();
     }
-    const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
+    // the grid size from the arguments, as in k_check
+    const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, perform_blocks(check_blocks_, kGroup)) : blockIdx.x;
     perform_body<kCubic, kGroup>(buf, check_blocks_, parity, blk);
 }
 
