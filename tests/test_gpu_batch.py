@@ -137,3 +137,27 @@ def test_batch_autotune_keeps_the_trajectories(gpu_ready):
         u.runSimulationOnGPU(55)
         assert np.array_equal(b.positions(m), u.getMonomerPositions(len(s["xyz"])))
         u.cleanup()
+
+
+def test_mixed_batch_full_size_matches_plain(gpu_ready):
+    """The 512-system mixed workload at full size (20.3 M monomers; box edges 32, 64 and 128,
+    chains of 16 to 128, volume fraction 0.3 to 0.5): 20 MCS as one batch, in chunks of 10,
+    end where each system ends after 20 MCS alone."""
+    from pscbfm_amd import workloads
+    specs = workloads.mixed_batch(512, seed=0xBA7C4)
+    built = [workloads.build(sp) for sp in specs]
+    print(f"case: mixed batch at full size, {len(built)} systems, "
+          f"{sum(len(x['xyz']) for x in built)} monomers", flush=True)
+    b = Batch(0)
+    ids = [b.add(updater_from_system(x, seed=sp["seed"], validate_only=True), len(x["xyz"]))
+           for x, sp in zip(built, specs)]
+    b.run(20, 10)
+    b.synchronize()
+    bad = []
+    for x, sp, m in zip(built, specs, ids):
+        u = updater_from_system(x, seed=sp["seed"], gpu=0)
+        u.runSimulationOnGPU(20)
+        if not np.array_equal(b.positions(m), u.getMonomerPositions(len(x["xyz"]))):
+            bad.append(m)
+        u.cleanup()
+    assert not bad, f"{len(bad)} of {len(ids)} members differ from their plain runs"
