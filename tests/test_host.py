@@ -211,6 +211,23 @@ def test_device_helpers_on_host():
     assert out.stdout.startswith("ok ")
 
 
+def test_host_code_under_sanitizers(tmp_path):
+    """The generators, initialize()'s planning and checks, the .bfm reader and
+    writer and the oracle's sweeps run clean under AddressSanitizer (leaks
+    included) and UndefinedBehaviorSanitizer (tests/cpp/host_sanitizer_check.cpp)."""
+    import os
+    import subprocess
+    exe = Path(__file__).resolve().parent / "cpp" / "host_sanitizer_check"
+    assert exe.exists(), "built by __graft_entry__.build()"
+    # verify_asan_link_order=0: a library the environment preloads may come
+    # before the sanitizer runtime; that is not an error of the code under test
+    env = dict(os.environ, ASAN_OPTIONS="halt_on_error=1:detect_leaks=1:verify_asan_link_order=0",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1", OMP_NUM_THREADS="2")
+    out = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True, timeout=300, env=env)
+    assert out.returncode == 0, out.stdout + out.stderr[-4000:]
+    assert out.stdout.startswith("ok ")
+
+
 def test_perform_group_must_be_compiled():
     from pscbfm_amd import _capi, systems, updater_from_system
     u = updater_from_system(systems.linear_melt(16, 8, 8, seed=1), seed=1, gpu=None, validate_only=True)
