@@ -53,6 +53,40 @@ def test_batch_members_match_oracle(gpu_ready, graphs, xcd_ranges):
         assert b.age(m) == age + 150
 
 
+def four_melts_in_a_slab(periodic):
+    """Four 16^3 melts (coordinates within [0, 14]) side by side in a 64 x 16 x 32 box."""
+    s = systems.linear_melt(16, 16, 16, seed=31)
+    parts, bonds = [], []
+    for k, off in enumerate([(0, 0, 0), (16, 0, 0), (32, 0, 16), (48, 0, 16)]):
+        parts.append(s["xyz"] + np.array(off, dtype=np.int32))
+        bonds.append(s["bonds"] + k * len(s["xyz"]))
+    return {"box": (64, 16, 32), "periodic": periodic, "xyz": np.concatenate(parts),
+            "bonds": np.concatenate(bonds), "forbidden": s["forbidden"]}
+
+
+def test_batch_kernel_classes(gpu_ready):
+    """One member of every kernel class (periodic or walled, cubic box or not) and a network
+    in one batch, the classes launched side by side in every slot; each member ends where the
+    oracle's replay of it alone ends."""
+    melt = systems.linear_melt(32, 100, 16, seed=6)
+    cases = [(melt, 0x5EED0000000000E1, 0),                                   # periodic, cubic
+             (dict(melt, periodic=(0, 1, 0)), 0x5EED0000000000E2, 40),         # walls, cubic
+             (four_melts_in_a_slab((1, 1, 1)), 0x5EED0000000000E3, 7),          # periodic, non-cubic
+             (four_melts_in_a_slab((1, 0, 1)), 0x5EED0000000000E4, 0),          # walls, non-cubic
+             (systems.diamond_network(64, (2, 2, 8), 32, seed=5), 0x5EED0000000000E5, 3)]
+    print(f"case: batch of the four kernel classes and a network", flush=True)
+    b = Batch(0)
+    ids = [b.add(updater_from_system(s, seed=seed, age=age, validate_only=True), len(s["xyz"]))
+           for s, seed, age in cases]
+    b.run(120, chunk=40)
+    for m, (s, seed, age) in zip(ids, cases):
+        want, acc, prop = oracle_final(s, seed, age, 120)
+        assert np.array_equal(b.positions(m), want), f"member {m} differs"
+        st = b.stats(m)
+        assert (st["accepted"], st["proposals"]) == (acc, prop)
+        assert st["accepted"] > 0
+
+
 def test_batch_subset_and_removal(gpu_ready):
     cases = mixed_systems(4, seed=9)
     b = Batch(0)
