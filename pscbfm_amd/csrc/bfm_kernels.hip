@@ -432,6 +432,33 @@ __global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__r
     perform_body<kCubic, kGroup>(buf, cb, e.parity, blk);
 }
 
+// Unwrapped coordinates of every monomer in original-id order: the
+// reference's doCopyBack (UpdaterGPUScBFM_AB_Type.cu:3402-3484: unsort, and
+// position + image * box from :1765-1769) done on the device, so that one
+// 12-byte record per monomer crosses to the host.  One lane per sorted slot;
+// padding slots (id word ~0) write nothing.  The sum is taken modulo 2^32, as
+// the int32 result of the 64-bit sum.
+__global__ __launch_bounds__(kBlock) void k_unwrap(const uint32_t *__restrict__ ppos, const uint32_t *__restrict__ ids,
+                                                   const int32_t *__restrict__ image, uint32_t n_pad, uint32_t bx,
+                                                   uint32_t by, uint32_t bz, int32_t *__restrict__ xyz)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n_pad)
+        return;
+    const uint32_t idn = ids[t];
+    if (idn == 0xFFFFFFFFu)
+        return;
+    const size_t id = idn & kIdMask;
+    const uint32_t p = ppos[t];
+    const uint32_t box[3] = {bx, by, bz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t w = (p >> (kCoordBits * uint32_t(a))) & kCoordMask;
+        const uint32_t img = uint32_t(image[size_t(a) * n_pad + t]);
+        xyz[3 * id + size_t(a)] = int32_t(w + img * box[a]);
+    }
+}
+
 // Plain streaming copy, one 16-byte element per lane and one pass of the
 // grid: the reference point for HBM counters (a copy of B bytes reads B and
 // writes B) and for the achievable bandwidth.  On the MI355X this form
@@ -549,6 +576,15 @@ hipError_t launch_perform_batch(uint32_t cls, const BatchSystem *sys, const Bloc
     case 8: launch_perform_batch_group<8>(cubic, grid, sys, blk_tab, sched, n_sys, slot, xcd_ranges, stream); break;
     default: launch_perform_batch_group<4>(cubic, grid, sys, blk_tab, sched, n_sys, slot, xcd_ranges, stream); break;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_unwrap(const SweepBuffers &buf, const uint32_t box[3], int32_t *xyz, hipStream_t stream)
+{
+    if (buf.n_pad == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_unwrap, dim3((buf.n_pad + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, buf.ppos, buf.ids,
+                       buf.image, buf.n_pad, box[0], box[1], box[2], xyz);
     return hipGetLastError();
 }
 

@@ -144,6 +144,10 @@ hipError_t launch_perform_batch(uint32_t cls, const BatchSystem *sys, const Bloc
                                 hipStream_t stream,
                                 uint32_t group = kPerformGroup);
 
+// Unwrapped coordinates (3 int32 per monomer, original-id order) of the
+// system in buf into xyz (3 * monomers entries on the device).
+hipError_t launch_unwrap(const SweepBuffers &buf, const uint32_t box[3], int32_t *xyz, hipStream_t stream);
+
 // Streaming copy of `bytes` (a multiple of 16) from src to dst.
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);
 

@@ -264,7 +264,6 @@ void UpdaterGPUScBFM_AB_Type::initialize()
     mCheckMs = mPerformMs = 0.0;
     mTimedLaunches = 0;
     mLastSortAge = mAge;   // plan() laid the colours out in spatial order
-    mLayoutMoved = false;
     mSorts = 0;
     mInitialized = true;
 }
@@ -284,7 +283,6 @@ void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs)
             mAge != mLastSortAge) {
             mResorter.run(s, mDevSlab, mLayout, mBuf, mStream);
             mLastSortAge = mAge;
-            mLayoutMoved = true;
             ++mSorts;
         }
         const uint64_t age = uint64_t(mAge);
@@ -350,25 +348,19 @@ void UpdaterGPUScBFM_AB_Type::copyBack()
         throw StateError("copy-back before initialize");
     hip_check(hipSetDevice(mGpu), "hipSetDevice");
     HostSystem &s = mSys;
-    hip_check(hipMemcpyAsync(s.ppos_sorted.data(), mBuf.ppos, sizeof(uint32_t) * s.n_pad, hipMemcpyDeviceToHost,
+    // unsort and unwrap on the device (wherever a re-sort has moved the
+    // monomers: the id words travel with them), then one copy of the result
+    if (s.n == 0)
+        return;
+    if (!mDevXyz)
+        hip_check(hipMalloc(&mDevXyz, sizeof(int32_t) * 3 * size_t(s.n)), "allocation of the copy-back buffer");
+    const uint32_t box[3] = {uint32_t(s.box[0]), uint32_t(s.box[1]), uint32_t(s.box[2])};
+    hip_check(launch_unwrap(mBuf, box, mDevXyz, mStream), "unwrap kernel");
+    hip_check(hipMemcpyAsync(s.xyz.data(), mDevXyz, sizeof(int32_t) * 3 * size_t(s.n), hipMemcpyDeviceToHost,
                              mStream),
               "download positions");
-    hip_check(hipMemcpyAsync(s.image_sorted.data(), mBuf.image, sizeof(int32_t) * 3 * size_t(s.n_pad),
-                             hipMemcpyDeviceToHost, mStream),
-              "download images");
-    if (mLayoutMoved)   // a re-sort moved the monomers: read where each one is now
-        hip_check(hipMemcpyAsync(s.ids_sorted.data(), mBuf.ids, sizeof(uint32_t) * s.n_pad, hipMemcpyDeviceToHost,
-                                 mStream),
-                  "download ids");
     hip_check(hipStreamSynchronize(mStream), "hipStreamSynchronize (copy-back)");
     collectTimings();
-    if (mLayoutMoved) {
-        for (uint32_t t = 0; t < s.n_pad; ++t)
-            if (s.ids_sorted[t] != 0xFFFFFFFFu)
-                s.to_sorted[s.ids_sorted[t] & kIdMask] = t;
-        mLayoutMoved = false;
-    }
-    unpack_positions(s);
 }
 
 void UpdaterGPUScBFM_AB_Type::runSimulationOnGPU(uint32_t n_mcs)
@@ -424,6 +416,10 @@ void UpdaterGPUScBFM_AB_Type::freeDevice()
     if (mDevSlab) {
         hipFree(mDevSlab);
         mDevSlab = nullptr;
+    }
+    if (mDevXyz) {
+        hipFree(mDevXyz);
+        mDevXyz = nullptr;
     }
     mBuf = SweepBuffers{};
     mInitialized = false;

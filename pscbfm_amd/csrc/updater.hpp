@@ -121,6 +121,7 @@ private:
     hipStream_t mStream = nullptr;
     SweepBuffers mBuf{};
     void *mDevSlab = nullptr;   // one allocation holds every device array
+    int32_t *mDevXyz = nullptr; // unwrapped coordinates for the copy-back (3 * n), made on first use
     size_t mDevBytes = 0;
     uint64_t mSubsteps = 0;     // substep counter, its parity selects buffers
     uint64_t mAttempts = 0;
@@ -128,7 +129,6 @@ private:
     SlabLayout mLayout{};
     Resorter mResorter;          // on-device spatial re-sort
     int64_t mLastSortAge = 0;    // age of the last sort (initialize sorts)
-    bool mLayoutMoved = false;   // the device order differs from the host's to_sorted
     uint64_t mSorts = 0;
     DeviceAnalyzer mAnalyzer;    // observables and AnalyzerMsd on the device
 
