@@ -324,6 +324,16 @@ def main() -> int:
                    "rng": "int_hash (the reference's default generator)"},
         "roofline": roofline,
     }
+    # the drop-in call as a LeMonADE host makes it: MCS plus the copy-back of
+    # every position to host memory (PCIe-inclusive; reported beside value)
+    pcie_mcs = 100
+    u.runSimulationOnGPU(pcie_mcs)
+    t = time.perf_counter()
+    u.runSimulationOnGPU(pcie_mcs)
+    dt_pcie = time.perf_counter() - t
+    out["pcie_inclusive"] = {"call": f"runSimulationOnGPU({pcie_mcs}) with copy-back", "value": pcie_mcs * n / dt_pcie,
+                             "unit": "attempts/s", "ms_per_call": dt_pcie * 1e3}
+    log(f"runSimulationOnGPU({pcie_mcs}) with copy-back: {dt_pcie * 1e3:.1f} ms -> {pcie_mcs * n / dt_pcie:.4g} attempts/s")
     u.cleanup()
     if not args.no_batched:
         out["batched"] = batched_leg(args)
