@@ -218,6 +218,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1000, help="timed Monte-Carlo steps")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large_melt")
+    ap.add_argument("--layout", choices=["morton", "id", "shuffled"], default="morton",
+                    help="monomer order within a colour: Morton order of position (the default), original id, "
+                         "or original id after a random relabelling of the monomers (input numbering "
+                         "without locality)")
+    ap.add_argument("--no-pcie", action="store_true",
+                    help="skip the PCIe-inclusive call after the timed region (for counter runs)")
     ap.add_argument("--cpu-seconds", type=float, default=24.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--time-every", type=int, default=8,
@@ -255,9 +261,15 @@ def main() -> int:
     t = time.perf_counter()
     system = build_workload(w)
     n = len(system["xyz"])
+    if args.layout == "shuffled":
+        import numpy as np
+        perm = np.random.default_rng(0x5407).permutation(n)        # new id of monomer i
+        xyz = np.empty_like(system["xyz"])
+        xyz[perm] = system["xyz"]
+        system = dict(system, xyz=xyz, bonds=perm[system["bonds"]])
     log(f"generated {args.workload}: {n} monomers in {time.perf_counter() - t:.1f} s")
     t = time.perf_counter()
-    u = updater_from_system(system, seed=w["seed"], gpu=0)
+    u = updater_from_system(system, seed=w["seed"], gpu=0, spatial_sorting=args.layout == "morton")
     colors, n_colors = u.getColors(n)
     log(f"initialized in {time.perf_counter() - t:.1f} s, {n_colors} colours")
 
@@ -298,7 +310,8 @@ def main() -> int:
     # and claim bit, and the clear of the other claim lattice
     b_layout = 8.0 + 8.0 * kbar + 9.0 / 8.0 + p_prop * (8.0 + 1.0 / 8.0) + clear
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(args.workload) if args.layout == "morton" else None,
                 "kernel": "k_check", "kernel_avg_ms": round(check_ms, 4),
                 "perform_avg_ms": round(perform_ms, 4), "bytes_per_attempt_check": round(b_check, 3),
                 "layout_bytes_per_attempt_check": round(b_layout, 3),
@@ -319,21 +332,24 @@ def main() -> int:
         "dtype": "int32",
         "data": "synthetic (seeded random-walk melt)" if w["kind"] == "melt" else
                 "synthetic (seeded diamond network)",
-        "config": {**workload_config(args.workload, w), "monomers": n, "colours": n_colors, "mcs": args.steps,
+        "config": {**workload_config(args.workload, w), "layout": args.layout, "monomers": n, "colours": n_colors,
+                   "mcs": args.steps,
                    "p_acc": round(p_acc, 4), "p_proposal": round(p_prop, 4),
                    "rng": "int_hash (the reference's default generator)"},
         "roofline": roofline,
     }
     # the drop-in call as a LeMonADE host makes it: MCS plus the copy-back of
     # every position to host memory (PCIe-inclusive; reported beside value)
-    pcie_mcs = 100
-    u.runSimulationOnGPU(pcie_mcs)
-    t = time.perf_counter()
-    u.runSimulationOnGPU(pcie_mcs)
-    dt_pcie = time.perf_counter() - t
-    out["pcie_inclusive"] = {"call": f"runSimulationOnGPU({pcie_mcs}) with copy-back", "value": pcie_mcs * n / dt_pcie,
-                             "unit": "attempts/s", "ms_per_call": dt_pcie * 1e3}
-    log(f"runSimulationOnGPU({pcie_mcs}) with copy-back: {dt_pcie * 1e3:.1f} ms -> {pcie_mcs * n / dt_pcie:.4g} attempts/s")
+    if not args.no_pcie:
+        pcie_mcs = 100
+        u.runSimulationOnGPU(pcie_mcs)
+        t = time.perf_counter()
+        u.runSimulationOnGPU(pcie_mcs)
+        dt_pcie = time.perf_counter() - t
+        out["pcie_inclusive"] = {"call": f"runSimulationOnGPU({pcie_mcs}) with copy-back",
+                                 "value": pcie_mcs * n / dt_pcie, "unit": "attempts/s", "ms_per_call": dt_pcie * 1e3}
+        log(f"runSimulationOnGPU({pcie_mcs}) with copy-back: {dt_pcie * 1e3:.1f} ms -> "
+            f"{pcie_mcs * n / dt_pcie:.4g} attempts/s")
     u.cleanup()
     if not args.no_batched:
         out["batched"] = batched_leg(args)
