@@ -554,16 +554,29 @@ int bfm_member_bytes(bfm_updater *h, uint64_t *bytes)
     });
 }
 
+// The system an updater hands to a batch or a server: planned from the
+// updater's inputs, or, for an initialized updater, re-planned from its
+// positions as of its last copy-back (its own planned arrays were staging for
+// its upload and are gone; the generator is keyed on id and age, so the
+// member continues the updater's trajectory whatever its layout).
+static pscbfm::HostSystem system_for_batch(pscbfm::UpdaterGPUScBFM_AB_Type &u)
+{
+    if (u.initialized()) {
+        pscbfm::HostSystem s(u.system());
+        pscbfm::plan(s);
+        return s;
+    }
+    if (!u.validated())
+        u.validate();
+    return pscbfm::HostSystem(u.system());
+}
+
 int bfm_batch_add(bfm_batch *b, bfm_updater *h, uint32_t *member)
 {
     NEED(b, "batch handle");
     NEED(h, "updater handle");
     NEED(member, "output pointer");
-    return guarded([&] {
-        if (!h->u.validated())
-            h->u.validate();
-        *member = b->e->add(pscbfm::HostSystem(h->u.system()), h->u.seed(), h->u.getAge());
-    });
+    return guarded([&] { *member = b->e->add(system_for_batch(h->u), h->u.seed(), h->u.getAge()); });
 }
 
 int bfm_batch_run(bfm_batch *b, uint32_t n_mcs, uint32_t chunk)
@@ -708,11 +721,7 @@ int bfm_server_submit(bfm_server *s, bfm_updater *h, uint32_t n_mcs, uint64_t *j
     NEED(s, "server handle");
     NEED(h, "updater handle");
     NEED(job, "output pointer");
-    return guarded([&] {
-        if (!h->u.validated())
-            h->u.validate();
-        *job = s->s->submit(h->u.system(), h->u.seed(), h->u.getAge(), n_mcs);
-    });
+    return guarded([&] { *job = s->s->submit(system_for_batch(h->u), h->u.seed(), h->u.getAge(), n_mcs); });
 }
 
 int bfm_server_step(bfm_server *s, uint32_t *pending)

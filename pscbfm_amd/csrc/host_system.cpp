@@ -325,6 +325,16 @@ void plan(HostSystem &s)
     }
 }
 
+void release_upload_staging(HostSystem &s)
+{
+    std::vector<uint32_t>().swap(s.to_sorted);
+    std::vector<uint32_t>().swap(s.ppos_sorted);
+    std::vector<uint32_t>().swap(s.ids_sorted);
+    std::vector<int32_t>().swap(s.image_sorted);
+    std::vector<uint32_t>().swap(s.nbr_sorted);
+    std::vector<uint32_t>().swap(s.lattice);
+}
+
 void unpack_positions(HostSystem &s)
 {
     for (uint32_t i = 0; i < s.n; ++i) {

@@ -74,6 +74,10 @@ uint32_t greedy_colouring(uint32_t n, const std::vector<uint8_t> &nnbr, const st
 // wrapped positions and periodic images in sorted order.
 void unpack_positions(HostSystem &s);
 
+// Frees the sorted per-monomer arrays and the lattice that plan() built for
+// the upload (keeps the colours, the segments and the geometry).
+void release_upload_staging(HostSystem &s);
+
 // Runs every check of initialize() and fills the derived fields.
 // Throws std::invalid_argument / std::runtime_error like the reference.
 void plan(HostSystem &s);

@@ -259,6 +259,10 @@ void UpdaterGPUScBFM_AB_Type::initialize()
         freeDevice();
         throw;
     }
+    // The sorted arrays were staging for the upload: from here on the device
+    // holds the state (the copy-back unsorts and unwraps there) and the sweep
+    // driver and the re-sort read only the colour segments.
+    release_upload_staging(mSys);
     mSubsteps = 0;
     mAttempts = 0;
     mCheckMs = mPerformMs = 0.0;

@@ -131,6 +131,37 @@ def test_server_jobs_match_alone(gpu_ready):
         assert 0 <= t_sub <= t_adm <= t_fin
 
 
+def test_running_updater_joins_a_batch_and_a_server(gpu_ready):
+    """An updater that was initialized and has run (positions copied back, layout re-sorted
+    on the device meanwhile) joins a batch and a server: both continue its trajectory,
+    ending where the updater alone ends."""
+    s = systems.linear_melt(32, 100, 16, seed=88)
+    n = len(s["xyz"])
+    seed = 0x5EED0000000000E7
+    print(f"case: running updater into a batch and a server, n={n}", flush=True)
+    u = updater_from_system(s, seed=seed, gpu=0, steps_between_sortings=10)
+    u.runSimulationOnGPU(30)
+    assert u.getAge() == 30
+    b = Batch(0)
+    m = b.add(u, n)
+    b.run(20, 10)
+    srv = Server(0, hbm_budget=0, chunk=10)
+    job = srv.submit(u, 20, n)
+    rounds = 0
+    while srv.step() > 0:
+        rounds += 1
+        assert rounds < 100
+    u.runSimulationOnGPU(20)
+    want = u.getMonomerPositions(n)
+    u.cleanup()
+    ref, _, _ = oracle_final(s, seed, 0, 50)
+    assert np.array_equal(want, ref)
+    assert b.age(m) == 50
+    assert np.array_equal(b.positions(m), want)
+    assert srv.state(job)[0] == JOB_DONE
+    assert np.array_equal(srv.result(job), want)
+
+
 def test_server_cancel_and_oversized(gpu_ready):
     cases = mixed_systems(3, seed=33)
     srv = Server(0, hbm_budget=0, chunk=50)
