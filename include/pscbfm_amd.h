@@ -273,7 +273,10 @@ typedef struct bfm_batch bfm_batch;
 int bfm_batch_create(bfm_batch **out, int gpu);
 int bfm_batch_destroy(bfm_batch *b);
 /* Adds the system a configured updater describes (setters done; it need not
- * be initialized; it keeps its own copy).  Uses its seed and age. */
+ * be initialized; the batch keeps its own copy).  Uses its seed and age.  An
+ * initialized updater hands over its positions as of its last copy-back
+ * (bfm_run_simulation_on_gpu or bfm_copy_back), so the member continues its
+ * trajectory from there. */
 int bfm_batch_add(bfm_batch *b, bfm_updater *h, uint32_t *member);
 /* Device bytes the updater's system takes as a batch member. */
 int bfm_member_bytes(bfm_updater *h, uint64_t *bytes);
@@ -311,7 +314,8 @@ typedef struct bfm_server bfm_server;
 /* hbm_budget = 0: 80% of the free device memory at creation. */
 int bfm_server_create(bfm_server **out, int gpu, uint64_t hbm_budget, uint32_t chunk_mcs);
 int bfm_server_destroy(bfm_server *s);
-/* Queues the configured updater's system (validated here) for n_mcs MCS.
+/* Queues the configured updater's system (validated here) for n_mcs MCS;
+ * an initialized updater hands over its positions as bfm_batch_add does.
  * A job larger than the whole budget is accepted as BFM_JOB_FAILED. */
 int bfm_server_submit(bfm_server *s, bfm_updater *h, uint32_t n_mcs, uint64_t *job);
 /* One scheduling round; *pending = jobs still queued or running. */
