@@ -274,9 +274,8 @@ int bfm_batch_create(bfm_batch **out, int gpu);
 int bfm_batch_destroy(bfm_batch *b);
 /* Adds the system a configured updater describes (setters done; it need not
  * be initialized; the batch keeps its own copy).  Uses its seed and age.  An
- * initialized updater hands over its positions as of its last copy-back
- * (bfm_run_simulation_on_gpu or bfm_copy_back), so the member continues its
- * trajectory from there. */
+ * initialized updater is copied back first and hands over its current state,
+ * so the member continues its trajectory. */
 int bfm_batch_add(bfm_batch *b, bfm_updater *h, uint32_t *member);
 /* Device bytes the updater's system takes as a batch member. */
 int bfm_member_bytes(bfm_updater *h, uint64_t *bytes);

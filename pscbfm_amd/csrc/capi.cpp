@@ -555,13 +555,14 @@ int bfm_member_bytes(bfm_updater *h, uint64_t *bytes)
 }
 
 // The system an updater hands to a batch or a server: planned from the
-// updater's inputs, or, for an initialized updater, re-planned from its
-// positions as of its last copy-back (its own planned arrays were staging for
-// its upload and are gone; the generator is keyed on id and age, so the
-// member continues the updater's trajectory whatever its layout).
+// updater's inputs, or, for an initialized updater, copied back and re-planned
+// from its current positions (its own planned arrays were staging for its
+// upload and are gone; the generator is keyed on id and age, so the member
+// continues the updater's trajectory whatever its layout).
 static pscbfm::HostSystem system_for_batch(pscbfm::UpdaterGPUScBFM_AB_Type &u)
 {
     if (u.initialized()) {
+        u.copyBack();
         pscbfm::HostSystem s(u.system());
         pscbfm::plan(s);
         return s;

@@ -140,7 +140,8 @@ def test_running_updater_joins_a_batch_and_a_server(gpu_ready):
     seed = 0x5EED0000000000E7
     print(f"case: running updater into a batch and a server, n={n}", flush=True)
     u = updater_from_system(s, seed=seed, gpu=0, steps_between_sortings=10)
-    u.runSimulationOnGPU(30)
+    u.runSimulationOnGPU(20)
+    u.runResident(10)            # no copy-back: the hand-over makes one
     assert u.getAge() == 30
     b = Batch(0)
     m = b.add(u, n)
