@@ -162,8 +162,13 @@ def served_leg(args, capacity: float) -> dict:
     from pscbfm_amd import Server, updater_from_system, workloads
     points = []
     mean_work = None
-    for k, factor in enumerate(args.load_factors):
-        jobs = workloads.job_stream(args.jobs, 1.0, seed=0x5E4E + k)   # arrival times rescaled below
+    # the swept points, then a longer stream at capacity whose throughput is
+    # less dominated by the drain at its end
+    runs = [(factor, args.jobs) for factor in args.load_factors]
+    if args.steady_jobs > 0:
+        runs.append((1.0, args.steady_jobs))
+    for k, (factor, n_jobs) in enumerate(runs):
+        jobs = workloads.job_stream(n_jobs, 1.0, seed=0x5E4E + k)   # arrival times rescaled below
         built = [workloads.build(j) for j in jobs]
         work = np.array([len(x["xyz"]) * j["mcs"] for x, j in zip(built, jobs)], dtype=float)
         mean_work = float(work.mean())
@@ -205,7 +210,7 @@ def served_leg(args, capacity: float) -> dict:
         pt = {"load_factor": factor, "rate_jobs_per_s": round(rate, 2), "jobs": len(jobs),
               "throughput": float(work.sum() / makespan), "p50_latency_s": float(np.percentile(lat, 50)),
               "p99_latency_s": float(np.percentile(lat, 99)), "graphs": graphs, "bit_exact_sampled": f"{exact}/{len(range(0, len(jobs), max(1, len(jobs) // 8)))}"}
-        log(f"served at {factor}x capacity ({rate:.1f} jobs/s): {pt['throughput']:.4g} attempts/s, "
+        log(f"served at {factor}x capacity ({rate:.1f} jobs/s, {len(jobs)} jobs): {pt['throughput']:.4g} attempts/s, "
             f"p50 {pt['p50_latency_s']*1e3:.0f} ms, p99 {pt['p99_latency_s']*1e3:.0f} ms, exact {pt['bit_exact_sampled']}, "
             f"chunk graphs {graphs['launches']} ({graphs['updates']} updated, {graphs['instantiations']} instantiated)")
         points.append(pt)
@@ -234,6 +239,8 @@ def main() -> int:
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--jobs", type=int, default=96, help="jobs per served-load point")
     ap.add_argument("--load-factors", type=float, nargs="*", default=[0.5, 1.0, 2.0])
+    ap.add_argument("--steady-jobs", type=int, default=384,
+                    help="jobs of one more served point at capacity (0 = skip)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=1024,

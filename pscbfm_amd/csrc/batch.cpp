@@ -24,12 +24,16 @@ PinnedPool::~PinnedPool()
 
 size_t PinnedPool::take(size_t bytes)
 {
+    // the smallest free buffer that fits; a buffer's event is queried only
+    // while its copy is pending (once seen complete it stays free)
     size_t best = mBufs.size();
     for (size_t k = 0; k < mBufs.size(); ++k) {
-        const Buf &b = mBufs[k];
+        Buf &b = mBufs[k];
         if (b.held || b.bytes < bytes || (best < mBufs.size() && mBufs[best].bytes <= b.bytes))
             continue;
-        if (hipEventQuery(b.done) == hipSuccess)
+        if (b.pending && hipEventQuery(b.done) == hipSuccess)
+            b.pending = false;
+        if (!b.pending)
             best = k;
     }
     if (best == mBufs.size()) {
@@ -48,6 +52,7 @@ size_t PinnedPool::take(size_t bytes)
 void PinnedPool::record(size_t k, hipStream_t stream)
 {
     hip_check(hipEventRecord(mBufs[k].done, stream), "hipEventRecord (staging)");
+    mBufs[k].pending = true;
 }
 
 bool PinnedPool::ready(size_t k) const
@@ -155,7 +160,7 @@ uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age)
     try {
         // staged through pinned memory, so that admitting a member never
         // waits for the chunks already queued
-        const size_t k = mPinned.take(L.total);
+        const size_t k = mPinned.take(upload_bytes(m->sys));
         m->dev.buf = upload_slab(m->sys, m->slab, L, seed, mStream, mPinned.data(k));
         mPinned.record(k, mStream);
         mPinned.give(k);

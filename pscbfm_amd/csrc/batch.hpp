@@ -52,6 +52,7 @@ private:
         size_t bytes = 0;
         hipEvent_t done = nullptr;
         bool held = false;
+        bool pending = false;   // recorded work not yet seen complete (queried only then)
     };
     std::vector<Buf> mBufs;
 };

@@ -35,14 +35,39 @@ SlabLayout slab_layout(const HostSystem &s)
     return L;
 }
 
+// Partner-table columns in use (the largest partner count); the slab is
+// zeroed first, so the columns above need no copy.
+static uint32_t used_partner_columns(const HostSystem &s)
+{
+    uint32_t cols = 0;
+    for (const uint8_t c : s.nnbr)
+        cols = std::max<uint32_t>(cols, c);
+    return cols;
+}
+
+static size_t staged(size_t bytes) { return (bytes + 15) / 16 * 16; }
+
+size_t upload_bytes(const HostSystem &s)
+{
+    return staged(sizeof(uint32_t) * s.n_pad) * 2 + staged(sizeof(int32_t) * 3 * size_t(s.n_pad)) +
+           staged(sizeof(uint32_t) * used_partner_columns(s) * size_t(s.n_pad)) +
+           staged(sizeof(uint32_t) * s.lattice_words) + staged(sizeof(Tables)) +
+           2 * staged(sizeof(uint32_t) * s.n_colours);
+}
+
 SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, uint64_t seed, hipStream_t stream,
                          void *staging)
 {
+    // with staging: the copied parts packed one after the other (upload_bytes)
+    size_t at = 0;
     auto up = [&](SlabPart p, const void *src, size_t bytes, const char *what) {
+        if (bytes == 0)
+            return;
         if (staging) {
-            void *dst = static_cast<char *>(staging) + L.offset[p];
+            void *dst = static_cast<char *>(staging) + at;
             std::memcpy(dst, src, bytes);
             src = dst;
+            at += staged(bytes);
         }
         hip_check(hipMemcpyAsync(slab_part(slab, L, p), src, bytes, hipMemcpyHostToDevice, stream), what);
     };
@@ -50,7 +75,7 @@ SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, u
     up(kPartPPos, s.ppos_sorted.data(), L.bytes[kPartPPos], "upload positions");
     up(kPartIds, s.ids_sorted.data(), L.bytes[kPartIds], "upload ids");
     up(kPartImage, s.image_sorted.data(), L.bytes[kPartImage], "upload images");
-    up(kPartNbr, s.nbr_sorted.data(), L.bytes[kPartNbr], "upload nbr");
+    up(kPartNbr, s.nbr_sorted.data(), sizeof(uint32_t) * used_partner_columns(s) * size_t(s.n_pad), "upload nbr");
     up(kPartLattice, s.lattice.data(), L.bytes[kPartLattice], "upload lattice");
     up(kPartTables, &s.tables, sizeof(Tables), "upload tables");
     up(kPartSegBegin, s.seg_begin.data(), sizeof(uint32_t) * s.n_colours, "upload segments");

@@ -37,6 +37,9 @@ struct SlabLayout {
 
 // Offsets (256-byte aligned) of every array of a planned system.
 SlabLayout slab_layout(const HostSystem &s);
+// Bytes of host staging memory upload_slab needs for s: the parts it copies,
+// packed (the partner table only up to the largest partner count).
+size_t upload_bytes(const HostSystem &s);
 
 // Zeroes the slab, uploads positions, partners, lattice, tables and colour
 // segments on `stream` (asynchronously: the HostSystem must stay alive until

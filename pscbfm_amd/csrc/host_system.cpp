@@ -307,7 +307,12 @@ void plan(HostSystem &s)
     s.ppos_sorted.assign(s.n_pad, 0u);
     s.ids_sorted.assign(s.n_pad, 0xFFFFFFFFu);
     s.image_sorted.assign(3 * size_t(s.n_pad), 0);
-    s.nbr_sorted.assign(size_t(kMaxConnectivity) * s.n_pad, 0);
+    // slot-major, only the columns in use (the device table has all
+    // kMaxConnectivity; the upload copies these and leaves the rest zero)
+    uint32_t cols = 0;
+    for (const uint8_t c : s.nnbr)
+        cols = std::max<uint32_t>(cols, c);
+    s.nbr_sorted.assign(size_t(cols) * s.n_pad, 0);
     for (uint32_t i = 0; i < s.n; ++i) {
         const uint32_t t = s.to_sorted[i];
         uint32_t w[3];

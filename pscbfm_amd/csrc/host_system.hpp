@@ -41,7 +41,7 @@ struct HostSystem {
     std::vector<uint32_t> ppos_sorted;  // n_pad packed wrapped positions
     std::vector<uint32_t> ids_sorted;   // n_pad: original id | partner count << kIdBits (padding: ~0)
     std::vector<int32_t> image_sorted;  // 3 * n_pad periodic images, axis-major
-    std::vector<uint32_t> nbr_sorted;  // kMaxConnectivity * n_pad, slot-major
+    std::vector<uint32_t> nbr_sorted;  // (largest partner count) * n_pad, slot-major
     std::vector<uint32_t> lattice;     // lattice_words occupancy bits
     double mean_bonds = 0.0;           // mean number of listed partners per monomer
 

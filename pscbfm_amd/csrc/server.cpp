@@ -42,14 +42,15 @@ const Server::Job &Server::job(uint64_t id) const
     return *it->second;
 }
 
-uint64_t Server::submit(const HostSystem &sys, uint64_t seed, int64_t age, uint32_t n_mcs)
+uint64_t Server::submit(HostSystem &&sys, uint64_t seed, int64_t age, uint32_t n_mcs)
 {
     auto j = std::make_unique<Job>();
-    j->sys = sys;
+    j->n = sys.n;
+    j->bytes = BatchEngine::bytesFor(sys);
+    j->sys = std::move(sys);
     j->seed = seed;
     j->age = age;
     j->n_mcs = n_mcs;
-    j->bytes = BatchEngine::bytesFor(j->sys);
     j->t_submit = now();
     const uint64_t id = mNext++;
     if (j->bytes > mBudget) {
@@ -81,6 +82,7 @@ void Server::collect(bool block)
         }
         mEngine.unpack(j.download);
         j.sys.xyz = mEngine.system(j.member).xyz;
+        j.sys.n = j.n;
         j.state = kDone;
         j.t_finish = now();
     }
@@ -97,7 +99,7 @@ uint32_t Server::step()
         if (mInUse + j.bytes > mBudget)
             break;
         try {
-            j.member = mEngine.add(HostSystem(j.sys), j.seed, j.age);
+            j.member = mEngine.add(std::move(j.sys), j.seed, j.age);   // the engine owns it now
             j.state = kRunning;
             j.t_admit = now();
             mInUse += j.bytes;
@@ -139,7 +141,7 @@ uint32_t Server::step()
         Job &j = job(id);
         const uint32_t ran = std::min(mChunk, j.n_mcs - j.done);
         j.done += ran;
-        mAttempts += uint64_t(ran) * j.sys.n;
+        mAttempts += uint64_t(ran) * j.n;
         if (j.done >= j.n_mcs) {
             j.download = mEngine.download(j.member);
             j.age = mEngine.age(j.member);

@@ -31,7 +31,7 @@ public:
 
     Server(int gpu, uint64_t hbm_budget, uint32_t chunk);
 
-    uint64_t submit(const HostSystem &sys, uint64_t seed, int64_t age, uint32_t n_mcs);
+    uint64_t submit(HostSystem &&sys, uint64_t seed, int64_t age, uint32_t n_mcs);
     // One round; returns the number of jobs still queued or running.
     uint32_t step();
     void cancel(uint64_t job);
@@ -54,7 +54,8 @@ public:
 
 private:
     struct Job {
-        HostSystem sys;
+        HostSystem sys;      // until admission; then the result's positions
+        uint32_t n = 0;      // monomers
         uint64_t seed = 0;
         int64_t age = 0;
         uint32_t n_mcs = 0;
