@@ -509,7 +509,9 @@ BatchEngine::Download BatchEngine::download(uint32_t id)
     return d;
 }
 
-void BatchEngine::unpack(const Download &d)
+void BatchEngine::unwrap_positions(uint32_t id) { unpack_positions(mMembers.at(id)->sys); }
+
+void BatchEngine::unpack(const Download &d, bool unwrap)
 {
     HostSystem &s = mMembers.at(d.member)->sys;
     const size_t pos_bytes = sizeof(uint32_t) * s.n_pad, img_bytes = sizeof(int32_t) * 3 * size_t(s.n_pad);
@@ -517,7 +519,8 @@ void BatchEngine::unpack(const Download &d)
     std::memcpy(s.ppos_sorted.data(), src, pos_bytes);
     std::memcpy(s.image_sorted.data(), src + pos_bytes, img_bytes);
     mPinned.give(d.buf);
-    unpack_positions(s);
+    if (unwrap)
+        unpack_positions(s);
 }
 
 void BatchEngine::counters(uint32_t id, uint64_t &accepted, uint64_t &proposals)

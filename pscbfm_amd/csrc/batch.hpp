@@ -89,7 +89,11 @@ public:
     Download download(uint32_t id);
     bool finished(const Download &d) const { return mPinned.ready(d.buf); }
     void wait(const Download &d) const { mPinned.wait(d.buf); }
-    void unpack(const Download &d);
+    // Copies a finished download into the member's host arrays and frees its
+    // staging buffer; with `unwrap` also unsorts and unwraps them into the
+    // member's positions (unwrap_positions does that later on request).
+    void unpack(const Download &d, bool unwrap = true);
+    void unwrap_positions(uint32_t id);
     // Frees a member's device memory; its id is not reused.
     void remove(uint32_t id);
 

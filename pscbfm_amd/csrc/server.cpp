@@ -61,6 +61,7 @@ uint64_t Server::submit(HostSystem &&sys, uint64_t seed, int64_t age, uint32_t n
         j->t_finish = j->t_submit;
     } else if (n_mcs == 0) {
         j->state = kDone;
+        j->unwrapped = true;   // the submitted positions are the result
         j->t_admit = j->t_finish = j->t_submit;
     } else {
         mQueue.push_back(id);
@@ -80,9 +81,9 @@ void Server::collect(bool block)
             left.push_back(mRetiring[k]);
             continue;
         }
-        mEngine.unpack(j.download);
-        j.sys.xyz = mEngine.system(j.member).xyz;
-        j.sys.n = j.n;
+        // the copy is taken now (the staging buffer goes back to the pool);
+        // unsorting and unwrapping wait until someone asks for the result
+        mEngine.unpack(j.download, false);
         j.state = kDone;
         j.t_finish = now();
     }
@@ -187,11 +188,17 @@ Server::State Server::state(uint64_t id) const { return job(id).state; }
 uint32_t Server::mcsDone(uint64_t id) const { return job(id).done; }
 const std::string &Server::error(uint64_t id) const { return job(id).error; }
 
-const HostSystem &Server::result(uint64_t id) const
+const HostSystem &Server::result(uint64_t id)
 {
-    const Job &j = job(id);
+    Job &j = job(id);
     if (j.state != kDone)
         throw std::invalid_argument("server: the job has no result (not done)");
+    if (!j.unwrapped) {
+        mEngine.unwrap_positions(j.member);
+        j.sys.xyz = mEngine.system(j.member).xyz;
+        j.sys.n = j.n;
+        j.unwrapped = true;
+    }
     return j.sys;
 }
 

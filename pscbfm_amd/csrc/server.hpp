@@ -40,7 +40,8 @@ public:
     State state(uint64_t job) const;
     uint32_t mcsDone(uint64_t job) const;
     const std::string &error(uint64_t job) const;
-    const HostSystem &result(uint64_t job) const;
+    // Final positions of a done job (unsorted and unwrapped on the first request).
+    const HostSystem &result(uint64_t job);
     // seconds since the server was created: submission, admission, completion
     void times(uint64_t job, double &submitted, double &admitted, double &finished) const;
 
@@ -56,6 +57,7 @@ private:
     struct Job {
         HostSystem sys;      // until admission; then the result's positions
         uint32_t n = 0;      // monomers
+        bool unwrapped = false;   // sys.xyz holds the final positions
         uint64_t seed = 0;
         int64_t age = 0;
         uint32_t n_mcs = 0;
