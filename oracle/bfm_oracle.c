@@ -287,32 +287,205 @@ int orc_check_system(const orc_system *s, const int32_t *pos, char *msg, int msg
 /* ------------------------------------------------------------------------ */
 /* colouring                                                                  */
 
-int orc_greedy_coloring(uint32_t n, const uint32_t *off, const uint32_t *idx,
-                        uint32_t *color, uint32_t *n_colors)
+/* Partners that count for the colouring: own links as listed, then every
+ * monomer that lists i without i listing it (by increasing id).  Returns the
+ * CSR of that graph in freshly allocated arrays. */
+static int coloring_graph(uint32_t n, const uint32_t *off, const uint32_t *idx, uint32_t **g_off,
+                          uint32_t **g_idx)
 {
-    const uint32_t none = 0xFFFFFFFFu;
-    uint32_t maxc = 0;
-    for (uint32_t i = 0; i < n; ++i)
-        color[i] = none;
+    uint32_t *deg = (uint32_t *)calloc((size_t)n + 1, sizeof(uint32_t));
+    if (!deg)
+        return -1;
     for (uint32_t i = 0; i < n; ++i) {
-        uint8_t used[256];
-        memset(used, 0, sizeof used);
+        deg[i] += off[i + 1] - off[i];
         for (uint32_t k = off[i]; k < off[i + 1]; ++k) {
-            uint32_t c = color[idx[k]];
-            if (c != none && c < 256)
-                used[c] = 1;
+            const uint32_t j = idx[k];
+            int back = 0;
+            for (uint32_t q = off[j]; q < off[j + 1]; ++q)
+                back |= idx[q] == i;
+            if (!back)
+                deg[j] += 1;
         }
-        uint32_t c = 0;
-        while (c < 256 && used[c])
-            ++c;
-        if (c >= 255)
-            return -1;
-        color[i] = c;
-        if (c + 1 > maxc)
-            maxc = c + 1;
     }
-    *n_colors = maxc;
+    uint32_t *o = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 1));
+    uint32_t *put = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 1));
+    if (!o || !put) {
+        free(deg); free(o); free(put);
+        return -1;
+    }
+    o[0] = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        o[i + 1] = o[i] + deg[i];
+    uint32_t *g = (uint32_t *)malloc(sizeof(uint32_t) * (o[n] ? o[n] : 1));
+    if (!g) {
+        free(deg); free(o); free(put);
+        return -1;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        put[i] = o[i];
+        for (uint32_t k = off[i]; k < off[i + 1]; ++k)
+            g[put[i]++] = idx[k];
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t k = off[i]; k < off[i + 1]; ++k) {
+            const uint32_t j = idx[k];
+            int back = 0;
+            for (uint32_t q = off[j]; q < off[j + 1]; ++q)
+                back |= idx[q] == i;
+            if (!back)
+                g[put[j]++] = i;
+        }
+    free(deg);
+    free(put);
+    *g_off = o;
+    *g_idx = g;
     return 0;
+}
+
+#define ORC_NO_COLOR 0xFFFFFFFFu
+#define ORC_COLOR_LIMIT 255u
+
+/* uncoloured partners of v (each listing counts), the first of them in *first */
+static uint32_t uncolored_partners(const uint32_t *o, const uint32_t *g, const uint32_t *color, uint32_t v,
+                                   uint32_t *first)
+{
+    uint32_t count = 0;
+    *first = ORC_NO_COLOR;
+    for (uint32_t k = o[v]; k < o[v + 1]; ++k) {
+        if (color[g[k]] != ORC_NO_COLOR)
+            continue;
+        if (count == 0)
+            *first = g[k];
+        ++count;
+    }
+    return count;
+}
+
+int orc_graph_coloring(uint32_t n, const uint32_t *off, const uint32_t *idx, int uniform,
+                       uint32_t *color, uint32_t *n_colors)
+{
+    uint32_t *o = NULL, *g = NULL;
+    if (coloring_graph(n, off, idx, &o, &g) != 0)
+        return -1;
+    uint64_t freq[256];
+    memset(freq, 0, sizeof freq);
+    uint32_t ncol = 0;
+    uint32_t *todo = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    if (!todo) {
+        free(o); free(g);
+        return -1;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        color[i] = ORC_NO_COLOR;
+    int rc = 0;
+    for (uint32_t start = 0; start < n && rc == 0; ++start) {
+        if (color[start] != ORC_NO_COLOR)
+            continue;
+        uint32_t top = 0;          /* stack of monomers with uncoloured partners left */
+        uint32_t cur = start;
+        for (;;) {
+            uint32_t first;
+            const uint32_t open = uncolored_partners(o, g, color, cur, &first);
+            unsigned char seen[256];
+            memset(seen, 0, sizeof seen);
+            for (uint32_t k = o[cur]; k < o[cur + 1]; ++k)
+                if (color[g[k]] != ORC_NO_COLOR)
+                    seen[color[g[k]]] = 1;
+            uint32_t c = 0;
+            while (c < ORC_COLOR_LIMIT && seen[c])
+                ++c;
+            if (c == ORC_COLOR_LIMIT) {
+                rc = -1;
+                break;
+            }
+            color[cur] = c;
+            freq[c] += 1;
+            if (c + 1 > ncol)
+                ncol = c + 1;
+            if (open == 0 && top == 0)
+                break;
+            if (open > 1)
+                todo[top++] = cur;
+            if (open > 0) {
+                cur = first;
+                continue;
+            }
+            /* dead end: back to the newest stacked monomer that still has an
+             * uncoloured partner (one with fewer than two leaves the stack) */
+            cur = ORC_NO_COLOR;
+            while (top > 0) {
+                uint32_t f;
+                const uint32_t left = uncolored_partners(o, g, color, todo[top - 1], &f);
+                if (left < 2)
+                    --top;
+                if (f != ORC_NO_COLOR) {
+                    cur = f;
+                    break;
+                }
+            }
+            if (cur == ORC_NO_COLOR)
+                break;
+        }
+    }
+    free(todo);
+    if (rc == 0 && uniform && ncol > 0) {
+        /* ranking of the colours by use, rarest first, ties in colour order;
+         * kept sorted by insertion after every change */
+        uint32_t rank_color[256];
+        uint64_t rank_count[256];
+        for (uint32_t c = 0; c < ncol; ++c) {
+            uint32_t at = c;
+            while (at > 0 && rank_count[at - 1] > freq[c]) {
+                rank_color[at] = rank_color[at - 1];
+                rank_count[at] = rank_count[at - 1];
+                --at;
+            }
+            rank_color[at] = c;
+            rank_count[at] = freq[c];
+        }
+        const uint64_t target = ((uint64_t)n + ncol - 1) / ncol;
+        for (uint32_t v = 0; v < n; ++v) {
+            if (freq[color[v]] <= target)
+                continue;
+            for (uint32_t r = 0; r < ncol && rank_count[r] < target; ++r) {
+                const uint32_t nc = rank_color[r];
+                int ok = 1;
+                for (uint32_t k = o[v]; k < o[v + 1]; ++k)
+                    if (color[g[k]] == nc)
+                        ok = 0;
+                if (!ok)
+                    continue;
+                const uint32_t oc = color[v];
+                freq[oc] -= 1;
+                freq[nc] += 1;
+                for (uint32_t q = 0; q < ncol; ++q) {
+                    if (rank_color[q] == oc)
+                        rank_count[q] -= 1;
+                    if (rank_color[q] == nc)
+                        rank_count[q] += 1;
+                }
+                /* stable re-sort: adjacent exchanges of out-of-order pairs */
+                for (int swapped = 1; swapped;) {
+                    swapped = 0;
+                    for (uint32_t q = 0; q + 1 < ncol; ++q)
+                        if (rank_count[q] > rank_count[q + 1]) {
+                            const uint32_t tc = rank_color[q];
+                            const uint64_t tn = rank_count[q];
+                            rank_color[q] = rank_color[q + 1];
+                            rank_count[q] = rank_count[q + 1];
+                            rank_color[q + 1] = tc;
+                            rank_count[q + 1] = tn;
+                            swapped = 1;
+                        }
+                }
+                color[v] = nc;
+            }
+        }
+    }
+    free(o);
+    free(g);
+    *n_colors = ncol;
+    return rc;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -16,7 +16,7 @@
  *   bond index        src/pscbfm/UpdaterGPUScBFM_AB_Type.cu:871-889
  *   direction table   src/pscbfm/UpdaterGPUScBFM_AB_Type.cu:1537-1541
  *   invariants        src/pscbfm/UpdaterGPUScBFM_AB_Type.cu:2789-2900
- *   colouring         src/pscbfm/graphColoring.tpp:131-294
+ *   colouring         src/pscbfm/graphColoring.tpp:131-294, :323-418
  *   integer-hash RNG  src/pscbfm/rngs/Hash.h:39-44,64-68 (default, .cu:1407)
  *
  * Parity status of each function is stated next to its declaration.
@@ -95,11 +95,24 @@ void orc_standard_bondset(uint8_t forbidden[512]);
  * Returns 0 if valid; otherwise a positive code and a message. */
 int orc_check_system(const orc_system *sys, const int32_t *pos, char *msg, int msglen);
 
-/* Greedy sequential colouring: monomers in increasing id, each takes the
- * smallest colour no already-coloured neighbour has.  Returns 0 or -1 when
- * more than 255 colours would be needed. */
-int orc_greedy_coloring(uint32_t n, const uint32_t *nbr_off, const uint32_t *nbr_idx,
-                        uint32_t *color, uint32_t *n_colors);
+/* The reference's graph colouring, graphColoring.tpp:103-439, as
+ * initializeSpeciesSorting calls it (UpdaterGPUScBFM_AB_Type.cu:1561-1566,
+ * bUniformColors = true): each connected polymer is walked from its lowest id
+ * (go to the first uncoloured partner, keep a stack of monomers with two or
+ * more, resume from the stack at a dead end), every monomer taking the
+ * lowest colour none of its partners has; with `uniform` a second pass in id
+ * order moves monomers of over-full colours (more than ceil(n/k)) to the
+ * rarest colours their partners do not have.
+ * (nbr_off, nbr_idx): each monomer's links in the order they were set
+ * (directed, as setConnectivity receives them); a link whose reverse is not
+ * listed also counts for its target, appended after that monomer's own list
+ * in increasing id.  Returns 0, or -1 when 255 colours do not suffice.
+ * Pinned by colourings that the reference's own graphColoring.tpp, compiled
+ * into oracle/_ref/coloring_ref, produced for the graphs in
+ * tests/golden/coloring_vectors.json; by 2 colours on bipartite graphs; by
+ * the chromatic number of complete graphs and odd cycles. */
+int orc_graph_coloring(uint32_t n, const uint32_t *nbr_off, const uint32_t *nbr_idx, int uniform,
+                       uint32_t *color, uint32_t *n_colors);
 
 /* Serial replay of the colour sweep: for every MCS, every colour in
  * orc_color_order, (1) every monomer of the colour proposes a move, checked

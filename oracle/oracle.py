@@ -66,8 +66,8 @@ def load() -> ctypes.CDLL:
         lib.orc_standard_bondset.restype = None
         lib.orc_check_system.argtypes = [POINTER(OrcSystem), c_void_p, c_char_p, c_int]
         lib.orc_check_system.restype = c_int
-        lib.orc_greedy_coloring.argtypes = [c_uint32, c_void_p, c_void_p, c_void_p, POINTER(c_uint32)]
-        lib.orc_greedy_coloring.restype = c_int
+        lib.orc_graph_coloring.argtypes = [c_uint32, c_void_p, c_void_p, c_int, c_void_p, POINTER(c_uint32)]
+        lib.orc_graph_coloring.restype = c_int
         lib.orc_sweep.argtypes = [POINTER(OrcSystem), c_void_p, c_uint64, c_uint32, c_int,
                                   POINTER(c_uint64), POINTER(c_uint64)]
         lib.orc_sweep.restype = c_int
@@ -141,8 +141,25 @@ def directed_csr(n: int, links: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     return off, dst.astype(np.uint32)
 
 
+def call_order_links(system: dict) -> np.ndarray:
+    """Directed links in the order the LeMonADE-style set-up issues them
+    (GPUScBFM_AB_Type.h:159-161 sets both directions of every bond): the
+    system's "links" as given, else (a, b), (b, a) for every bond (a, b) in
+    turn -- the order pscbfm_amd.updater_from_system uses."""
+    if system.get("links") is not None:
+        return np.asarray(system["links"], dtype=np.int64).reshape(-1, 2)
+    b = np.asarray(system["bonds"], dtype=np.int64).reshape(-1, 2)
+    both = np.empty((2 * b.shape[0], 2), dtype=np.int64)
+    both[0::2] = b
+    both[1::2] = b[:, ::-1]
+    return both
+
+
 class System:
-    """Holds the arrays behind an OrcSystem so they outlive the struct."""
+    """Holds the arrays behind an OrcSystem so they outlive the struct.
+
+    Without `colors`, the monomers are coloured by the oracle's restatement of
+    the reference's colouring over the links in call order."""
 
     def __init__(self, system: dict, seed: int, colors: np.ndarray | None = None, rng: int = RNG_INT_HASH):
         self.xyz0 = np.ascontiguousarray(system["xyz"], dtype=np.int32).reshape(-1, 3)
@@ -153,7 +170,8 @@ class System:
         else:
             self.off, self.idx = csr(n, system["bonds"])
         if colors is None:
-            colors, nc = greedy_coloring(n, self.off, self.idx)
+            c_off, c_idx = directed_csr(n, call_order_links(system))
+            colors, nc = graph_coloring(n, c_off, c_idx)
         else:
             colors = np.ascontiguousarray(colors, dtype=np.uint32)
             nc = int(colors.max()) + 1 if n else 1
@@ -175,10 +193,15 @@ class System:
         self.n_colors = nc
 
 
-def greedy_coloring(n: int, off: np.ndarray, idx: np.ndarray) -> tuple[np.ndarray, int]:
+def graph_coloring(n: int, off: np.ndarray, idx: np.ndarray, uniform: bool = True) -> tuple[np.ndarray, int]:
+    """The reference's colouring (graphColoring.tpp) of the directed link
+    lists (off, idx) in call order; uniform = the rebalance the updater uses."""
+    off = np.ascontiguousarray(off, dtype=np.uint32)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32) if len(idx) else np.zeros(1, dtype=np.uint32)
     col = np.zeros(max(n, 1), dtype=np.uint32)
     nc = c_uint32()
-    rc = load().orc_greedy_coloring(n, off.ctypes.data, idx.ctypes.data, col.ctypes.data, ctypes.byref(nc))
+    rc = load().orc_graph_coloring(n, off.ctypes.data, idx.ctypes.data, int(bool(uniform)), col.ctypes.data,
+                                   ctypes.byref(nc))
     if rc != 0:
         raise RuntimeError("oracle colouring needs more than 255 colours")
     return col[:n], nc.value

@@ -123,46 +123,146 @@ uint32_t host_site(const Geometry &g, const Tables &t, int32_t x, int32_t y, int
     return s;
 }
 
-uint32_t greedy_colouring(uint32_t n, const std::vector<uint8_t> &nnbr, const std::vector<uint32_t> &nbr,
-                          std::vector<uint32_t> &colour)
+void colouring_adjacency(uint32_t n, const std::vector<uint8_t> &nnbr, const std::vector<uint32_t> &nbr,
+                         std::vector<uint32_t> &off, std::vector<uint32_t> &adj)
 {
-    // symmetrised adjacency in CSR form
-    std::vector<uint32_t> deg(n + 1, 0);
+    // links i -> j whose reverse j -> i is not listed: j gets i appended
+    auto lists = [&](uint32_t a, uint32_t b) {
+        for (uint32_t k = 0; k < nnbr[a]; ++k)
+            if (nbr[size_t(a) * kMaxConnectivity + k] == b)
+                return true;
+        return false;
+    };
+    std::vector<uint32_t> extra_count(n, 0);
+    std::vector<std::pair<uint32_t, uint32_t>> extra;   // (monomer, appended partner), by partner id
     for (uint32_t i = 0; i < n; ++i)
-        for (uint32_t j = 0; j < nnbr[i]; ++j) {
-            const uint32_t p = nbr[size_t(i) * kMaxConnectivity + j];
-            ++deg[i + 1];
-            ++deg[p + 1];
+        for (uint32_t k = 0; k < nnbr[i]; ++k) {
+            const uint32_t j = nbr[size_t(i) * kMaxConnectivity + k];
+            if (!lists(j, i)) {
+                extra.emplace_back(j, i);
+                ++extra_count[j];
+            }
         }
+    off.assign(size_t(n) + 1, 0);
     for (uint32_t i = 0; i < n; ++i)
-        deg[i + 1] += deg[i];
-    std::vector<uint32_t> adj(deg[n]);
-    std::vector<uint32_t> fill(deg.begin(), deg.end() - 1);
+        off[i + 1] = off[i] + nnbr[i] + extra_count[i];
+    adj.assign(off[n], 0);
+    std::vector<uint32_t> fill(off.begin(), off.end() - 1);
     for (uint32_t i = 0; i < n; ++i)
-        for (uint32_t j = 0; j < nnbr[i]; ++j) {
-            const uint32_t p = nbr[size_t(i) * kMaxConnectivity + j];
-            adj[fill[i]++] = p;
-            adj[fill[p]++] = i;
-        }
-    const uint32_t none = 0xFFFFFFFFu;
-    colour.assign(n, none);
-    uint32_t n_colours = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        uint64_t used[4] = {0, 0, 0, 0};
-        for (uint32_t k = deg[i]; k < deg[i + 1]; ++k) {
+        for (uint32_t k = 0; k < nnbr[i]; ++k)
+            adj[fill[i]++] = nbr[size_t(i) * kMaxConnectivity + k];
+    for (const auto &e : extra)
+        adj[fill[e.first]++] = e.second;
+}
+
+uint32_t graph_colouring(uint32_t n, const std::vector<uint8_t> &nnbr, const std::vector<uint32_t> &nbr,
+                         bool uniform, std::vector<uint32_t> &colour)
+{
+    std::vector<uint32_t> off, adj;
+    colouring_adjacency(n, nnbr, nbr, off, adj);
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    constexpr uint32_t kColourLimit = 255;   // graphColoring.tpp:114: at most 255 colours
+    colour.assign(n, kNone);
+    std::vector<uint64_t> used_by;           // monomers per colour
+
+    // Walk of graphColoring.tpp:131-294.  Each connected polymer is coloured
+    // from its lowest id by following links: a monomer takes the lowest
+    // colour none of its partners has; the walk goes on to its first
+    // uncoloured partner and remembers the monomer while it has two or more;
+    // at a dead end it resumes from the most recent remembered monomer that
+    // still has an uncoloured partner.
+    auto open_partners = [&](uint32_t v, uint32_t &first) {
+        uint32_t open = 0;
+        first = kNone;
+        for (uint32_t k = off[v]; k < off[v + 1]; ++k)
+            if (colour[adj[k]] == kNone) {
+                if (first == kNone)
+                    first = adj[k];
+                ++open;
+            }
+        return open;
+    };
+    auto lowest_free = [&](uint32_t v) {
+        uint64_t taken[4] = {0, 0, 0, 0};
+        for (uint32_t k = off[v]; k < off[v + 1]; ++k) {
             const uint32_t c = colour[adj[k]];
-            if (c != none)
-                used[c >> 6] |= uint64_t(1) << (c & 63);
+            if (c != kNone)
+                taken[c >> 6] |= uint64_t(1) << (c & 63u);
         }
         uint32_t c = 0;
-        while (c < 256 && ((used[c >> 6] >> (c & 63)) & 1u))
+        while (c < kColourLimit && ((taken[c >> 6] >> (c & 63u)) & 1u))
             ++c;
-        if (c >= 255)
+        if (c == kColourLimit)
             throw std::runtime_error("graph colouring needs more than 255 colours");
-        colour[i] = c;
-        n_colours = std::max(n_colours, c + 1);
+        return c;
+    };
+    std::vector<uint32_t> remembered;
+    for (uint32_t root = 0; root < n; ++root) {
+        if (colour[root] != kNone)
+            continue;
+        remembered.clear();
+        uint32_t v = root;
+        while (v != kNone) {
+            uint32_t next = kNone;
+            const uint32_t open = open_partners(v, next);
+            const uint32_t c = lowest_free(v);
+            colour[v] = c;
+            if (c >= used_by.size())
+                used_by.resize(c + 1, 0);
+            ++used_by[c];
+            if (open > 1)
+                remembered.push_back(v);
+            v = next;
+            while (v == kNone && !remembered.empty()) {
+                const uint32_t u = remembered.back();
+                if (open_partners(u, v) < 2)
+                    remembered.pop_back();
+            }
+        }
     }
-    return n_colours;
+    const uint32_t k_colours = uint32_t(used_by.size());
+
+    // Uniform rebalance of graphColoring.tpp:323-418 (switched on by
+    // initializeSpeciesSorting, UpdaterGPUScBFM_AB_Type.cu:1565): in id order,
+    // a monomer of a colour used more than ceil(n / k) times tries the colours
+    // used less than that, rarest first, and takes each one none of its
+    // partners has; the ranking is re-sorted (stably) after every change, and
+    // the monomer goes on through the ranking after a change, as the
+    // reference's loop does.
+    if (uniform && k_colours > 1) {
+        const uint64_t target = (uint64_t(n) + k_colours - 1) / k_colours;
+        std::vector<std::pair<uint32_t, uint64_t>> rank(k_colours);   // (colour, monomers)
+        for (uint32_t c = 0; c < k_colours; ++c)
+            rank[c] = {c, used_by[c]};
+        auto by_count = [](const std::pair<uint32_t, uint64_t> &a, const std::pair<uint32_t, uint64_t> &b) {
+            return a.second < b.second;
+        };
+        std::stable_sort(rank.begin(), rank.end(), by_count);
+        for (uint32_t v = 0; v < n; ++v) {
+            if (used_by[colour[v]] <= target)
+                continue;
+            for (size_t r = 0; r < rank.size() && rank[r].second < target; ++r) {
+                const uint32_t want = rank[r].first;
+                bool clash = false;
+                for (uint32_t k = off[v]; k < off[v + 1] && !clash; ++k)
+                    clash = colour[adj[k]] == want;
+                if (clash)
+                    continue;
+                const uint32_t had = colour[v];
+                --used_by[had];
+                ++used_by[want];
+                for (auto &e : rank) {
+                    if (e.first == had)
+                        --e.second;
+                    if (e.first == want)
+                        ++e.second;
+                }
+                std::stable_sort(rank.begin(), rank.end(), by_count);
+                colour[v] = want;
+            }
+        }
+    }
+    return k_colours;
 }
 
 void plan(HostSystem &s)
@@ -262,7 +362,7 @@ void plan(HostSystem &s)
     s.mean_bonds = double(n_links) / double(s.n);
 
     // colouring and colour-sorted layout (initializeSpeciesSorting, :1561-1686)
-    s.n_colours = greedy_colouring(s.n, s.nnbr, s.nbr, s.colour);
+    s.n_colours = graph_colouring(s.n, s.nnbr, s.nbr, true, s.colour);
     s.seg_count.assign(s.n_colours, 0);
     for (uint32_t i = 0; i < s.n; ++i)
         ++s.seg_count[s.colour[i]];

@@ -64,11 +64,23 @@ void make_geometry(const uint64_t box[3], const bool periodic[3], Geometry &g, T
                    uint64_t &lattice_words);
 uint32_t host_site(const Geometry &g, const Tables &t, int32_t x, int32_t y, int32_t z);
 
-// Greedy sequential colouring over the symmetrised bond graph: monomers in
-// increasing id take the smallest colour none of their already coloured
-// partners has.  Throws std::runtime_error above 255 colours.
-uint32_t greedy_colouring(uint32_t n, const std::vector<uint8_t> &nnbr, const std::vector<uint32_t> &nbr,
-                          std::vector<uint32_t> &colour);
+// Colouring graph of the bond lists: every monomer's listed partners in the
+// order they were listed (setConnectivity), then the monomers that list it
+// without being listed by it (in increasing id), so that a one-way link still
+// keeps its two ends apart.  For symmetric lists, as the LeMonADE adapter
+// sets them (GPUScBFM_AB_Type.h:159-161), these are the lists themselves.
+void colouring_adjacency(uint32_t n, const std::vector<uint8_t> &nnbr, const std::vector<uint32_t> &nbr,
+                         std::vector<uint32_t> &off, std::vector<uint32_t> &adj);
+
+// The reference's graph colouring (graphColoring.tpp:103-439, called with
+// bUniformColors = true by initializeSpeciesSorting,
+// UpdaterGPUScBFM_AB_Type.cu:1561-1566): a walk along each connected polymer
+// giving every monomer the lowest colour its partners do not have, then (with
+// `uniform`) a rebalance toward ceil(n / colours) monomers per colour.
+// A bipartite bond graph gets 2 colours.  Returns the colour count; throws
+// std::runtime_error above 255 colours.
+uint32_t graph_colouring(uint32_t n, const std::vector<uint8_t> &nnbr, const std::vector<uint32_t> &nbr,
+                         bool uniform, std::vector<uint32_t> &colour);
 
 // Unwrapped coordinates of every monomer (by original id) from the packed
 // wrapped positions and periodic images in sorted order.

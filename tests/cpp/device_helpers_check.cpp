@@ -8,6 +8,7 @@
 //    the array) against the full shuffle colour_order.
 // Prints "ok <cases>" and exits 0 on success.
 #include <cstdio>
+#include <cstring>
 #include <cstdlib>
 #include <set>
 
@@ -70,8 +71,39 @@ bool check_one(const uint64_t box[3], const Geometry &g, const Tables &t, const 
 
 }  // namespace
 
-int main()
+// "colour" mode: the same graph format as oracle/ref_coloring_driver.cpp on
+// stdin ("uniform n", then n lines "k j_1 ... j_k"), the product's colouring
+// printed as "n_colours c_0 ... c_{n-1}" per graph.
+static int colour_graphs()
 {
+    int uniform = 0;
+    unsigned long n = 0;
+    while (std::scanf("%d %lu", &uniform, &n) == 2) {
+        std::vector<uint8_t> nnbr(n, 0);
+        std::vector<uint32_t> nbr(n * kMaxConnectivity, 0);
+        for (unsigned long i = 0; i < n; ++i) {
+            unsigned k = 0;
+            if (std::scanf("%u", &k) != 1 || k > kMaxConnectivity)
+                return 1;
+            nnbr[i] = uint8_t(k);
+            for (unsigned q = 0; q < k; ++q)
+                if (std::scanf("%u", &nbr[i * kMaxConnectivity + q]) != 1)
+                    return 1;
+        }
+        std::vector<uint32_t> colour;
+        const uint32_t k = graph_colouring(uint32_t(n), nnbr, nbr, uniform != 0, colour);
+        std::printf("%u", k);
+        for (uint32_t c : colour)
+            std::printf(" %u", c);
+        std::printf("\n");
+    }
+    return 0;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc > 1 && std::strcmp(argv[1], "colour") == 0)
+        return colour_graphs();
     const uint64_t boxes[][3] = {{8, 8, 8},    {16, 16, 16}, {8, 16, 32},    {32, 8, 16},  {16, 64, 8},
                                  {64, 64, 64}, {512, 512, 512}, {1024, 8, 8}, {8, 1024, 64}, {128, 256, 1024}};
     const bool periodic[3] = {true, true, true};

@@ -198,7 +198,7 @@ void check_oracle(const Sys &s, uint64_t seed)
     }
     std::vector<uint32_t> colour(n);
     uint32_t n_colours = 0;
-    expect(orc_greedy_coloring(n, off.data(), idx.data(), colour.data(), &n_colours) == 0, "oracle colouring");
+    expect(orc_graph_coloring(n, off.data(), idx.data(), 1, colour.data(), &n_colours) == 0, "oracle colouring");
     uint8_t forbidden[512];
     orc_standard_bondset(forbidden);
     orc_system sys{};

@@ -77,6 +77,48 @@ def test_validate_accepts_melt_and_colours_like_oracle():
     assert np.array_equal(col, o.colors)
 
 
+def test_product_colouring_matches_reference_vectors():
+    """The product's colouring (host_system.cpp graph_colouring, run by
+    tests/cpp/device_helpers_check) equals the reference's own graphColoring.tpp
+    on every golden graph, with and without the uniform rebalance."""
+    import json
+    import subprocess
+    exe = Path(__file__).resolve().parent / "cpp" / "device_helpers_check"
+    assert exe.exists(), "built by __graft_entry__.build()"
+    graphs = json.loads((Path(__file__).resolve().parent / "golden" / "coloring_vectors.json").read_text())["graphs"]
+    text = "".join(f"{g['uniform']} {len(g['lists'])}\n" +
+                   "".join(f"{len(l)} " + " ".join(map(str, l)) + "\n" for l in g["lists"]) for g in graphs)
+    out = subprocess.run([str(exe), "colour"], input=text, capture_output=True, text=True, timeout=60, check=True)
+    rows = out.stdout.strip().split("\n")
+    assert len(rows) == len(graphs)
+    for g, row in zip(graphs, rows):
+        assert [int(v) for v in row.split()] == [g["n_colors"]] + g["colors"], (g["name"], g["uniform"])
+
+
+@pytest.mark.parametrize("kind", ["network", "cluster_gel", "odd_chains", "one_way"])
+def test_validate_colours_like_oracle(kind):
+    """initialize()'s colouring equals the oracle's on generated systems: a
+    diamond network (2 colours: the bond graph is bipartite), tetrahedral
+    clusters with chains (4 colours, rebalanced), chains of odd length and
+    one-way links (each link keeps both ends apart)."""
+    if kind == "network":
+        s, want = systems.diamond_network(64, (2, 2, 8), 32, seed=3), 2
+    elif kind == "cluster_gel":
+        s, want = systems.cluster_gel(32, 0.8, 0.5, seed=4), 4
+    elif kind == "odd_chains":
+        s, want = systems.linear_melt(32, 60, 7, seed=6), 2
+    else:
+        s = systems.linear_melt(32, 40, 8, seed=7)
+        b = s["bonds"]
+        s = dict(s, links=np.concatenate([b[0::2], b[1::2, ::-1]]))    # every bond listed once
+        want = 2
+    u = updater_from_system(s, seed=1, validate_only=True)
+    col, nc = u.getColors(len(s["xyz"]))
+    o = orc.System(s, 1)
+    assert nc == o.n_colors == want
+    assert np.array_equal(col, o.colors)
+
+
 def test_validate_rejects_wrong_bond_set():
     s = valid_melt()
     f = s["forbidden"].copy()

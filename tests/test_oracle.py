@@ -3,6 +3,7 @@ published Philox4x32-10 known-answer vectors, the textbook BFM bond set, the
 reference's own front-cell enumeration, brute force and closed forms.
 """
 import itertools
+import json
 import shutil
 import subprocess
 from pathlib import Path
@@ -13,6 +14,7 @@ import pytest
 from oracle import oracle as orc
 
 ROOT = Path(__file__).resolve().parents[1]
+GOLDEN = ROOT / "tests" / "golden"
 
 
 # ---- Philox4x32-10 ---------------------------------------------------------
@@ -215,18 +217,124 @@ def test_check_system_catches_planted_errors():
     assert orc.check_system(same, x)[0] != 0     # bonded monomers sharing a colour
 
 
-def test_greedy_colouring_alternates_chains():
+def test_colouring_alternates_chains():
     s = melt(16, 16, 16, 3)
     sys_ = orc.System(s, 1)
     assert sys_.n_colors == 2
     assert np.array_equal(sys_.colors, np.tile([0, 1], len(s["xyz"]) // 2))
 
 
+def _golden_colourings():
+    return json.loads((GOLDEN / "coloring_vectors.json").read_text())["graphs"]
+
+
+def _lists_csr(lists):
+    off = np.zeros(len(lists) + 1, dtype=np.uint32)
+    off[1:] = np.cumsum([len(l) for l in lists])
+    idx = np.array([j for l in lists for j in l], dtype=np.uint32)
+    return off, idx
+
+
+def test_colouring_matches_reference_vectors():
+    """The oracle's colouring equals what the reference's own graphColoring.tpp
+    (compiled into oracle/_ref/coloring_ref) gave for every golden graph, with
+    and without the uniform rebalance."""
+    graphs = _golden_colourings()
+    assert len(graphs) >= 40
+    for g in graphs:
+        off, idx = _lists_csr(g["lists"])
+        col, nc = orc.graph_coloring(len(g["lists"]), off, idx, uniform=bool(g["uniform"]))
+        assert nc == g["n_colors"], g["name"]
+        assert col.tolist() == g["colors"], (g["name"], g["uniform"])
+
+
+def test_colouring_against_live_reference():
+    """Where oracle/_ref is built: random graphs fresh from the reference's
+    colouring, including random link orders and one-per-monomer degrees up to 8."""
+    import subprocess
+    exe = ROOT / "oracle" / "_ref" / "coloring_ref"
+    if not exe.exists():
+        pytest.skip("oracle/_ref/coloring_ref is built only where the reference sources are present")
+    rng = np.random.default_rng(77)
+    for trial in range(40):
+        n = int(rng.integers(2, 300))
+        deg = np.zeros(n, dtype=int)
+        lists = [[] for _ in range(n)]
+        seen = set()
+        for _ in range(int(rng.integers(0, 4 * n))):
+            a, b = (int(v) for v in rng.integers(0, n, 2))
+            if a == b or (min(a, b), max(a, b)) in seen or deg[a] >= 8 or deg[b] >= 8:
+                continue
+            seen.add((min(a, b), max(a, b)))
+            lists[a].append(b)
+            lists[b].append(a)
+            deg[a] += 1
+            deg[b] += 1
+        for l in lists:
+            rng.shuffle(l)
+        for uniform in (1, 0):
+            text = f"{uniform} {n}\n" + "".join(f"{len(l)} " + " ".join(map(str, l)) + "\n" for l in lists)
+            ref = subprocess.run([str(exe)], input=text, capture_output=True, text=True, check=True).stdout.split()
+            off, idx = _lists_csr(lists)
+            col, nc = orc.graph_coloring(n, off, idx, uniform=bool(uniform))
+            assert [nc] + col.tolist() == [int(v) for v in ref], (trial, uniform)
+
+
+def _proper(n, off, idx, col):
+    return all(col[i] != col[j] for i in range(n) for j in idx[off[i]:off[i + 1]])
+
+
+def test_colouring_fixed_by_mathematics():
+    """Without reference output: the walk colours a bipartite graph with exactly
+    2 colours (trees, even rings, a diamond network); a complete graph K_k needs
+    k; an odd ring 3; every colouring is proper, also after the rebalance; the
+    rebalance never adds a colour and leaves no colour above ceil(n/k) when
+    a free swap exists (K4 clusters mixed with chains)."""
+    rng = np.random.default_rng(5)
+    # bipartite: random trees with shuffled ids
+    for t in range(20):
+        n = int(rng.integers(2, 200))
+        bonds = []
+        deg = np.zeros(n, dtype=int)
+        for i in range(1, n):
+            for _ in range(20):
+                p = int(rng.integers(0, i))
+                if deg[p] < 8:
+                    break
+            bonds.append((i, p))
+            deg[i] += 1
+            deg[p] += 1
+        perm = rng.permutation(n)
+        links = np.array([(perm[a], perm[b]) for a, b in bonds] + [(perm[b], perm[a]) for a, b in bonds])
+        off, idx = orc.directed_csr(n, links)
+        for uniform in (True, False):
+            col, nc = orc.graph_coloring(n, off, idx, uniform)
+            assert nc == 2 and _proper(n, off, idx, col)
+    for k in (6, 8, 7, 9):
+        ring = np.array([(i, (i + 1) % k) for i in range(k)] + [((i + 1) % k, i) for i in range(k)])
+        off, idx = orc.directed_csr(k, ring)
+        assert orc.graph_coloring(k, off, idx)[1] == (2 if k % 2 == 0 else 3)
+    for k in range(1, 10):
+        comp = np.array([(a, b) for a in range(k) for b in range(k) if a != b]).reshape(-1, 2)
+        off, idx = orc.directed_csr(k, comp)
+        col, nc = orc.graph_coloring(k, off, idx)
+        assert nc == k and sorted(col.tolist()) == list(range(k))
+    from pscbfm_amd import systems
+    d = systems.diamond_network(64, (2, 2, 8), 32, seed=3)
+    o = orc.System(d, 1)
+    assert o.n_colors == 2 and orc.check_system(o, d["xyz"])[0] == 0
+    g = systems.cluster_gel(32, 0.8, 0.4, seed=8)
+    o = orc.System(g, 1)
+    counts = np.bincount(o.colors)
+    assert o.n_colors == 4 and orc.check_system(o, g["xyz"])[0] == 0
+    assert counts.max() <= -(-len(g["xyz"]) // 4)
+
+
 def test_three_forms_agree_on_network():
     from pscbfm_amd import systems
     s = systems.diamond_network(64, (2, 2, 8), 32, seed=3)
     sys_ = orc.System(s, 17)
-    assert sys_.n_colors == 3
+    assert sys_.n_colors == 2
     outs = [orc.sweep(sys_, s["xyz"], 0, 30, form) for form in (0, 1, 2)]
     for x, acc, prop in outs[1:]:
         assert np.array_equal(x, outs[0][0])
