@@ -89,6 +89,8 @@ int bfm_get_age(const bfm_updater *h, int64_t *age);
 /* setRng, UpdaterGPUScBFM_AB_Type.h:506: BFM_RNG_INT_HASH or BFM_RNG_PHILOX
  * (Saru, PCG and XORWOW are not provided). */
 int bfm_set_rng(bfm_updater *h, int rng);
+/* getRng, UpdaterGPUScBFM_AB_Type.h:509. */
+int bfm_get_rng(const bfm_updater *h, int *rng);
 
 /* Seed of the RNGs.  The reference seeds LeMonADE's r250 from a seed file
  * (src/pscbfm/mainCUDASimulatorGPUScBFM_AB_Type.cpp:123-134); here the 64-bit
@@ -101,6 +103,8 @@ int bfm_set_seed(bfm_updater *h, uint64_t seed);
  * Re-sorting never changes a trajectory here because the generator is keyed
  * on the original monomer id. */
 int bfm_set_steps_between_sortings(bfm_updater *h, int64_t steps);
+/* getStepsBetweenSortings, UpdaterGPUScBFM_AB_Type.h:514. */
+int bfm_get_steps_between_sortings(const bfm_updater *h, int64_t *steps);
 
 /* Layout of each colour segment (no counterpart in the reference's API; its
  * doSpatialSorting, UpdaterGPUScBFM_AB_Type.cu:1836-1935, is always on).  On

@@ -22,6 +22,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "pscbfm_amd.h"
 
@@ -61,6 +62,9 @@ public:
 
     void setGpu(int gpu) { miGpu = gpu; }
     void setSeed(uint64_t seed) { mSeed = seed; }
+    // GPUScBFM_AB_Type.h:110-111: the generator to use (BFM_RNG_INT_HASH or
+    // BFM_RNG_PHILOX); -1, the default, keeps the updater's own default.
+    void setRng(int rng) { miRng = rng; }
 
     // GPUScBFM_AB_Type.h:120-184: forward box, periodicity, age, positions,
     // attributes, both link directions and the bond set, then initialize.
@@ -71,6 +75,8 @@ public:
         auto &ing = mIngredients;
         const auto &mol = ing.getMolecules();
         check(bfm_set_gpu(mH, miGpu), "setGpu");
+        if (miRng != -1)   // forwarded only when set, as GPUScBFM_AB_Type.h:132-133
+            check(bfm_set_rng(mH, miRng), "setRng");
         check(bfm_set_seed(mH, mSeed), "setSeed");
         check(bfm_set_age(mH, int64_t(ing.modifyMolecules().getAge())), "setAge");
         check(bfm_set_periodicity(mH, ing.isPeriodicX(), ing.isPeriodicY(), ing.isPeriodicZ()), "setPeriodicity");
@@ -101,13 +107,14 @@ public:
         auto &mol = ing.modifyMolecules();
         check(bfm_set_age(mH, int64_t(mol.getAge())), "setAge");
         check(bfm_run_simulation_on_gpu(mH, mnSteps), "runSimulationOnGPU");
-        for (uint32_t i = 0; i < uint32_t(mol.size()); ++i) {
-            int32_t x, y, z;
-            check(bfm_get_monomer_position_in_x(mH, i, &x), "getMonomerPositionInX");
-            check(bfm_get_monomer_position_in_y(mH, i, &y), "getMonomerPositionInY");
-            check(bfm_get_monomer_position_in_z(mH, i, &z), "getMonomerPositionInZ");
-            mol[i].setAllCoordinates(x, y, z);
-        }
+        // one bulk read of the copied-back positions instead of the
+        // reference's three getters per monomer (GPUScBFM_AB_Type.h:207-214)
+        const uint32_t n = uint32_t(mol.size());
+        mPositions.resize(3 * size_t(n));
+        check(bfm_get_monomer_positions(mH, mPositions.data(), n), "getMonomerPositions");
+        for (uint32_t i = 0; i < n; ++i)
+            mol[i].setAllCoordinates(mPositions[3 * size_t(i)], mPositions[3 * size_t(i) + 1],
+                                     mPositions[3 * size_t(i) + 2]);
         mol.setAge(mol.getAge() + mnSteps);
         return true;
     }
@@ -123,6 +130,8 @@ private:
     uint32_t mnSteps;
     int miGpu;
     uint64_t mSeed;
+    int miRng = -1;
+    std::vector<int32_t> mPositions;
     bfm_updater *mH = nullptr;
 };
 

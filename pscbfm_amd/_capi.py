@@ -118,8 +118,10 @@ SIGNATURES = {
     "bfm_set_age": (c_int, [_H, c_int64]),
     "bfm_get_age": (c_int, [_H, POINTER(c_int64)]),
     "bfm_set_rng": (c_int, [_H, c_int]),
+    "bfm_get_rng": (c_int, [_H, POINTER(c_int)]),
     "bfm_set_seed": (c_int, [_H, c_uint64]),
     "bfm_set_steps_between_sortings": (c_int, [_H, c_int64]),
+    "bfm_get_steps_between_sortings": (c_int, [_H, POINTER(c_int64)]),
     "bfm_validate": (c_int, [_H]),
     "bfm_initialize": (c_int, [_H]),
     "bfm_run_simulation_on_gpu": (c_int, [_H, c_uint32]),

@@ -157,6 +157,14 @@ int bfm_set_rng(bfm_updater *h, int rng)
     return guarded([&] { h->u.setRng(rng); });
 }
 
+int bfm_get_rng(const bfm_updater *h, int *rng)
+{
+    NEED_HANDLE(h);
+    if (!rng)
+        return fail(BFM_E_INVALID_ARGUMENT, "null output pointer");
+    return guarded([&] { *rng = h->u.getRng(); });
+}
+
 int bfm_set_seed(bfm_updater *h, uint64_t seed)
 {
     NEED_HANDLE(h);
@@ -167,6 +175,14 @@ int bfm_set_steps_between_sortings(bfm_updater *h, int64_t steps)
 {
     NEED_HANDLE(h);
     return guarded([&] { h->u.setStepsBetweenSortings(steps); });
+}
+
+int bfm_get_steps_between_sortings(const bfm_updater *h, int64_t *steps)
+{
+    NEED_HANDLE(h);
+    if (!steps)
+        return fail(BFM_E_INVALID_ARGUMENT, "null output pointer");
+    return guarded([&] { *steps = h->u.getStepsBetweenSortings(); });
 }
 
 int bfm_set_spatial_sorting(bfm_updater *h, int on)

@@ -56,9 +56,11 @@ public:
     void setAge(int64_t age) { mAge = age; }
     int64_t getAge() const { return mAge; }
     void setRng(int rng);
+    int getRng() const { return int(mSys.rng); }
     void setSeed(uint64_t seed) { mSeed = seed; }
     uint64_t seed() const { return mSeed; }
     void setStepsBetweenSortings(int64_t steps);
+    int64_t getStepsBetweenSortings() const { return mStepsBetweenSortings; }
     void setSpatialSorting(bool on);
 
     // host half of initialize(): checks and colouring, no device work

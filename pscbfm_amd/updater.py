@@ -72,11 +72,21 @@ class UpdaterGPUScBFM_AB_Type:
     def setRng(self, rng: int) -> None:
         _capi.check(self._lib.bfm_set_rng(self._h, rng))
 
+    def getRng(self) -> int:
+        v = ctypes.c_int()
+        _capi.check(self._lib.bfm_get_rng(self._h, byref(v)))
+        return v.value
+
     def setSeed(self, seed: int) -> None:
         _capi.check(self._lib.bfm_set_seed(self._h, seed & 0xFFFFFFFFFFFFFFFF))
 
     def setStepsBetweenSortings(self, steps: int) -> None:
         _capi.check(self._lib.bfm_set_steps_between_sortings(self._h, steps))
+
+    def getStepsBetweenSortings(self) -> int:
+        v = c_int64()
+        _capi.check(self._lib.bfm_get_steps_between_sortings(self._h, byref(v)))
+        return v.value
 
     def initialize(self) -> None:
         _capi.check(self._lib.bfm_initialize(self._h))

@@ -2,12 +2,14 @@
 // LeMonADE Ingredients interface the reference adapter uses
 // (src/pscbfm/GPUScBFM_AB_Type.h:120-242), the way a LeMonADE TaskManager
 // would: initialize, execute x calls, cleanup.
-//   lemonade_adapter_demo <in.txt> <out.txt>
+//   lemonade_adapter_demo <in.txt> <out.txt> [rng]
+// (rng: the generator forwarded through setRng, GPUScBFM_AB_Type.h:110-111)
 // in.txt:  box n_monomers n_bonds seed steps_per_call calls age
 //          then n_monomers lines "x y z", then n_bonds lines "i j"
 // out.txt: final age, then n_monomers lines "x y z"
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <iostream>
 #include <vector>
@@ -67,8 +69,8 @@ struct Ingredients {
 
 int main(int argc, char **argv)
 {
-    if (argc != 3) {
-        std::cerr << "usage: lemonade_adapter_demo in.txt out.txt\n";
+    if (argc != 3 && argc != 4) {
+        std::cerr << "usage: lemonade_adapter_demo in.txt out.txt [rng]\n";
         return 2;
     }
     std::ifstream in(argv[1]);
@@ -92,6 +94,8 @@ int main(int argc, char **argv)
         return 2;
     }
     pscbfm_amd::GPUScBFM_AB_Type<Ingredients, Vec3> updater(ing, uint32_t(steps), 0, seed);
+    if (argc == 4)
+        updater.setRng(std::atoi(argv[3]));
     try {
         updater.initialize();
         for (uint64_t c = 0; c < calls; ++c)

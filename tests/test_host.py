@@ -64,6 +64,21 @@ def test_setter_errors_mirror_reference():
         u.setStepsBetweenSortings(0)
 
 
+def test_getters_return_what_was_set():
+    """getRng / getStepsBetweenSortings (UpdaterGPUScBFM_AB_Type.h:509,514):
+    the defaults (IntHash, .cu:1407; 5000 MCS, .cu:1379), then the set values."""
+    u = pscbfm_amd.UpdaterGPUScBFM_AB_Type()
+    assert u.getRng() == 1
+    assert u.getStepsBetweenSortings() == 5000
+    u.setRng(3)
+    u.setStepsBetweenSortings(77)
+    assert u.getRng() == 3
+    assert u.getStepsBetweenSortings() == 77
+    with pytest.raises(InvalidArgument):
+        u.setRng(2)
+    assert u.getRng() == 3
+
+
 def valid_melt():
     return systems.linear_melt(16, 16, 16, seed=3)
 
