@@ -296,6 +296,8 @@ int bfm_batch_get_stats(bfm_batch *b, uint32_t member, bfm_stats *out);
 /* bfm_observe for a member (its resident state; waits for the member's queued
  * work). */
 int bfm_batch_observe(bfm_batch *b, uint32_t member, bfm_observables *out);
+/* Frees the member's device memory (in stream order, after its queued work)
+ * and its host arrays; its id is not reused. */
 int bfm_batch_remove(bfm_batch *b, uint32_t member);
 int bfm_batch_set_graphs(bfm_batch *b, int on);
 /* As bfm_set_xcd_ranges, over the batch's whole grid (each member's blocks
@@ -334,6 +336,13 @@ int bfm_server_release(bfm_server *s, uint64_t job);
 /* Queued / running jobs, device bytes in use and budget, attempts done. */
 int bfm_server_info(bfm_server *s, uint32_t *queued, uint32_t *running, uint64_t *bytes_in_use,
                     uint64_t *budget, uint64_t *attempts);
+/* Host memory the server holds for its jobs (queued systems, results not yet
+ * released, the systems of admitted jobs), the jobs it still holds (every job
+ * until bfm_server_release) and the batch members it still tracks.  A job's
+ * host arrays go with bfm_server_job_result's first call (the positions stay
+ * until release) or with release; a served process that releases its jobs
+ * returns to its baseline. */
+int bfm_server_host_bytes(bfm_server *s, uint64_t *host_bytes, uint32_t *jobs, uint32_t *members);
 /* How the chunks were launched: graph launches, graphs written by an update of
  * an existing executable graph, graphs instantiated anew (any may be NULL). */
 int bfm_server_graph_stats(bfm_server *s, uint64_t *launches, uint64_t *updates, uint64_t *instantiations);

@@ -178,6 +178,7 @@ SIGNATURES = {
     "bfm_server_release": (c_int, [_H, c_uint64]),
     "bfm_server_info": (c_int, [_H, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint64),
                                 POINTER(c_uint64), POINTER(c_uint64)]),
+    "bfm_server_host_bytes": (c_int, [_H, POINTER(c_uint64), POINTER(c_uint32), POINTER(c_uint32)]),
     "bfm_server_graph_stats": (c_int, [_H, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]),
     "bfm_batch_graph_stats": (c_int, [_H, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]),
     "bfm_device_count": (c_int, [POINTER(c_int)]),

@@ -161,6 +161,12 @@ class Server:
         return {"queued": q.value, "running": r.value, "bytes_in_use": used.value, "budget": budget.value,
                 "attempts": att.value}
 
+    def host_memory(self) -> dict:
+        """Host bytes held for jobs, jobs held, batch members still tracked."""
+        b, j, m = c_uint64(), c_uint32(), c_uint32()
+        _capi.check(self._lib.bfm_server_host_bytes(self._h, byref(b), byref(j), byref(m)))
+        return {"host_bytes": b.value, "jobs": j.value, "members": m.value}
+
     def graph_stats(self) -> dict:
         """Chunk graphs launched, updated in place, instantiated."""
         v = [c_uint64() for _ in range(3)]

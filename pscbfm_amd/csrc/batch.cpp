@@ -147,6 +147,53 @@ void BatchEngine::release(void *p)
 
 size_t BatchEngine::bytesFor(const HostSystem &sys) { return slab_layout(sys).total; }
 
+BatchEngine::Member &BatchEngine::member(uint32_t id)
+{
+    if (id >= mMembers.size() || !mMembers[id])
+        throw std::invalid_argument("batch: no such member (never added, or released)");
+    return *mMembers[id];
+}
+
+const BatchEngine::Member &BatchEngine::member(uint32_t id) const
+{
+    if (id >= mMembers.size() || !mMembers[id])
+        throw std::invalid_argument("batch: no such member (never added, or released)");
+    return *mMembers[id];
+}
+
+template <class T>
+static void drop(std::vector<T> &v)
+{
+    std::vector<T>().swap(v);
+}
+
+template <class T>
+static size_t held(const std::vector<T> &v)
+{
+    return sizeof(T) * v.capacity();
+}
+
+size_t BatchEngine::hostBytes() const
+{
+    size_t b = 0;
+    for (const auto &m : mMembers)
+        if (m) {
+            const HostSystem &s = m->sys;
+            b += held(s.xyz) + held(s.attribute) + held(s.nnbr) + held(s.nbr) + held(s.colour) + held(s.seg_begin) +
+                 held(s.seg_count) + held(s.to_sorted) + held(s.ppos_sorted) + held(s.ids_sorted) +
+                 held(s.image_sorted) + held(s.nbr_sorted) + held(s.lattice);
+        }
+    return b;
+}
+
+uint32_t BatchEngine::liveMembers() const
+{
+    uint32_t k = 0;
+    for (const auto &m : mMembers)
+        k += m ? 1u : 0u;
+    return k;
+}
+
 uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age)
 {
     hip_check(hipSetDevice(mGpu), "hipSetDevice");
@@ -168,6 +215,18 @@ uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age)
         release(m->slab);
         throw;
     }
+    // The upload went through pinned staging, so the host copies are done
+    // with: a member keeps only what its device state does not hold -- the
+    // bond lists (for the analyzers) and the slot of every id (to unsort its
+    // downloads); positions come back into arrays sized when they arrive.
+    drop(m->sys.ppos_sorted);
+    drop(m->sys.ids_sorted);
+    drop(m->sys.image_sorted);
+    drop(m->sys.nbr_sorted);
+    drop(m->sys.lattice);
+    drop(m->sys.attribute);
+    drop(m->sys.xyz);
+    drop(m->sys.colour);
     m->dev.seg_begin = static_cast<const uint32_t *>(slab_part(m->slab, L, kPartSegBegin));
     m->dev.seg_count = static_cast<const uint32_t *>(slab_part(m->slab, L, kPartSegCount));
     m->dev.n_colours = m->sys.n_colours;
@@ -179,7 +238,7 @@ uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age)
 
 void BatchEngine::remove(uint32_t id)
 {
-    Member &m = *mMembers.at(id);
+    Member &m = member(id);
     if (!m.slab)
         return;
     // stream order: launches already queued finish before the memory is freed
@@ -190,9 +249,26 @@ void BatchEngine::remove(uint32_t id)
     mCompIds.clear();   // forces a rebuild of the tables
 }
 
+void BatchEngine::forget(uint32_t id)
+{
+    if (id >= mMembers.size() || !mMembers[id])
+        return;
+    if (mMembers[id]->slab)
+        remove(id);
+    mMembers[id].reset();
+}
+
+std::vector<int32_t> BatchEngine::takePositions(uint32_t id)
+{
+    Member &m = member(id);
+    std::vector<int32_t> xyz = std::move(m.sys.xyz);
+    forget(id);
+    return xyz;
+}
+
 void BatchEngine::observe(uint32_t id, Observables &out)
 {
-    Member &m = *mMembers.at(id);
+    Member &m = member(id);
     if (!m.slab)
         throw std::invalid_argument("batch: member was removed");
     hip_check(hipSetDevice(mGpu), "hipSetDevice");
@@ -288,7 +364,7 @@ void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
     }
     mMaxColours = 0;
     for (uint32_t j = 0; j < ids.size(); ++j) {
-        Member &m = *mMembers.at(ids[j]);
+        Member &m = member(ids[j]);
         if (!m.slab)
             throw std::invalid_argument("batch: member was removed");
         const uint32_t cls = kernel_class(m.sys.geom);
@@ -478,7 +554,7 @@ void BatchEngine::run(uint32_t n_mcs, uint32_t chunk)
 {
     std::vector<uint32_t> ids;
     for (uint32_t i = 0; i < mMembers.size(); ++i)
-        if (mMembers[i]->slab)
+        if (active(i))
             ids.push_back(i);
     runMembers(ids, n_mcs, chunk);
 }
@@ -492,7 +568,7 @@ void BatchEngine::copyBack(uint32_t id)
 
 BatchEngine::Download BatchEngine::download(uint32_t id)
 {
-    Member &m = *mMembers.at(id);
+    Member &m = member(id);
     if (!m.slab)
         throw std::invalid_argument("batch: member was removed");
     hip_check(hipSetDevice(mGpu), "hipSetDevice");
@@ -509,23 +585,30 @@ BatchEngine::Download BatchEngine::download(uint32_t id)
     return d;
 }
 
-void BatchEngine::unwrap_positions(uint32_t id) { unpack_positions(mMembers.at(id)->sys); }
+void BatchEngine::unwrap_positions(uint32_t id)
+{
+    HostSystem &s = member(id).sys;
+    s.xyz.resize(3 * size_t(s.n));
+    unpack_positions(s);
+}
 
 void BatchEngine::unpack(const Download &d, bool unwrap)
 {
-    HostSystem &s = mMembers.at(d.member)->sys;
+    HostSystem &s = member(d.member).sys;
     const size_t pos_bytes = sizeof(uint32_t) * s.n_pad, img_bytes = sizeof(int32_t) * 3 * size_t(s.n_pad);
     const char *src = static_cast<const char *>(mPinned.data(d.buf));
+    s.ppos_sorted.resize(s.n_pad);
+    s.image_sorted.resize(3 * size_t(s.n_pad));
     std::memcpy(s.ppos_sorted.data(), src, pos_bytes);
     std::memcpy(s.image_sorted.data(), src + pos_bytes, img_bytes);
     mPinned.give(d.buf);
     if (unwrap)
-        unpack_positions(s);
+        unwrap_positions(d.member);
 }
 
 void BatchEngine::counters(uint32_t id, uint64_t &accepted, uint64_t &proposals)
 {
-    Member &m = *mMembers.at(id);
+    Member &m = member(id);
     accepted = proposals = 0;
     if (!m.slab)
         return;

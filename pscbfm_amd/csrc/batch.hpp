@@ -96,14 +96,24 @@ public:
     void unwrap_positions(uint32_t id);
     // Frees a member's device memory; its id is not reused.
     void remove(uint32_t id);
+    // Drops a removed member's host arrays too (result taken or not wanted);
+    // its id stays unused.  No-op for a member already forgotten.
+    void forget(uint32_t id);
+    // Unsorted, unwrapped positions of a downloaded member, moved out of the
+    // engine (the member's host arrays are dropped with them).
+    std::vector<int32_t> takePositions(uint32_t id);
+    // Host memory the members' systems still hold.
+    size_t hostBytes() const;
+    // Members still tracked (added and not forgotten).
+    uint32_t liveMembers() const;
 
     // Observables of a member's resident state (the device analyzers of a
     // plain updater, on the batch's stream; waits for the member's work).
     void observe(uint32_t id, Observables &out);
 
-    const HostSystem &system(uint32_t id) const { return mMembers.at(id)->sys; }
-    int64_t age(uint32_t id) const { return mMembers.at(id)->age; }
-    bool active(uint32_t id) const { return mMembers.at(id)->slab != nullptr; }
+    const HostSystem &system(uint32_t id) const { return member(id).sys; }
+    int64_t age(uint32_t id) const { return member(id).age; }
+    bool active(uint32_t id) const { return id < mMembers.size() && mMembers[id] && mMembers[id]->slab; }
     size_t size() const { return mMembers.size(); }
     size_t deviceBytes() const { return mBytes; }
     void counters(uint32_t id, uint64_t &accepted, uint64_t &proposals);
@@ -139,6 +149,8 @@ private:
         std::unique_ptr<DeviceAnalyzer> analyzer;   // made by the first observe()
     };
 
+    Member &member(uint32_t id);
+    const Member &member(uint32_t id) const;
     void rebuildTables(const std::vector<uint32_t> &ids);
     void runChunk(uint32_t mcs);
     // device memory in stream order (the pool of the device) when available

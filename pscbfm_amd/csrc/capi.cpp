@@ -677,7 +677,7 @@ int bfm_batch_remove(bfm_batch *b, uint32_t member)
     return guarded([&] {
         if (member >= b->e->size())
             throw std::invalid_argument("batch: no such member");
-        b->e->remove(member);
+        b->e->forget(member);
     });
 }
 
@@ -807,6 +807,19 @@ int bfm_server_release(bfm_server *s, uint64_t job)
 {
     NEED(s, "server handle");
     return guarded([&] { s->s->release(job); });
+}
+
+int bfm_server_host_bytes(bfm_server *s, uint64_t *host_bytes, uint32_t *jobs, uint32_t *members)
+{
+    NEED(s, "server handle");
+    return guarded([&] {
+        if (host_bytes)
+            *host_bytes = s->s->hostBytes();
+        if (jobs)
+            *jobs = s->s->jobsHeld();
+        if (members)
+            *members = s->s->engine().liveMembers();
+    });
 }
 
 int bfm_server_info(bfm_server *s, uint32_t *queued, uint32_t *running, uint64_t *bytes_in_use, uint64_t *budget,

@@ -1,7 +1,7 @@
 // resort.hip -- on-device spatial re-sort within colour segments (gfx950).
 #include "resort.hpp"
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 
@@ -92,14 +92,12 @@ void Resorter::release()
     mWorkBytes = mSortBytes = 0;
 }
 
-void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const SweepBuffers &buf, hipStream_t stream)
+void Resorter::reserve(const HostSystem &s, const SlabLayout &L)
 {
-    const uint32_t n_pad = s.n_pad;
-    // the gathered arrays mirror the slab's [ppos, ids, image, nbr] range, so
-    // one device-to-device copy puts them back
-    const size_t first = L.offset[kPartPPos];
-    const size_t range = L.offset[kPartNbr] + L.bytes[kPartNbr] - first;
-    const size_t words = (size_t(n_pad) * sizeof(uint32_t) + 255) / 256 * 256;
+    // the gathered arrays mirror the slab's [ppos, ids, image, nbr] range,
+    // then keys, sorted keys, identity, permutation and its inverse
+    const size_t range = L.offset[kPartNbr] + L.bytes[kPartNbr] - L.offset[kPartPPos];
+    const size_t words = (size_t(s.n_pad) * sizeof(uint32_t) + 255) / 256 * 256;
     const size_t need = range + 5 * words;
     if (mWorkBytes < need) {
         if (mWork)
@@ -109,19 +107,12 @@ void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const S
         hip_check(hipMalloc(&mWork, need), "hipMalloc (re-sort workspace)");
         mWorkBytes = need;
     }
-    char *w = static_cast<char *>(mWork);
-    auto mirror = [&](SlabPart p) { return w + (L.offset[p] - first); };
-    uint32_t *keys = reinterpret_cast<uint32_t *>(w + range);
-    uint32_t *keys_sorted = reinterpret_cast<uint32_t *>(w + range + words);
-    uint32_t *ident = reinterpret_cast<uint32_t *>(w + range + 2 * words);
-    uint32_t *perm = reinterpret_cast<uint32_t *>(w + range + 3 * words);
-    uint32_t *inv = reinterpret_cast<uint32_t *>(w + range + 4 * words);
-
-    const uint32_t bits = site_bits(s);
     size_t temp = 0;
-    hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, keys, keys_sorted, ident, perm,
-                                                 int(std::max<uint32_t>(s.max_segment, 1)), 0, int(bits), stream),
+    uint32_t *none = nullptr;
+    hip_check(rocprim::radix_sort_pairs(nullptr, temp, none, none, none, none,
+                                        size_t(std::max<uint32_t>(s.max_segment, 1)), 0u, site_bits(s)),
               "radix sort (size query)");
+    temp = std::max<size_t>(temp, 256);
     if (mSortBytes < temp) {
         if (mSortTemp)
             hip_check(hipFree(mSortTemp), "hipFree");
@@ -130,6 +121,24 @@ void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const S
         hip_check(hipMalloc(&mSortTemp, temp), "hipMalloc (re-sort temporary)");
         mSortBytes = temp;
     }
+}
+
+void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const SweepBuffers &buf, hipStream_t stream)
+{
+    const uint32_t n_pad = s.n_pad;
+    const size_t first = L.offset[kPartPPos];
+    const size_t range = L.offset[kPartNbr] + L.bytes[kPartNbr] - first;
+    const size_t words = (size_t(n_pad) * sizeof(uint32_t) + 255) / 256 * 256;
+    if (mWorkBytes < range + 5 * words || !mSortTemp)
+        throw std::logic_error("re-sort workspace not reserved for this system");
+    char *w = static_cast<char *>(mWork);
+    auto mirror = [&](SlabPart p) { return w + (L.offset[p] - first); };
+    uint32_t *keys = reinterpret_cast<uint32_t *>(w + range);
+    uint32_t *keys_sorted = reinterpret_cast<uint32_t *>(w + range + words);
+    uint32_t *ident = reinterpret_cast<uint32_t *>(w + range + 2 * words);
+    uint32_t *perm = reinterpret_cast<uint32_t *>(w + range + 3 * words);
+    uint32_t *inv = reinterpret_cast<uint32_t *>(w + range + 4 * words);
+    const uint32_t bits = site_bits(s);
 
     const dim3 grid((n_pad + kResortBlock - 1) / kResortBlock), block(kResortBlock);
     hipLaunchKernelGGL(k_keys, grid, block, 0, stream, buf.ppos, buf.tables, n_pad, keys, ident, perm);
@@ -139,8 +148,8 @@ void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const S
         if (n < 2)
             continue;
         size_t tb = mSortBytes;
-        hip_check(hipcub::DeviceRadixSort::SortPairs(mSortTemp, tb, keys + b, keys_sorted + b, ident + b, perm + b,
-                                                     int(n), 0, int(bits), stream),
+        hip_check(rocprim::radix_sort_pairs(mSortTemp, tb, keys + b, keys_sorted + b, ident + b, perm + b, size_t(n),
+                                            0u, bits, stream),
                   "radix sort");
     }
     hipLaunchKernelGGL(k_inverse, grid, block, 0, stream, perm, n_pad, inv);

@@ -4,7 +4,7 @@
 // mnStepsBetweenSortings MCS (doSpatialSorting, UpdaterGPUScBFM_AB_Type.cu:
 // 1836-1935).  Here the key is the lattice site of a monomer's lower corner
 // (the compact Morton order the lattice uses); corners never coincide, so the
-// order is unique.  Each colour segment is radix-sorted on the device, the
+// order is unique.  Each colour segment is radix-sorted on the device (rocPRIM), the
 // per-monomer arrays are gathered into the new order and the partner indices
 // are remapped through the inverse permutation.  The generator is keyed on the
 // original id, so a re-sort never changes the trajectory.
@@ -28,8 +28,11 @@ public:
     Resorter(const Resorter &) = delete;
     Resorter &operator=(const Resorter &) = delete;
 
+    // Allocates (or grows) the device workspace a re-sort of s needs; the
+    // sweep drivers call it before their run loops, which then allocate nothing.
+    void reserve(const HostSystem &s, const SlabLayout &L);
     // Re-sorts the system held in `slab` (layout L, kernels' view buf) on
-    // `stream`.  The device workspace is allocated on the first call.
+    // `stream`, in the workspace reserve() made (radix sort of rocPRIM).
     void run(const HostSystem &s, void *slab, const SlabLayout &L, const SweepBuffers &buf, hipStream_t stream);
     // Frees the workspace (call with the device current).
     void release();

@@ -101,6 +101,8 @@ uint32_t Server::step()
             break;
         try {
             j.member = mEngine.add(std::move(j.sys), j.seed, j.age);   // the engine owns it now
+            j.sys = HostSystem();
+            j.admitted = true;
             j.state = kRunning;
             j.t_admit = now();
             mInUse += j.bytes;
@@ -166,7 +168,8 @@ void Server::cancel(uint64_t id)
     } else if (j.state == kRunning) {
         if (std::find(mRetiring.begin(), mRetiring.end(), id) != mRetiring.end())
             return;   // finished on the device; its result is on its way
-        mEngine.remove(j.member);
+        mEngine.forget(j.member);   // device memory freed in stream order, host arrays now
+        j.admitted = false;
         mInUse -= j.bytes;
         mRunning.erase(std::find(mRunning.begin(), mRunning.end(), id));
     } else {
@@ -181,6 +184,8 @@ void Server::release(uint64_t id)
     Job &j = job(id);
     if (j.state == kQueued || j.state == kRunning)
         throw std::invalid_argument("server: release of a job that has not finished");
+    if (j.admitted)
+        mEngine.forget(j.member);   // its result was never asked for
     mJobs.erase(id);
 }
 
@@ -188,16 +193,32 @@ Server::State Server::state(uint64_t id) const { return job(id).state; }
 uint32_t Server::mcsDone(uint64_t id) const { return job(id).done; }
 const std::string &Server::error(uint64_t id) const { return job(id).error; }
 
+size_t Server::hostBytes() const
+{
+    size_t b = mEngine.hostBytes();
+    for (const auto &kv : mJobs) {
+        const HostSystem &s = kv.second->sys;
+        b += sizeof(int32_t) * (s.xyz.capacity() + s.attribute.capacity()) + s.nnbr.capacity() +
+             sizeof(uint32_t) * (s.nbr.capacity() + s.colour.capacity() + s.to_sorted.capacity() +
+                                 s.ppos_sorted.capacity() + s.ids_sorted.capacity() + s.nbr_sorted.capacity() +
+                                 s.lattice.capacity()) +
+             sizeof(int32_t) * s.image_sorted.capacity();
+    }
+    return b;
+}
+
 const HostSystem &Server::result(uint64_t id)
 {
     Job &j = job(id);
     if (j.state != kDone)
         throw std::invalid_argument("server: the job has no result (not done)");
     if (!j.unwrapped) {
+        // the member's host arrays leave the engine with the positions
         mEngine.unwrap_positions(j.member);
-        j.sys.xyz = mEngine.system(j.member).xyz;
+        j.sys.xyz = mEngine.takePositions(j.member);
         j.sys.n = j.n;
         j.unwrapped = true;
+        j.admitted = false;
     }
     return j.sys;
 }

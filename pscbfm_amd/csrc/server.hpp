@@ -51,6 +51,9 @@ public:
     uint64_t budget() const { return mBudget; }
     uint64_t bytesInUse() const { return mInUse; }
     uint64_t attemptsDone() const { return mAttempts; }
+    // Host memory held for jobs: queued systems, results, the engine's members.
+    size_t hostBytes() const;
+    uint32_t jobsHeld() const { return uint32_t(mJobs.size()); }
     BatchEngine &engine() { return mEngine; }
 
 private:
@@ -65,6 +68,7 @@ private:
         uint64_t bytes = 0;
         State state = kQueued;
         uint32_t member = 0;
+        bool admitted = false;   // the engine still holds its member
         BatchEngine::Download download{};   // while retiring
         std::string error;
         double t_submit = 0, t_admit = -1, t_finish = -1;

@@ -254,6 +254,9 @@ void UpdaterGPUScBFM_AB_Type::initialize()
         hip_check(hipStreamCreate(&mStream), "hipStreamCreate");
         mBuf = upload_slab(s, mDevSlab, L, mSeed, mStream);
         mBuf.xcd_ranges = mXcdRanges ? 1u : 0u;
+        // the periodic re-sort's workspace, so that runs never allocate
+        if (s.spatial_order)
+            mResorter.reserve(s, L);
         hip_check(hipStreamSynchronize(mStream), "hipStreamSynchronize (initialize)");
     } catch (...) {
         freeDevice();

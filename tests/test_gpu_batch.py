@@ -87,6 +87,35 @@ def test_batch_kernel_classes(gpu_ready):
         assert st["accepted"] > 0
 
 
+def test_batch_with_four_colour_members(gpu_ready):
+    """Members with 4 colours (tetrahedral cluster gels), 2 colours (melts, the
+    diamond network) and 4 colours with walls in one batch: in every slot a
+    2-colour member idles for the colour slots it does not have, and each
+    member ends where the oracle's replay of it alone ends."""
+    cases = [(systems.cluster_gel(32, 0.8, 0.5, seed=51), 0x5EED0000000000D1, 0),
+             (systems.cluster_gel(16, 0.9, 0.3, seed=52), 0x5EED0000000000D2, 77),
+             (dict(systems.cluster_gel(64, 0.7, 0.6, seed=53), periodic=(0, 1, 1)), 0x5EED0000000000D3, 5),
+             (systems.linear_melt(32, 100, 16, seed=54), 0x5EED0000000000D4, 0),
+             (systems.diamond_network(64, (2, 2, 8), 32, seed=55), 0x5EED0000000000D5, 9)]
+    print(f"case: batch with four-colour members, {len(cases)} members", flush=True)
+    b = Batch(0)
+    ids = []
+    for s, seed, age in cases:
+        u = updater_from_system(s, seed=seed, age=age, validate_only=True)
+        colours, nc = u.getColors(len(s["xyz"]))
+        own = orc.System(s, seed)
+        assert nc == own.n_colors and np.array_equal(colours, own.colors)
+        ids.append(b.add(u, len(s["xyz"])))
+    assert [orc.System(s, 1).n_colors for s, _, _ in cases] == [4, 4, 4, 2, 2]
+    b.run(90, chunk=40)     # 40, 40, 10
+    for m, (s, seed, age) in zip(ids, cases):
+        want, acc, prop = oracle_final(s, seed, age, 90)
+        assert np.array_equal(b.positions(m), want), f"member {m} differs"
+        st = b.stats(m)
+        assert (st["accepted"], st["proposals"]) == (acc, prop)
+        assert st["accepted"] > 0
+
+
 def test_batch_subset_and_removal(gpu_ready):
     cases = mixed_systems(4, seed=9)
     b = Batch(0)

@@ -262,3 +262,29 @@ def test_network_full_size_20_mcs(gpu_ready):
     s = systems.diamond_network(256, (8, 8, 32), 32, seed=0x5EED000000000003)
     assert len(s["xyz"]) == 1064960
     assert_parity(s, seed=0x5EED000000000003, chunks=[20])
+
+
+@pytest.mark.parametrize("rng", [orc.RNG_INT_HASH, orc.RNG_PHILOX])
+def test_four_colour_cluster_gel(gpu_ready, rng):
+    """Tetrahedral clusters (every pair of a cluster bonded: the bond graph
+    needs 4 colours) mixed with chains of four; the rebalance spreads the
+    chains over all 4 colours.  Every MCS sweeps 4 colours in a permutation of
+    4, 8 kernels per MCS."""
+    s = systems.cluster_gel(32, 0.8, 0.5, seed=41)
+    n = len(s["xyz"])
+    u = updater_from_system(s, seed=1, validate_only=True)
+    colors, nc = u.getColors(n)
+    own = orc.System(s, 1)
+    assert nc == own.n_colors == 4
+    assert np.array_equal(colors, own.colors)
+    st = assert_parity(s, seed=0x5EED0000000000C4, chunks=[150, 250], rng=rng)
+    assert st["accepted"] > 0
+    assert st["substeps"] == 4 * 400
+
+
+def test_four_colour_gel_with_resort_and_walls(gpu_ready):
+    """A denser cluster gel between walls on one axis, re-sorted on the device
+    every 9 MCS: still the oracle's trajectory."""
+    s = dict(systems.cluster_gel(64, 0.9, 0.7, seed=42), periodic=(1, 0, 1))
+    st = assert_parity(s, seed=0x5EED0000000000C5, chunks=[60, 40], sort_every=9)
+    assert st["accepted"] > 0 and st["sorts"] >= 10
