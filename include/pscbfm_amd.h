@@ -141,6 +141,21 @@ int bfm_initialize(bfm_updater *h);
  * advances by n_mcs. */
 int bfm_run_simulation_on_gpu(bfm_updater *h, uint32_t n_mcs);
 
+/* No counterpart in the reference: bfm_run_simulation_on_gpu for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same result as
+ * bfm_run_simulation_on_gpu (n_mcs MCS, positions copied back into the
+ * handle's host mirror, the age advanced), but it calls no hipMalloc, hipFree
+ * or other allocating routine: the copy-back is staged in the library's
+ * static device scratch of 32 MiB per GPU (held until the process ends), so
+ * the system may have at most bfm_noalloc_max_monomers() = 2,796,202 monomers
+ * (BFM_E_INVALID_ARGUMENT otherwise).  No kernel timing.  Every call on one
+ * GPU shares that GPU's scratch: calls that overlap on one GPU (from several
+ * threads) race; calls one after another do not.  After bfm_initialize, whose
+ * allocations it relies on. */
+int bfm_run_simulation_on_gpu_noalloc(bfm_updater *h, uint32_t n_mcs);
+uint32_t bfm_noalloc_max_monomers(void);
+
 /* getMonomerPositionIn{X,Y,Z}, UpdaterGPUScBFM_AB_Type.cu:2612-2614
  * (unwrapped coordinates). */
 int bfm_get_monomer_position_in_x(const bfm_updater *h, uint32_t i, int32_t *x);

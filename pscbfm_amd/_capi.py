@@ -125,6 +125,8 @@ SIGNATURES = {
     "bfm_validate": (c_int, [_H]),
     "bfm_initialize": (c_int, [_H]),
     "bfm_run_simulation_on_gpu": (c_int, [_H, c_uint32]),
+    "bfm_run_simulation_on_gpu_noalloc": (c_int, [_H, c_uint32]),
+    "bfm_noalloc_max_monomers": (c_uint32, []),
     "bfm_get_monomer_position_in_x": (c_int, [_H, c_uint32, POINTER(c_int32)]),
     "bfm_get_monomer_position_in_y": (c_int, [_H, c_uint32, POINTER(c_int32)]),
     "bfm_get_monomer_position_in_z": (c_int, [_H, c_uint32, POINTER(c_int32)]),

@@ -30,6 +30,7 @@ SOURCES = [
     "bfm_io.cpp",
     "analyzers_host.cpp",
     "analyzers.hip",
+    "noalloc.hip",
 ]
 
 # Packed FP32 instructions stay off in every HIP object (there is no float

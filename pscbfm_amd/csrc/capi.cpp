@@ -10,6 +10,7 @@
 #include "../../include/pscbfm_amd.h"
 #include "bfm_io.hpp"
 #include "server.hpp"
+#include "noalloc.hpp"
 #include "updater.hpp"
 
 struct bfm_updater {
@@ -208,6 +209,14 @@ int bfm_run_simulation_on_gpu(bfm_updater *h, uint32_t n_mcs)
     NEED_HANDLE(h);
     return guarded([&] { h->u.runSimulationOnGPU(n_mcs); });
 }
+
+int bfm_run_simulation_on_gpu_noalloc(bfm_updater *h, uint32_t n_mcs)
+{
+    NEED_HANDLE(h);
+    return guarded([&] { h->u.runSimulationOnGPUNoAlloc(n_mcs); });
+}
+
+uint32_t bfm_noalloc_max_monomers(void) { return pscbfm::kNoAllocMaxMonomers; }
 
 int bfm_run_resident(bfm_updater *h, uint32_t n_mcs)
 {

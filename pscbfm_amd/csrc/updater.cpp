@@ -2,6 +2,7 @@
 #include "updater.hpp"
 
 #include "device_slab.hpp"
+#include "noalloc.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -275,14 +276,14 @@ void UpdaterGPUScBFM_AB_Type::initialize()
     mInitialized = true;
 }
 
-void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs)
+void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs, bool timing)
 {
     if (!mInitialized)
         throw StateError("runSimulationOnGPU before initialize");
     hip_check(hipSetDevice(mGpu), "hipSetDevice");
     mBuf.seed = mSeed;
     const HostSystem &s = mSys;
-    std::vector<uint32_t> order(s.n_colours);
+    uint32_t order[256];   // at most 255 colours; no allocation in the loop
     for (uint32_t m = 0; m < n_mcs; ++m) {
         // periodic re-sort at the start of every mnStepsBetweenSortings-th MCS
         // (runSimulationOnGPU, UpdaterGPUScBFM_AB_Type.cu:3005-3012)
@@ -293,13 +294,13 @@ void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs)
             ++mSorts;
         }
         const uint64_t age = uint64_t(mAge);
-        colour_order(mSeed, age, s.n_colours, order.data());
+        colour_order(mSeed, age, s.n_colours, order);
         for (uint32_t k = 0; k < s.n_colours; ++k) {
             const uint32_t c = order[k];
             const uint32_t parity = uint32_t(mSubsteps & 1u);
             // timed substeps: start and stop events of each kernel's own dispatch
             hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-            const bool timed = mTimingEvery != 0 && mSubsteps % mTimingEvery == 0;
+            const bool timed = timing && mTimingEvery != 0 && mSubsteps % mTimingEvery == 0;
             if (timed)
                 for (auto &x : ev)
                     x = takeEvent();
@@ -374,6 +375,35 @@ void UpdaterGPUScBFM_AB_Type::runSimulationOnGPU(uint32_t n_mcs)
 {
     runResident(n_mcs);
     copyBack();
+}
+
+void UpdaterGPUScBFM_AB_Type::runSimulationOnGPUNoAlloc(uint32_t n_mcs)
+{
+    HostSystem &s = mSys;
+    if (s.n > kNoAllocMaxMonomers) {
+        std::ostringstream msg;
+        msg << "runSimulationOnGPU without allocation takes at most " << kNoAllocMaxMonomers
+            << " monomers (its copy-back scratch holds " << (kNoAllocScratchBytes >> 20) << " MiB); this system has "
+            << s.n;
+        throw std::invalid_argument(msg.str());
+    }
+    if (!mInitialized)
+        throw StateError("runSimulationOnGPU before initialize");
+    hip_check(hipSetDevice(mGpu), "hipSetDevice");
+    // The sweep allocates nothing after initialize (its re-sort workspace is
+    // reserved there); the copy-back is staged in the library's device scratch
+    // of this GPU, shared by every call on it: calls that overlap on one GPU
+    // race.  The unwrap kernel writes every monomer's entry, so nothing in the
+    // scratch needs clearing first.
+    int32_t *staged = static_cast<int32_t *>(noalloc_scratch(mGpu));
+    runResident(n_mcs, false);
+    if (s.n == 0)
+        return;
+    const uint32_t box[3] = {uint32_t(s.box[0]), uint32_t(s.box[1]), uint32_t(s.box[2])};
+    hip_check(launch_unwrap(mBuf, box, staged, mStream), "unwrap kernel");
+    hip_check(hipMemcpyAsync(s.xyz.data(), staged, sizeof(int32_t) * 3 * size_t(s.n), hipMemcpyDeviceToHost, mStream),
+              "download positions");
+    hip_check(hipStreamSynchronize(mStream), "hipStreamSynchronize (copy-back)");
 }
 
 void UpdaterGPUScBFM_AB_Type::deviceCounters(uint64_t &accepted, uint64_t &proposals) const

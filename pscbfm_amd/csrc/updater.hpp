@@ -73,7 +73,10 @@ public:
     void cleanup();
 
     // extensions
-    void runResident(uint32_t n_mcs);
+    void runResident(uint32_t n_mcs) { runResident(n_mcs, true); }
+    // runSimulationOnGPU with its copy-back staged in the library's static
+    // device scratch: no device or host allocation, no kernel timing.
+    void runSimulationOnGPUNoAlloc(uint32_t n_mcs);
     void copyBack();
     void setKernelTiming(uint32_t every) { mTimingEvery = every; }
     void setXcdRanges(bool on) { mBuf.xcd_ranges = on ? 1u : 0u; mXcdRanges = on; }
@@ -109,6 +112,8 @@ public:
 
 private:
     int32_t at(T_Id i, int a) const;
+    // the sweep driver; `timing` = honour setKernelTiming
+    void runResident(uint32_t n_mcs, bool timing);
     void freeDevice();
     void collectTimings() const;
 

@@ -94,6 +94,12 @@ class UpdaterGPUScBFM_AB_Type:
     def runSimulationOnGPU(self, n_mcs: int) -> None:
         _capi.check(self._lib.bfm_run_simulation_on_gpu(self._h, n_mcs))
 
+    def runSimulationOnGPUNoAlloc(self, n_mcs: int) -> None:
+        """runSimulationOnGPU without any allocation: the copy-back is staged in
+        the library's static device scratch (at most bfm_noalloc_max_monomers
+        monomers; calls that overlap on one GPU race)."""
+        _capi.check(self._lib.bfm_run_simulation_on_gpu_noalloc(self._h, n_mcs))
+
     def getMonomerPositionInX(self, i: int) -> int:
         v = c_int32()
         _capi.check(self._lib.bfm_get_monomer_position_in_x(self._h, i, byref(v)))
