@@ -159,9 +159,12 @@ def served_leg(args, capacity: float) -> dict:
     """Poisson job arrivals at rates swept around the batched capacity;
     continuous batching admitted by free HBM; p50/p99 completion latency.
 
-    A job counts as finished when its client holds the result: the loop asks
-    for each job's positions (unsorted and unwrapped on the host) as soon as
-    the server reports it done, on the same thread that drives step().
+    A job counts as finished when its client holds the result: the loop copies
+    each job's positions out (unsorted and unwrapped on the device at
+    retirement) and releases the job as soon as the server reports it done, on
+    the same thread that drives step().  Jobs are submitted with the
+    validated system handed over (bfm_server_submit_take); their updaters are
+    built and validated before the clock starts, as clients would.
     Throughput is given twice: total work over the makespan (first arrival to
     last result, so the drain at the end is included) and over the steady
     window between the arrivals of the 20th-percentile and the last job (the
@@ -192,13 +195,14 @@ def served_leg(args, capacity: float) -> dict:
         while True:
             now = time.perf_counter() - t0
             while nxt < len(jobs) and arrivals[nxt] <= now:
-                ids[nxt] = srv.submit(upd[nxt], jobs[nxt]["mcs"], len(built[nxt]["xyz"]))
+                ids[nxt] = srv.submit(upd[nxt], jobs[nxt]["mcs"], len(built[nxt]["xyz"]), take=True)
                 running.add(nxt)
                 nxt += 1
             pending = srv.step()
             for j in list(running):
                 if srv.state(ids[j])[0] >= 2:      # done, failed or cancelled
                     results[j] = srv.result(ids[j])
+                    srv.release(ids[j])            # the client holds its copy
                     finish[j] = time.perf_counter() - t0
                     running.discard(j)
             now = time.perf_counter() - t0

@@ -338,11 +338,20 @@ int bfm_server_destroy(bfm_server *s);
  * an initialized updater hands over its positions as bfm_batch_add does.
  * A job larger than the whole budget is accepted as BFM_JOB_FAILED. */
 int bfm_server_submit(bfm_server *s, bfm_updater *h, uint32_t n_mcs, uint64_t *job);
+/* As bfm_server_submit, but a configured updater that is not initialized
+ * hands its validated system over instead of a copy: nothing is copied, and
+ * the updater is left as after bfm_cleanup (its seed and age kept).  An
+ * initialized updater is submitted as by bfm_server_submit.  For serving
+ * loops whose clients build one updater per job. */
+int bfm_server_submit_take(bfm_server *s, bfm_updater *h, uint32_t n_mcs, uint64_t *job);
 /* One scheduling round; *pending = jobs still queued or running. */
 int bfm_server_step(bfm_server *s, uint32_t *pending);
 int bfm_server_job_state(bfm_server *s, uint64_t job, int *state, uint32_t *mcs_done);
 /* Seconds since server creation at submission, admission and completion (-1 if not yet). */
 int bfm_server_job_times(bfm_server *s, uint64_t job, double *submitted, double *admitted, double *finished);
+/* Final positions (3 * n int32, by id; n = the job's monomer count).  They are
+ * unsorted and unwrapped on the device when the job retires and held in
+ * pinned host memory until bfm_server_release. */
 int bfm_server_job_result(bfm_server *s, uint64_t job, int32_t *xyz, uint32_t n);
 /* Thread-local message of a failed job. */
 int bfm_server_job_error(bfm_server *s, uint64_t job, char *buf, uint32_t len);

@@ -171,6 +171,7 @@ SIGNATURES = {
     "bfm_server_create": (c_int, [POINTER(c_void_p), c_int, c_uint64, c_uint32]),
     "bfm_server_destroy": (c_int, [_H]),
     "bfm_server_submit": (c_int, [_H, _H, c_uint32, POINTER(c_uint64)]),
+    "bfm_server_submit_take": (c_int, [_H, _H, c_uint32, POINTER(c_uint64)]),
     "bfm_server_step": (c_int, [_H, POINTER(c_uint32)]),
     "bfm_server_job_state": (c_int, [_H, c_uint64, POINTER(c_int), POINTER(c_uint32)]),
     "bfm_server_job_times": (c_int, [_H, c_uint64, POINTER(c_double), POINTER(c_double), POINTER(c_double)]),

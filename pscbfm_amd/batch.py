@@ -115,9 +115,13 @@ class Server:
             self._lib.bfm_server_destroy(h)
             self._h = None
 
-    def submit(self, u: UpdaterGPUScBFM_AB_Type, n_mcs: int, n_monomers: int) -> int:
+    def submit(self, u: UpdaterGPUScBFM_AB_Type, n_mcs: int, n_monomers: int, take: bool = False) -> int:
+        """Queues the updater's system for n_mcs MCS.  take=True hands a
+        not-initialized updater's validated system over without a copy (the
+        updater is left as after cleanup)."""
         j = c_uint64()
-        _capi.check(self._lib.bfm_server_submit(self._h, u._h, n_mcs, byref(j)))
+        fn = self._lib.bfm_server_submit_take if take else self._lib.bfm_server_submit
+        _capi.check(fn(self._h, u._h, n_mcs, byref(j)))
         self._n[j.value] = n_monomers
         return j.value
 

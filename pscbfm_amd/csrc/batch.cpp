@@ -112,6 +112,8 @@ BatchEngine::~BatchEngine()
     for (auto &m : mMembers)
         if (m && m->slab)
             hipFree(m->slab);
+    for (auto &f : mFreeSlabs)
+        hipFree(f.second);
     if (mDevSys)
         hipFree(mDevSys);
     for (uint32_t c = 0; c < kClasses; ++c) {
@@ -146,6 +148,40 @@ void BatchEngine::release(void *p)
 }
 
 size_t BatchEngine::bytesFor(const HostSystem &sys) { return slab_layout(sys).total; }
+
+// Size class of a slab: rounded up to 1/8 of the next power of two (at most
+// 12.5 % over), so that members of similar size share freed slabs.
+static size_t slab_class(size_t bytes)
+{
+    size_t p = 4096;
+    while (p < bytes)
+        p <<= 1;
+    const size_t step = std::max<size_t>(p / 8, 4096);
+    return (bytes + step - 1) / step * step;
+}
+
+void *BatchEngine::takeSlab(size_t bytes, size_t &klass)
+{
+    klass = slab_class(bytes);
+    for (size_t k = mFreeSlabs.size(); k-- > 0;)
+        if (mFreeSlabs[k].first == klass) {
+            void *p = mFreeSlabs[k].second;
+            mFreeSlabs[k] = mFreeSlabs.back();
+            mFreeSlabs.pop_back();
+            return p;
+        }
+    return allocate(klass, "allocation of a batch member");
+}
+
+void BatchEngine::giveSlab(void *slab, size_t klass)
+{
+    // freed slabs kept beyond 1024 go back to the device pool
+    if (mFreeSlabs.size() >= 1024) {
+        release(mFreeSlabs.front().second);
+        mFreeSlabs.erase(mFreeSlabs.begin());
+    }
+    mFreeSlabs.emplace_back(klass, slab);
+}
 
 BatchEngine::Member &BatchEngine::member(uint32_t id)
 {
@@ -194,7 +230,7 @@ uint32_t BatchEngine::liveMembers() const
     return k;
 }
 
-uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age)
+uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age, bool keep_topology)
 {
     hip_check(hipSetDevice(mGpu), "hipSetDevice");
     auto m = std::make_unique<Member>();
@@ -202,7 +238,7 @@ uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age)
     m->seed = seed;
     m->age = age;
     const SlabLayout L = slab_layout(m->sys);
-    m->slab = allocate(L.total, "allocation of a batch member");
+    m->slab = takeSlab(L.total, m->slab_class);
     m->bytes = L.total;
     try {
         // staged through pinned memory, so that admitting a member never
@@ -212,13 +248,18 @@ uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age)
         mPinned.record(k, mStream);
         mPinned.give(k);
     } catch (...) {
-        release(m->slab);
+        giveSlab(m->slab, m->slab_class);
         throw;
     }
     // The upload went through pinned staging, so the host copies are done
-    // with: a member keeps only what its device state does not hold -- the
-    // bond lists (for the analyzers) and the slot of every id (to unsort its
-    // downloads); positions come back into arrays sized when they arrive.
+    // with: a member keeps only its sizes, geometry and colour segments, and
+    // its bond lists for the analyzers (downloads are unsorted and unwrapped
+    // on the device; positions come back into arrays sized when they arrive).
+    if (!keep_topology) {
+        drop(m->sys.nnbr);
+        drop(m->sys.nbr);
+    }
+    drop(m->sys.to_sorted);
     drop(m->sys.ppos_sorted);
     drop(m->sys.ids_sorted);
     drop(m->sys.image_sorted);
@@ -241,8 +282,8 @@ void BatchEngine::remove(uint32_t id)
     Member &m = member(id);
     if (!m.slab)
         return;
-    // stream order: launches already queued finish before the memory is freed
-    release(m.slab);
+    // stream order: launches already queued finish before the memory is reused
+    giveSlab(m.slab, m.slab_class);
     m.slab = nullptr;
     m.analyzer.reset();
     mBytes -= m.bytes;
@@ -256,14 +297,6 @@ void BatchEngine::forget(uint32_t id)
     if (mMembers[id]->slab)
         remove(id);
     mMembers[id].reset();
-}
-
-std::vector<int32_t> BatchEngine::takePositions(uint32_t id)
-{
-    Member &m = member(id);
-    std::vector<int32_t> xyz = std::move(m.sys.xyz);
-    forget(id);
-    return xyz;
 }
 
 void BatchEngine::observe(uint32_t id, Observables &out)
@@ -367,6 +400,8 @@ void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
         Member &m = member(ids[j]);
         if (!m.slab)
             throw std::invalid_argument("batch: member was removed");
+        if (m.spent)
+            throw std::logic_error("batch: member ran after its final download");
         const uint32_t cls = kernel_class(m.sys.geom);
         for (uint32_t b = 0; b < m.check_blocks_max; ++b)
             cblk[cls].push_back(BlockEntry{j, b});
@@ -561,49 +596,60 @@ void BatchEngine::run(uint32_t n_mcs, uint32_t chunk)
 
 void BatchEngine::copyBack(uint32_t id)
 {
-    const Download d = download(id);
+    Download d = download(id);
     wait(d);
-    unpack(d);
+    HostSystem &s = member(id).sys;
+    s.xyz.resize(3 * size_t(s.n));
+    copyOut(d, s.xyz.data());
+    give(d);
 }
 
-BatchEngine::Download BatchEngine::download(uint32_t id)
+BatchEngine::Download BatchEngine::download(uint32_t id, bool final)
 {
     Member &m = member(id);
     if (!m.slab)
         throw std::invalid_argument("batch: member was removed");
     hip_check(hipSetDevice(mGpu), "hipSetDevice");
     const HostSystem &s = m.sys;
-    const size_t pos_bytes = sizeof(uint32_t) * s.n_pad, img_bytes = sizeof(int32_t) * 3 * size_t(s.n_pad);
+    const size_t bytes = sizeof(int32_t) * 3 * size_t(s.n);
     Download d;
-    d.member = id;
-    d.buf = mPinned.take(pos_bytes + img_bytes);
-    char *dst = static_cast<char *>(mPinned.data(d.buf));
-    hip_check(hipMemcpyAsync(dst, m.dev.buf.ppos, pos_bytes, hipMemcpyDeviceToHost, mStream), "download positions");
-    hip_check(hipMemcpyAsync(dst + pos_bytes, m.dev.buf.image, img_bytes, hipMemcpyDeviceToHost, mStream),
-              "download images");
+    d.n = s.n;
+    d.buf = mPinned.take(std::max<size_t>(bytes, 16));
+    d.held = true;
+    if (s.n) {
+        // unsort and unwrap on the device (as the plain copy-back does), then
+        // one copy of 12 bytes per monomer into pinned memory
+        // a final download stages in the slab from the lattice on (the
+        // partner table behind it alone holds 32 bytes per slot); the unwrap
+        // reads only positions, ids and images, which lie before it
+        const SlabLayout L = slab_layout(s);
+        int32_t *xyz = final ? static_cast<int32_t *>(slab_part(m.slab, L, kPartLattice))
+                             : static_cast<int32_t *>(allocate(bytes, "allocation of a download buffer"));
+        const uint32_t box[3] = {uint32_t(s.box[0]), uint32_t(s.box[1]), uint32_t(s.box[2])};
+        hip_check(launch_unwrap(m.dev.buf, box, xyz, mStream), "unwrap kernel");
+        hip_check(hipMemcpyAsync(mPinned.data(d.buf), xyz, bytes, hipMemcpyDeviceToHost, mStream),
+                  "download positions");
+        if (final)
+            m.spent = true;
+        else
+            release(xyz);
+    }
     mPinned.record(d.buf, mStream);
     return d;
 }
 
-void BatchEngine::unwrap_positions(uint32_t id)
+void BatchEngine::copyOut(const Download &d, int32_t *xyz) const
 {
-    HostSystem &s = member(id).sys;
-    s.xyz.resize(3 * size_t(s.n));
-    unpack_positions(s);
+    if (!d.held)
+        throw std::logic_error("batch: download already given back");
+    std::memcpy(xyz, mPinned.data(d.buf), sizeof(int32_t) * 3 * size_t(d.n));
 }
 
-void BatchEngine::unpack(const Download &d, bool unwrap)
+void BatchEngine::give(Download &d)
 {
-    HostSystem &s = member(d.member).sys;
-    const size_t pos_bytes = sizeof(uint32_t) * s.n_pad, img_bytes = sizeof(int32_t) * 3 * size_t(s.n_pad);
-    const char *src = static_cast<const char *>(mPinned.data(d.buf));
-    s.ppos_sorted.resize(s.n_pad);
-    s.image_sorted.resize(3 * size_t(s.n_pad));
-    std::memcpy(s.ppos_sorted.data(), src, pos_bytes);
-    std::memcpy(s.image_sorted.data(), src + pos_bytes, img_bytes);
-    mPinned.give(d.buf);
-    if (unwrap)
-        unwrap_positions(d.member);
+    if (d.held)
+        mPinned.give(d.buf);
+    d.held = false;
 }
 
 void BatchEngine::counters(uint32_t id, uint64_t &accepted, uint64_t &proposals)

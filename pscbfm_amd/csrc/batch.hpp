@@ -45,6 +45,7 @@ public:
     bool ready(size_t k) const;                  // that work has run
     void wait(size_t k) const;
     void give(size_t k) { mBufs[k].held = false; }
+    size_t bytes(size_t k) const { return mBufs[k].bytes; }
 
 private:
     struct Buf {
@@ -65,8 +66,9 @@ public:
     BatchEngine &operator=(const BatchEngine &) = delete;
 
     // Takes a planned system (HostSystem after plan()); allocates and uploads
-    // its device slab at once.  Returns the member id.
-    uint32_t add(HostSystem &&sys, uint64_t seed, int64_t age);
+    // its device slab at once.  Returns the member id.  Without
+    // keep_topology the member drops its bond lists (no observe()).
+    uint32_t add(HostSystem &&sys, uint64_t seed, int64_t age, bool keep_topology = true);
     // Device bytes a planned system would take as a member.
     static size_t bytesFor(const HostSystem &sys);
 
@@ -75,33 +77,34 @@ public:
     // Runs n_mcs MCS on the members listed (others idle).
     void runMembers(const std::vector<uint32_t> &ids, uint32_t n_mcs, uint32_t chunk);
 
-    // Copies a member's positions back into its HostSystem (unsorted, by id).
+    // Copies a member's positions back into its HostSystem's xyz (by id).
     void copyBack(uint32_t id);
-    // The same in two halves: download() queues the copy of the member's
-    // positions into pinned memory behind the work queued so far and returns
-    // at once; once finished(), unpack() fills the member's HostSystem.  The
-    // member may be removed in between (its memory is freed in stream order,
-    // after the copy).
+    // The same in two halves: download() queues, behind the work queued so
+    // far, the unsort and unwrap of the member's positions on the device and
+    // the copy of the result (12 bytes per monomer, by id) into pinned memory,
+    // and returns at once; once finished(), copyOut() reads it.  The member
+    // may be removed in between (its memory is freed in stream order, after
+    // the copy).  give() returns the pinned buffer.
     struct Download {
         size_t buf = 0;
-        uint32_t member = 0;
+        uint32_t n = 0;
+        bool held = false;
     };
-    Download download(uint32_t id);
+    // final = true: the member's last download before it is removed; the
+    // result is staged in the member's own slab (behind its positions, over
+    // the lattice and partner tables it no longer needs), so nothing is
+    // allocated.  The member must not run again.
+    Download download(uint32_t id, bool final = false);
     bool finished(const Download &d) const { return mPinned.ready(d.buf); }
     void wait(const Download &d) const { mPinned.wait(d.buf); }
-    // Copies a finished download into the member's host arrays and frees its
-    // staging buffer; with `unwrap` also unsorts and unwraps them into the
-    // member's positions (unwrap_positions does that later on request).
-    void unpack(const Download &d, bool unwrap = true);
-    void unwrap_positions(uint32_t id);
+    void copyOut(const Download &d, int32_t *xyz) const;
+    void give(Download &d);
+    size_t pinnedBytes(const Download &d) const { return d.held ? mPinned.bytes(d.buf) : 0; }
     // Frees a member's device memory; its id is not reused.
     void remove(uint32_t id);
-    // Drops a removed member's host arrays too (result taken or not wanted);
+    // Drops a member's host arrays too (removing it first if it is active);
     // its id stays unused.  No-op for a member already forgotten.
     void forget(uint32_t id);
-    // Unsorted, unwrapped positions of a downloaded member, moved out of the
-    // engine (the member's host arrays are dropped with them).
-    std::vector<int32_t> takePositions(uint32_t id);
     // Host memory the members' systems still hold.
     size_t hostBytes() const;
     // Members still tracked (added and not forgotten).
@@ -144,6 +147,8 @@ private:
         uint64_t substeps = 0;
         void *slab = nullptr;
         size_t bytes = 0;
+        size_t slab_class = 0;   // bytes actually held (the size class)
+        bool spent = false;      // its final download has overwritten its lattice
         BatchSystem dev{};
         uint32_t check_blocks_max = 0;
         std::unique_ptr<DeviceAnalyzer> analyzer;   // made by the first observe()
@@ -156,6 +161,12 @@ private:
     // device memory in stream order (the pool of the device) when available
     void *allocate(size_t bytes, const char *what);
     void release(void *p);
+    // Member slabs: a removed member's slab goes to a free list by size class
+    // and the next member of that class takes it.  Every use is on mStream, so
+    // a new member's upload runs after the old member's last work.
+    void *takeSlab(size_t bytes, size_t &klass);
+    void giveSlab(void *slab, size_t klass);
+    std::vector<std::pair<size_t, void *>> mFreeSlabs;   // (size class, slab)
     template <class T>
     void ensureDevice(T *&p, size_t &cap, size_t n, const char *what);
 

@@ -750,6 +750,26 @@ int bfm_server_submit(bfm_server *s, bfm_updater *h, uint32_t n_mcs, uint64_t *j
     return guarded([&] { *job = s->s->submit(system_for_batch(h->u), h->u.seed(), h->u.getAge(), n_mcs); });
 }
 
+int bfm_server_submit_take(bfm_server *s, bfm_updater *h, uint32_t n_mcs, uint64_t *job)
+{
+    NEED(s, "server handle");
+    NEED(h, "updater handle");
+    NEED(job, "output pointer");
+    return guarded([&] {
+        const uint64_t seed = h->u.seed();
+        const int64_t age = h->u.getAge();
+        if (h->u.initialized()) {
+            *job = s->s->submit(system_for_batch(h->u), seed, age, n_mcs);
+            return;
+        }
+        // validated here if needed, then moved: nothing is copied
+        pscbfm::HostSystem sys = h->u.takeSystem();
+        h->u.setSeed(seed);
+        h->u.setAge(age);
+        *job = s->s->submit(std::move(sys), seed, age, n_mcs);
+    });
+}
+
 int bfm_server_step(bfm_server *s, uint32_t *pending)
 {
     NEED(s, "server handle");
@@ -784,12 +804,7 @@ int bfm_server_job_result(bfm_server *s, uint64_t job, int32_t *xyz, uint32_t n)
 {
     NEED(s, "server handle");
     NEED(xyz, "coordinate array");
-    return guarded([&] {
-        const auto &sys = s->s->result(job);
-        if (n != sys.n)
-            throw std::invalid_argument("bfm_server_job_result: n must equal the job's monomer count");
-        std::memcpy(xyz, sys.xyz.data(), sizeof(int32_t) * 3 * size_t(n));
-    });
+    return guarded([&] { s->s->result(job, xyz, n); });
 }
 
 int bfm_server_job_error(bfm_server *s, uint64_t job, char *buf, uint32_t len)

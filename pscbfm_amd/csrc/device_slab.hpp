@@ -11,21 +11,24 @@
 
 namespace pscbfm {
 
+// Parts in slab order.  The parts an upload writes come first and are
+// followed by the partner table, whose first columns (the largest partner
+// count) end the uploaded prefix; everything after it starts zeroed.
 enum SlabPart {
     kPartPPos,
-    kPartIds,
     kPartImage,
-    kPartNbr,
+    kPartIds,
+    kPartTables,
+    kPartSegBegin,
+    kPartSegCount,
     kPartLattice,
+    kPartNbr,
     kPartClaim0,
     kPartClaim1,
     kPartSurv,
     kPartScount,
     kPartAccepted,
     kPartProposals,
-    kPartTables,
-    kPartSegBegin,
-    kPartSegCount,
     kPartCount
 };
 
@@ -33,21 +36,23 @@ struct SlabLayout {
     size_t offset[kPartCount];
     size_t bytes[kPartCount];
     size_t total;
+    size_t upload;   // bytes of the uploaded prefix (through the used partner columns)
 };
 
 // Offsets (256-byte aligned) of every array of a planned system.
 SlabLayout slab_layout(const HostSystem &s);
-// Bytes of host staging memory upload_slab needs for s: the parts it copies,
-// packed (the partner table only up to the largest partner count).
+// Bytes of host staging memory upload_slab needs for s: the uploaded prefix
+// of the slab, laid out as in the slab.
 size_t upload_bytes(const HostSystem &s);
 
-// Zeroes the slab, uploads positions, partners, lattice, tables and colour
-// segments on `stream` (asynchronously: the HostSystem must stay alive until
-// the stream has passed this point) and returns the kernels' view of it.
-// Clears the slab and uploads the system's arrays on `stream`.  With a pinned
-// `staging` buffer of L.total bytes the host arrays are first copied into it
-// (at their slab offsets), so that the uploads never wait for the stream;
-// the caller keeps the buffer until the uploads have run.
+// Uploads positions, images, ids, tables, colour segments, lattice and
+// partners on `stream` and zeroes the rest of the slab; returns the kernels'
+// view of it.  Without staging the host arrays are copied part by part
+// (asynchronously: the HostSystem must stay alive until the stream has passed
+// this point).  With a pinned `staging` buffer of upload_bytes(s) the arrays
+// are first written into it at their slab offsets, so that the upload is one
+// copy that never waits for the stream, plus one clear; the caller keeps the
+// buffer until the upload has run.
 SweepBuffers upload_slab(const HostSystem &s, void *slab, const SlabLayout &L, uint64_t seed, hipStream_t stream,
                          void *staging = nullptr);
 

@@ -4,6 +4,8 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <stdexcept>
+#include <utility>
 
 #include "updater.hpp"
 
@@ -92,13 +94,33 @@ void Resorter::release()
     mWorkBytes = mSortBytes = 0;
 }
 
+namespace {
+
+// The workspace: the four gathered arrays (each 256-byte aligned, sized as
+// in the slab), then keys, sorted keys, identity, permutation and inverse.
+struct ResortSpace {
+    size_t ppos, ids, image, nbr, keys, words, total;
+};
+
+ResortSpace resort_space(const HostSystem &s, const SlabLayout &L)
+{
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    ResortSpace w{};
+    w.ppos = 0;
+    w.ids = w.ppos + up(L.bytes[kPartPPos]);
+    w.image = w.ids + up(L.bytes[kPartIds]);
+    w.nbr = w.image + up(L.bytes[kPartImage]);
+    w.keys = w.nbr + up(L.bytes[kPartNbr]);
+    w.words = up(size_t(s.n_pad) * sizeof(uint32_t));
+    w.total = w.keys + 5 * w.words;
+    return w;
+}
+
+}  // namespace
+
 void Resorter::reserve(const HostSystem &s, const SlabLayout &L)
 {
-    // the gathered arrays mirror the slab's [ppos, ids, image, nbr] range,
-    // then keys, sorted keys, identity, permutation and its inverse
-    const size_t range = L.offset[kPartNbr] + L.bytes[kPartNbr] - L.offset[kPartPPos];
-    const size_t words = (size_t(s.n_pad) * sizeof(uint32_t) + 255) / 256 * 256;
-    const size_t need = range + 5 * words;
+    const size_t need = resort_space(s, L).total;
     if (mWorkBytes < need) {
         if (mWork)
             hip_check(hipFree(mWork), "hipFree");
@@ -126,18 +148,15 @@ void Resorter::reserve(const HostSystem &s, const SlabLayout &L)
 void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const SweepBuffers &buf, hipStream_t stream)
 {
     const uint32_t n_pad = s.n_pad;
-    const size_t first = L.offset[kPartPPos];
-    const size_t range = L.offset[kPartNbr] + L.bytes[kPartNbr] - first;
-    const size_t words = (size_t(n_pad) * sizeof(uint32_t) + 255) / 256 * 256;
-    if (mWorkBytes < range + 5 * words || !mSortTemp)
+    const ResortSpace sp = resort_space(s, L);
+    if (mWorkBytes < sp.total || !mSortTemp)
         throw std::logic_error("re-sort workspace not reserved for this system");
     char *w = static_cast<char *>(mWork);
-    auto mirror = [&](SlabPart p) { return w + (L.offset[p] - first); };
-    uint32_t *keys = reinterpret_cast<uint32_t *>(w + range);
-    uint32_t *keys_sorted = reinterpret_cast<uint32_t *>(w + range + words);
-    uint32_t *ident = reinterpret_cast<uint32_t *>(w + range + 2 * words);
-    uint32_t *perm = reinterpret_cast<uint32_t *>(w + range + 3 * words);
-    uint32_t *inv = reinterpret_cast<uint32_t *>(w + range + 4 * words);
+    uint32_t *keys = reinterpret_cast<uint32_t *>(w + sp.keys);
+    uint32_t *keys_sorted = reinterpret_cast<uint32_t *>(w + sp.keys + sp.words);
+    uint32_t *ident = reinterpret_cast<uint32_t *>(w + sp.keys + 2 * sp.words);
+    uint32_t *perm = reinterpret_cast<uint32_t *>(w + sp.keys + 3 * sp.words);
+    uint32_t *inv = reinterpret_cast<uint32_t *>(w + sp.keys + 4 * sp.words);
     const uint32_t bits = site_bits(s);
 
     const dim3 grid((n_pad + kResortBlock - 1) / kResortBlock), block(kResortBlock);
@@ -155,11 +174,15 @@ void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const S
     hipLaunchKernelGGL(k_inverse, grid, block, 0, stream, perm, n_pad, inv);
     hip_check(hipGetLastError(), "re-sort inverse");
     hipLaunchKernelGGL(k_gather, grid, block, 0, stream, buf.ppos, buf.ids, buf.image, buf.nbr, perm, inv, n_pad,
-                       reinterpret_cast<uint32_t *>(mirror(kPartPPos)), reinterpret_cast<uint32_t *>(mirror(kPartIds)),
-                       reinterpret_cast<int32_t *>(mirror(kPartImage)), reinterpret_cast<uint32_t *>(mirror(kPartNbr)));
+                       reinterpret_cast<uint32_t *>(w + sp.ppos), reinterpret_cast<uint32_t *>(w + sp.ids),
+                       reinterpret_cast<int32_t *>(w + sp.image), reinterpret_cast<uint32_t *>(w + sp.nbr));
     hip_check(hipGetLastError(), "re-sort gather");
-    hip_check(hipMemcpyAsync(static_cast<char *>(slab) + first, w, range, hipMemcpyDeviceToDevice, stream),
-              "re-sort copy");
+    const std::pair<SlabPart, size_t> back[] = {
+        {kPartPPos, sp.ppos}, {kPartIds, sp.ids}, {kPartImage, sp.image}, {kPartNbr, sp.nbr}};
+    for (const auto &pb : back)
+        hip_check(hipMemcpyAsync(slab_part(slab, L, pb.first), w + pb.second, L.bytes[pb.first],
+                                 hipMemcpyDeviceToDevice, stream),
+                  "re-sort copy");
 }
 
 }  // namespace pscbfm

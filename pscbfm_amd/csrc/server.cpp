@@ -2,6 +2,7 @@
 #include "server.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <sstream>
 #include <stdexcept>
 
@@ -59,9 +60,9 @@ uint64_t Server::submit(HostSystem &&sys, uint64_t seed, int64_t age, uint32_t n
         j->state = kFailed;
         j->error = msg.str();
         j->t_finish = j->t_submit;
+        j->sys = HostSystem();
     } else if (n_mcs == 0) {
         j->state = kDone;
-        j->unwrapped = true;   // the submitted positions are the result
         j->t_admit = j->t_finish = j->t_submit;
     } else {
         mQueue.push_back(id);
@@ -81,9 +82,10 @@ void Server::collect(bool block)
             left.push_back(mRetiring[k]);
             continue;
         }
-        // the copy is taken now (the staging buffer goes back to the pool);
-        // unsorting and unwrapping wait until someone asks for the result
-        mEngine.unpack(j.download, false);
+        // the positions are in pinned memory, unsorted and unwrapped: the
+        // member's host arrays are no longer needed
+        mEngine.forget(j.member);
+        j.admitted = false;
         j.state = kDone;
         j.t_finish = now();
     }
@@ -100,7 +102,7 @@ uint32_t Server::step()
         if (mInUse + j.bytes > mBudget)
             break;
         try {
-            j.member = mEngine.add(std::move(j.sys), j.seed, j.age);   // the engine owns it now
+            j.member = mEngine.add(std::move(j.sys), j.seed, j.age, false);   // the engine owns it now
             j.sys = HostSystem();
             j.admitted = true;
             j.state = kRunning;
@@ -146,7 +148,7 @@ uint32_t Server::step()
         j.done += ran;
         mAttempts += uint64_t(ran) * j.n;
         if (j.done >= j.n_mcs) {
-            j.download = mEngine.download(j.member);
+            j.download = mEngine.download(j.member, true);
             j.age = mEngine.age(j.member);
             mEngine.remove(j.member);   // freed after the download, in stream order
             mInUse -= j.bytes;
@@ -185,7 +187,8 @@ void Server::release(uint64_t id)
     if (j.state == kQueued || j.state == kRunning)
         throw std::invalid_argument("server: release of a job that has not finished");
     if (j.admitted)
-        mEngine.forget(j.member);   // its result was never asked for
+        mEngine.forget(j.member);
+    mEngine.give(j.download);
     mJobs.erase(id);
 }
 
@@ -202,25 +205,22 @@ size_t Server::hostBytes() const
              sizeof(uint32_t) * (s.nbr.capacity() + s.colour.capacity() + s.to_sorted.capacity() +
                                  s.ppos_sorted.capacity() + s.ids_sorted.capacity() + s.nbr_sorted.capacity() +
                                  s.lattice.capacity()) +
-             sizeof(int32_t) * s.image_sorted.capacity();
+             sizeof(int32_t) * s.image_sorted.capacity() + mEngine.pinnedBytes(kv.second->download);
     }
     return b;
 }
 
-const HostSystem &Server::result(uint64_t id)
+void Server::result(uint64_t id, int32_t *xyz, uint32_t n) const
 {
-    Job &j = job(id);
+    const Job &j = job(id);
     if (j.state != kDone)
         throw std::invalid_argument("server: the job has no result (not done)");
-    if (!j.unwrapped) {
-        // the member's host arrays leave the engine with the positions
-        mEngine.unwrap_positions(j.member);
-        j.sys.xyz = mEngine.takePositions(j.member);
-        j.sys.n = j.n;
-        j.unwrapped = true;
-        j.admitted = false;
-    }
-    return j.sys;
+    if (n != j.n)
+        throw std::invalid_argument("server: the result has as many positions as the job has monomers");
+    if (j.download.held)
+        mEngine.copyOut(j.download, xyz);          // unsorted and unwrapped on the device
+    else
+        std::memcpy(xyz, j.sys.xyz.data(), sizeof(int32_t) * 3 * size_t(n));   // a job of 0 MCS
 }
 
 void Server::times(uint64_t id, double &submitted, double &admitted, double &finished) const

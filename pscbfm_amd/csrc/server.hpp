@@ -40,8 +40,9 @@ public:
     State state(uint64_t job) const;
     uint32_t mcsDone(uint64_t job) const;
     const std::string &error(uint64_t job) const;
-    // Final positions of a done job (unsorted and unwrapped on the first request).
-    const HostSystem &result(uint64_t job);
+    // Final positions of a done job (3 * n int32, by id), from pinned memory
+    // where its retirement left them; the job holds them until released.
+    void result(uint64_t job, int32_t *xyz, uint32_t n) const;
     // seconds since the server was created: submission, admission, completion
     void times(uint64_t job, double &submitted, double &admitted, double &finished) const;
 
@@ -58,9 +59,8 @@ public:
 
 private:
     struct Job {
-        HostSystem sys;      // until admission; then the result's positions
+        HostSystem sys;      // until admission (a job of 0 MCS: its result)
         uint32_t n = 0;      // monomers
-        bool unwrapped = false;   // sys.xyz holds the final positions
         uint64_t seed = 0;
         int64_t age = 0;
         uint32_t n_mcs = 0;

@@ -234,6 +234,17 @@ void UpdaterGPUScBFM_AB_Type::validate()
     mValidated = true;
 }
 
+HostSystem UpdaterGPUScBFM_AB_Type::takeSystem()
+{
+    if (mInitialized)
+        throw StateError("takeSystem of an initialized updater");
+    if (!mValidated)
+        validate();
+    HostSystem out = std::move(mSys);
+    cleanup();
+    return out;
+}
+
 void UpdaterGPUScBFM_AB_Type::initialize()
 {
     validate();

@@ -65,6 +65,9 @@ public:
 
     // host half of initialize(): checks and colouring, no device work
     void validate();
+    // The validated system (validated here if needed) moved out of an updater
+    // that is not initialized; the updater is left as after cleanup().
+    HostSystem takeSystem();
     void initialize();
     void runSimulationOnGPU(uint32_t n_mcs);
     int32_t getMonomerPositionInX(T_Id i) const { return at(i, 0); }

@@ -94,8 +94,33 @@ def test_server_host_memory_returns_to_baseline(gpu_ready):
     while srv.step() > 0:
         pass
     for k in (0, 1, 2):
-        srv.result(jobs[k])                  # taken: the member's arrays leave the engine
-    assert srv.host_memory()["members"] == 2   # jobs 3 and 4 still hold their members
+        srv.result(jobs[k])
+    held = srv.host_memory()
+    # retired jobs keep only their results (pinned), not their systems
+    assert held["members"] == 0 and held["jobs"] == 6
+    assert held["host_bytes"] >= sum(12 * len(x["xyz"]) for x in built[:5])
     for j in jobs:
         srv.release(j)
     assert srv.host_memory() == {"host_bytes": 0, "jobs": 0, "members": 0}
+
+
+def test_submit_take_hands_the_system_over(gpu_ready):
+    """bfm_server_submit_take moves a validated updater's system into the job
+    (the updater is left empty) and gives the same result as a copied one."""
+    from pscbfm_amd import systems
+    s = systems.linear_melt(32, 100, 16, seed=61)
+    n = len(s["xyz"])
+    srv = Server(0, hbm_budget=0, chunk=50)
+    u_copy = updater_from_system(s, seed=0x5EED0000000000B1, age=3, validate_only=True)
+    u_take = updater_from_system(s, seed=0x5EED0000000000B1, age=3, validate_only=True)
+    a = srv.submit(u_copy, 150, n)
+    b = srv.submit(u_take, 150, n, take=True)
+    assert u_take.getAge() == 3
+    with pytest.raises(Exception):
+        u_take.getColors(n)                  # nothing left in the updater
+    u_copy.getColors(n)                      # a copied updater keeps its system
+    while srv.step() > 0:
+        pass
+    assert np.array_equal(srv.result(a), srv.result(b))
+    want, _, _ = orc.sweep(orc.System(s, 0x5EED0000000000B1), s["xyz"], 3, 150, 0)
+    assert np.array_equal(srv.result(b), want)
