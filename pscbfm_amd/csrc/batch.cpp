@@ -97,18 +97,19 @@ BatchEngine::~BatchEngine()
     hipSetDevice(mGpu);
     if (mStream)
         hipStreamSynchronize(mStream);
+    for (auto &g : mGraphs)
+        hipGraphExecDestroy(g.exec);
     for (int b = 0; b < 2; ++b) {
-        if (mGraph[b])
-            hipGraphExecDestroy(mGraph[b]);
         if (mDone[b])
             hipEventDestroy(mDone[b]);
         if (mHostChunk[b])
             hipHostFree(mHostChunk[b]);
-        if (mDevChunk[b])
-            hipFree(mDevChunk[b]);
-        if (mDevSched[b])
-            hipFree(mDevSched[b]);
+
     }
+    if (mDevChunk)
+        hipFree(mDevChunk);
+    if (mDevSched)
+        hipFree(mDevSched);
     for (auto &m : mMembers)
         if (m && m->slab)
             hipFree(m->slab);
@@ -380,6 +381,43 @@ void BatchEngine::ensureDevice(T *&p, size_t &cap, size_t n, const char *what)
     p = nullptr;
     cap = std::max<size_t>(n, 1);
     p = static_cast<T *>(allocate(sizeof(T) * cap, what));
+    ++mTableGen;   // graphs captured with the old buffer are stale
+}
+
+// Capacity a table that holds `need` entries keeps: its current one while the
+// need lies in [cap/2, cap], else another (decided by the caller).
+static uint32_t table_capacity(uint32_t need, uint32_t cap)
+{
+    if (need == 0 || cap == 0)
+        return need == 0 ? 0u : ~0u;
+    return need <= cap && 2 * uint64_t(need) >= cap ? cap : ~0u;
+}
+
+// The block table of one class at capacity: with XCD ranges, the real
+// blocks are dealt to the 8 XCDs' logical ranges in proportion to the range
+// lengths (each XCD sweeps a contiguous run of them, its idle entries last);
+// without, they fill the prefix.  An idle entry exits at its first test.
+static void lay_out_blocks(const std::vector<BlockEntry> &real, uint32_t cap, bool xcd_ranges,
+                           std::vector<BlockEntry> &out)
+{
+    out.assign(cap, BlockEntry{0u, 0xFFFFFFFFu});
+    const uint64_t n = real.size();
+    if (!xcd_ranges) {
+        std::copy(real.begin(), real.end(), out.begin());
+        return;
+    }
+    const uint32_t per = cap / kXcds, rem = cap % kXcds;
+    uint32_t start = 0, placed = 0;
+    uint64_t through = 0;
+    for (uint32_t x = 0; x < kXcds; ++x) {
+        const uint32_t len = per + (x < rem ? 1u : 0u);
+        through += len;
+        const uint32_t upto = uint32_t(n * through / cap);   // real blocks placed through XCD x
+        for (uint32_t k = placed; k < upto; ++k)
+            out[start + (k - placed)] = real[k];
+        placed = upto;
+        start += len;
+    }
 }
 
 void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
@@ -389,12 +427,8 @@ void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
     // vectors of the previous rebuild must have been copied before reuse.
     hip_check(hipEventSynchronize(mTablesCopied), "hipEventSynchronize (tables)");
     std::vector<BatchSystem> &sys = mHostSys;
-    std::vector<BlockEntry> *cblk = mHostCheckBlk, *pblk = mHostPerformBlk;
+    std::vector<BlockEntry> real_c[kClasses], real_p[kClasses];
     sys.clear();
-    for (uint32_t c = 0; c < kClasses; ++c) {
-        cblk[c].clear();
-        pblk[c].clear();
-    }
     mMaxColours = 0;
     for (uint32_t j = 0; j < ids.size(); ++j) {
         Member &m = member(ids[j]);
@@ -404,38 +438,62 @@ void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
             throw std::logic_error("batch: member ran after its final download");
         const uint32_t cls = kernel_class(m.sys.geom);
         for (uint32_t b = 0; b < m.check_blocks_max; ++b)
-            cblk[cls].push_back(BlockEntry{j, b});
+            real_c[cls].push_back(BlockEntry{j, b});
         const uint32_t pb = perform_blocks(m.check_blocks_max, mPerformGroup);
         for (uint32_t b = 0; b < pb; ++b)
-            pblk[cls].push_back(BlockEntry{j, b});
+            real_p[cls].push_back(BlockEntry{j, b});
         sys.push_back(m.dev);
         mMaxColours = std::max(mMaxColours, m.sys.n_colours);
     }
+    // The three tables are resized together (so that a growing population
+    // changes the launch shape once, not three times): when any need leaves
+    // its band [cap/2, cap], every table takes the need plus 1/2 (the exact
+    // need the first time).
+    bool resize = table_capacity(uint32_t(sys.size()), mSysCapN) != mSysCapN;
+    for (uint32_t c = 0; c < kClasses; ++c)
+        resize = resize || table_capacity(uint32_t(real_c[c].size()), mCheckCapN[c]) != mCheckCapN[c] ||
+                 table_capacity(uint32_t(real_p[c].size()), mPerformCapN[c]) != mPerformCapN[c];
+    if (resize) {
+        const bool first = mSysCapN == 0;
+        auto fit = [first](size_t need) {
+            const uint64_t c = first ? need : need + need / 2;
+            return need ? uint32_t((c + 63) / 64 * 64) : 0u;
+        };
+        mSysCapN = fit(sys.size());
+        for (uint32_t c = 0; c < kClasses; ++c) {
+            mCheckCapN[c] = fit(real_c[c].size());
+            mPerformCapN[c] = fit(real_p[c].size());
+        }
+    }
+    // idle systems: no colours, so the schedule gives them no work
+    BatchSystem idle = sys.front();
+    idle.n_colours = 0;
+    sys.resize(mSysCapN, idle);
     ensureDevice(mDevSys, mSysCap, sys.size(), "allocation of the batch systems");
     hip_check(hipMemcpyAsync(mDevSys, sys.data(), sizeof(BatchSystem) * sys.size(), hipMemcpyHostToDevice, mStream),
               "upload batch systems");
     for (uint32_t c = 0; c < kClasses; ++c) {
-        mCheckBlocks[c] = uint32_t(cblk[c].size());
-        mPerformBlocks[c] = uint32_t(pblk[c].size());
-        if (cblk[c].empty())
+        if (!mCheckCapN[c])
             continue;
-        ensureDevice(mDevCheckBlk[c], mCheckCap[c], cblk[c].size(), "allocation of the batch blocks");
-        ensureDevice(mDevPerformBlk[c], mPerformCap[c], pblk[c].size(), "allocation of the batch blocks");
-        hip_check(hipMemcpyAsync(mDevCheckBlk[c], cblk[c].data(), sizeof(BlockEntry) * cblk[c].size(),
+        lay_out_blocks(real_c[c], mCheckCapN[c], mXcdRanges, mHostCheckBlk[c]);
+        lay_out_blocks(real_p[c], mPerformCapN[c], mXcdRanges, mHostPerformBlk[c]);
+        ensureDevice(mDevCheckBlk[c], mCheckCap[c], mCheckCapN[c], "allocation of the batch blocks");
+        ensureDevice(mDevPerformBlk[c], mPerformCap[c], mPerformCapN[c], "allocation of the batch blocks");
+        hip_check(hipMemcpyAsync(mDevCheckBlk[c], mHostCheckBlk[c].data(), sizeof(BlockEntry) * mCheckCapN[c],
                                  hipMemcpyHostToDevice, mStream),
                   "upload batch blocks");
-        hip_check(hipMemcpyAsync(mDevPerformBlk[c], pblk[c].data(), sizeof(BlockEntry) * pblk[c].size(),
+        hip_check(hipMemcpyAsync(mDevPerformBlk[c], mHostPerformBlk[c].data(), sizeof(BlockEntry) * mPerformCapN[c],
                                  hipMemcpyHostToDevice, mStream),
                   "upload batch blocks");
     }
     hip_check(hipEventRecord(mTablesCopied, mStream), "hipEventRecord");
     mCompIds = ids;
-    ++mCompVersion;
 }
 
 void BatchEngine::runChunk(uint32_t mcs)
 {
-    const uint32_t n_sys = uint32_t(mCompIds.size());
+    const uint32_t n_real = uint32_t(mCompIds.size());
+    const uint32_t n_sys = mSysCapN;   // the tables' capacity; systems past n_real idle
     const int b = mBuf;
     mBuf ^= 1;
 
@@ -444,128 +502,120 @@ void BatchEngine::runChunk(uint32_t mcs)
         hip_check(hipEventSynchronize(mDone[b]), "hipEventSynchronize (chunk table)");
     else
         hip_check(hipEventCreateWithFlags(&mDone[b], hipEventDisableTiming), "hipEventCreate");
-    if (n_sys > mChunkCap[b]) {
+    if (n_sys > mHostChunkCap[b]) {
         if (mHostChunk[b])
             hip_check(hipHostFree(mHostChunk[b]), "hipHostFree");
-        release(mDevChunk[b]);
         mHostChunk[b] = nullptr;
-        mDevChunk[b] = nullptr;
         hip_check(hipHostMalloc(&mHostChunk[b], sizeof(ChunkEntry) * n_sys, hipHostMallocDefault),
                   "hipHostMalloc (chunk table)");
-        mDevChunk[b] = static_cast<ChunkEntry *>(allocate(sizeof(ChunkEntry) * n_sys, "allocation of a chunk table"));
-        mChunkCap[b] = n_sys;
-        if (mGraph[b]) {
-            hip_check(hipGraphExecDestroy(mGraph[b]), "hipGraphExecDestroy");
-            mGraph[b] = nullptr;
-        }
+        mHostChunkCap[b] = n_sys;
     }
+    ensureDevice(mDevChunk, mChunkCap, n_sys, "allocation of a chunk table");
 
     // where every member starts this chunk; the kernels derive colour, parity
     // and substep seed of each slot from it
     ChunkEntry *chunk = mHostChunk[b];
-    for (uint32_t j = 0; j < n_sys; ++j) {
+    for (uint32_t j = 0; j < n_real; ++j) {
         Member &M = *mMembers[mCompIds[j]];
         chunk[j].age = uint64_t(M.age);
         chunk[j].substeps = M.substeps;
         M.age += mcs;
         M.substeps += uint64_t(mcs) * M.sys.n_colours;
     }
-    hip_check(hipMemcpyAsync(mDevChunk[b], chunk, sizeof(ChunkEntry) * n_sys, hipMemcpyHostToDevice, mStream),
+    for (uint32_t j = n_real; j < n_sys; ++j)
+        chunk[j] = ChunkEntry{0, 0};
+    hip_check(hipMemcpyAsync(mDevChunk, chunk, sizeof(ChunkEntry) * n_sys, hipMemcpyHostToDevice, mStream),
               "upload chunk table");
     const uint32_t slots = mcs * mMaxColours;
     const size_t entries = size_t(slots) * n_sys;
-    if (entries > mSchedCap[b]) {
-        release(mDevSched[b]);
-        mDevSched[b] = static_cast<SchedEntry *>(allocate(sizeof(SchedEntry) * entries, "allocation of a schedule"));
-        mSchedCap[b] = entries;
-        if (mGraph[b]) {
-            hip_check(hipGraphExecDestroy(mGraph[b]), "hipGraphExecDestroy");
-            mGraph[b] = nullptr;
-        }
-    }
+    ensureDevice(mDevSched, mSchedCap, entries, "allocation of a schedule");
 
     auto launch_all = [&]() {
-        hip_check(launch_schedule(mDevSys, mDevChunk[b], n_sys, mcs, mMaxColours, mDevSched[b], mStream),
+        hip_check(launch_schedule(mDevSys, mDevChunk, n_sys, mcs, mMaxColours, mDevSched, mStream),
                   "schedule kernel");
         for (uint32_t slot = 0; slot < slots; ++slot)
             for (uint32_t c = 0; c < kClasses; ++c) {
-                if (!mCheckBlocks[c])
+                if (!mCheckCapN[c])
                     continue;
-                hip_check(launch_check_batch(c, mDevSys, mDevCheckBlk[c], mCheckBlocks[c], mDevSched[b], n_sys, slot,
+                hip_check(launch_check_batch(c, mDevSys, mDevCheckBlk[c], mCheckCapN[c], mDevSched, n_sys, slot,
                                              mXcdRanges ? 1u : 0u, mStream),
                           "batched check kernel");
-                hip_check(launch_perform_batch(c, mDevSys, mDevPerformBlk[c], mPerformBlocks[c], mDevSched[b], n_sys,
+                hip_check(launch_perform_batch(c, mDevSys, mDevPerformBlk[c], mPerformCapN[c], mDevSched, n_sys,
                                                slot, mXcdRanges ? 1u : 0u, mStream, mPerformGroup),
                           "batched perform kernel");
             }
     };
-    // A composition that has a graph replays it.  A new composition whose
-    // launch sequence has the same shape (MCS, colour slots, kernel classes)
-    // as the graph of this buffer is captured and written into that graph by
-    // an update (new arguments and grid sizes), far cheaper than a new
-    // instantiation; a composition running for the second time is captured
-    // and instantiated; any other chunk is launched kernel by kernel, which
-    // starts it soonest.  The graph of buffer b last ran for chunk k-2, which
-    // the wait above has seen finish.
-    uint32_t class_mask = 0;
-    for (uint32_t c = 0; c < kClasses; ++c)
-        class_mask |= mCheckBlocks[c] ? 1u << c : 0u;
-    const uint64_t shape = (uint64_t(mcs) << 32) | (uint64_t(mPerformGroup) << 16) | (uint64_t(mMaxColours) << 8) |
-                           class_mask;
-    const bool have = mGraph[b] && mGraphVersion[b] == mCompVersion && mGraphShape[b] == shape &&
-                      mGraphXcd[b] == mXcdRanges;
-    const bool same_shape = mGraph[b] && mGraphShape[b] == shape && mGraphXcd[b] == mXcdRanges;
-    const bool reuse = mRanVersion == mCompVersion && mRanShape == shape;
-    mRanVersion = mCompVersion;
-    mRanShape = shape;
-    if (mUseGraphs && (have || same_shape || reuse)) {
-        if (!have) {
-            hipGraph_t g = nullptr;
-            hip_check(hipStreamBeginCapture(mStream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            try {
-                launch_all();
-            } catch (...) {
-                hipStreamEndCapture(mStream, &g);
-                if (g)
-                    hipGraphDestroy(g);
-                throw;
-            }
-            hip_check(hipStreamEndCapture(mStream, &g), "hipStreamEndCapture");
-            bool updated = false;
-            if (mGraph[b] && mGraphShape[b] == shape) {
-                hipGraphNode_t bad = nullptr;
-                hipGraphExecUpdateResult res = hipGraphExecUpdateError;
-                updated = hipGraphExecUpdate(mGraph[b], g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
-                hipGetLastError();   // a refused update is not an error of this call
-                if (updated)
-                    ++mGraphUpdates;
-            }
-            if (!updated) {
-                if (mGraph[b]) {
-                    hip_check(hipGraphExecDestroy(mGraph[b]), "hipGraphExecDestroy");
-                    mGraph[b] = nullptr;
-                }
-                const hipError_t e = hipGraphInstantiate(&mGraph[b], g, nullptr, nullptr, 0);
-                if (e != hipSuccess) {
-                    hipGraphDestroy(g);
-                    hip_check(e, "hipGraphInstantiate");
-                }
-                ++mGraphInstantiations;
-            }
-            hipGraphDestroy(g);
-            mGraphVersion[b] = mCompVersion;
-            mGraphShape[b] = shape;
-            mGraphXcd[b] = mXcdRanges;
-        }
-        hip_check(hipGraphLaunch(mGraph[b], mStream), "hipGraphLaunch");
-        ++mGraphLaunches;
-    } else {
-        launch_all();
-    }
     uint32_t classes = 0;
     for (uint32_t c = 0; c < kClasses; ++c)
-        classes += mCheckBlocks[c] ? 1u : 0u;
+        classes += mCheckCapN[c] ? 1u : 0u;
     mKernelLaunches += 2ull * slots * classes + 1;
+    if (!mUseGraphs) {
+        launch_all();
+        hip_check(hipEventRecord(mDone[b], mStream), "hipEventRecord");
+        return;
+    }
+    // A chunk's launches depend on the table capacities, the chunk length,
+    // the colour slots and the launch configuration, not on which members
+    // fill the tables: one graph per such shape.
+    uint64_t key[3] = {(uint64_t(mcs) << 32) | (uint64_t(mMaxColours) << 16) | (uint64_t(mPerformGroup) << 8) |
+                           (mXcdRanges ? 1u : 0u),
+                       uint64_t(n_sys), 0};
+    for (uint32_t c = 0; c < kClasses; ++c)
+        key[2] = key[2] * 1000003u + (uint64_t(mCheckCapN[c]) << 24) + mPerformCapN[c];
+    CachedGraph *g = nullptr;
+    for (auto &x : mGraphs)
+        if (x.gen == mTableGen && x.key[0] == key[0] && x.key[1] == key[1] && x.key[2] == key[2])
+            g = &x;
+    if (!g) {
+        hipGraph_t graph = nullptr;
+        hip_check(hipStreamBeginCapture(mStream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            launch_all();
+        } catch (...) {
+            hipStreamEndCapture(mStream, &graph);
+            if (graph)
+                hipGraphDestroy(graph);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(mStream, &graph), "hipStreamEndCapture");
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        hip_check(e, "hipGraphInstantiate");
+        ++mGraphInstantiations;
+        // Drop stale graphs, then the least recently used beyond the cache
+        // size.  A graph's last launch may be chunk k-1, still queued or
+        // running (chunk k-2 has finished, its event was waited on above):
+        // its executable must outlive it, so wait for chunk k-1 before
+        // destroying any.
+        const bool evict = mGraphs.size() >= kGraphCache ||
+                           std::any_of(mGraphs.begin(), mGraphs.end(),
+                                       [&](const CachedGraph &x) { return x.gen != mTableGen; });
+        if (evict && mDone[b ^ 1])
+            hip_check(hipEventSynchronize(mDone[b ^ 1]), "hipEventSynchronize (graph in flight)");
+        for (size_t k = mGraphs.size(); k-- > 0;)
+            if (mGraphs[k].gen != mTableGen) {
+                hip_check(hipGraphExecDestroy(mGraphs[k].exec), "hipGraphExecDestroy");
+                mGraphs.erase(mGraphs.begin() + long(k));
+            }
+        if (mGraphs.size() >= kGraphCache) {
+            size_t old = 0;
+            for (size_t k = 1; k < mGraphs.size(); ++k)
+                if (mGraphs[k].used < mGraphs[old].used)
+                    old = k;
+            hip_check(hipGraphExecDestroy(mGraphs[old].exec), "hipGraphExecDestroy");
+            mGraphs.erase(mGraphs.begin() + long(old));
+        }
+        CachedGraph x;
+        x.exec = exec;
+        std::copy(key, key + 3, x.key);
+        x.gen = mTableGen;
+        mGraphs.push_back(x);
+        g = &mGraphs.back();
+    }
+    g->used = ++mUse;
+    hip_check(hipGraphLaunch(g->exec, mStream), "hipGraphLaunch");
+    ++mGraphLaunches;
     hip_check(hipEventRecord(mDone[b], mStream), "hipEventRecord");
 }
 

@@ -122,7 +122,12 @@ public:
     void counters(uint32_t id, uint64_t &accepted, uint64_t &proposals);
     hipStream_t stream() const { return mStream; }
     void setUseGraphs(bool on) { mUseGraphs = on; }
-    void setXcdRanges(bool on) { mXcdRanges = on; }
+    void setXcdRanges(bool on)
+    {
+        if (on != mXcdRanges)
+            mCompIds.clear();   // the block tables are laid out for the placement
+        mXcdRanges = on;
+    }
     bool xcdRanges() const { return mXcdRanges; }
     // check waves per perform wave (2, 4 or 8); takes effect at the next chunk
     void setPerformGroup(uint32_t group);
@@ -184,35 +189,47 @@ private:
     std::vector<BlockEntry> mHostCheckBlk[kClasses], mHostPerformBlk[kClasses];
     hipEvent_t mTablesCopied = nullptr;
     BatchSystem *mDevSys = nullptr;
-    size_t mSysCap = 0;
+    size_t mSysCap = 0;          // allocated entries
     uint32_t mMaxColours = 0;
     // per kernel-variant class: block -> system tables
     BlockEntry *mDevCheckBlk[kClasses] = {}, *mDevPerformBlk[kClasses] = {};
-    uint32_t mCheckBlocks[kClasses] = {}, mPerformBlocks[kClasses] = {};
-    size_t mCheckCap[kClasses] = {}, mPerformCap[kClasses] = {};
+    size_t mCheckCap[kClasses] = {}, mPerformCap[kClasses] = {};   // allocated entries
 
     // chunk tables of the last two chunks (double-buffered so that the host
     // can write chunk k+1's table while chunk k runs)
     int mBuf = 0;
     ChunkEntry *mHostChunk[2] = {nullptr, nullptr};   // pinned
-    ChunkEntry *mDevChunk[2] = {nullptr, nullptr};
-    size_t mChunkCap[2] = {0, 0};
-    SchedEntry *mDevSched[2] = {nullptr, nullptr};    // written by the schedule kernel
-    size_t mSchedCap[2] = {0, 0};
+    size_t mHostChunkCap[2] = {0, 0};
+    // one device chunk table and schedule: the upload and the schedule kernel
+    // of chunk k+1 are queued behind chunk k's kernels, which read them
+    ChunkEntry *mDevChunk = nullptr;
+    size_t mChunkCap = 0;
+    SchedEntry *mDevSched = nullptr;    // written by the schedule kernel
+    size_t mSchedCap = 0;
     hipEvent_t mDone[2] = {nullptr, nullptr};
 
-    // captured chunks, one per schedule buffer
+    // Tables are sized to capacities, with idle entries (a block entry that
+    // exits at once, a system with no colours) beyond the composition, so a
+    // chunk's launch sequence depends only on the capacities: one captured
+    // graph per shape is replayed whatever members come and go.  Capacities
+    // grow by 1/2 over the need and shrink when the need halves.
+    uint32_t mSysCapN = 0;
+    uint32_t mCheckCapN[kClasses] = {}, mPerformCapN[kClasses] = {};
+    uint64_t mTableGen = 0;   // bumped whenever a table the graphs read is reallocated
     bool mUseGraphs = true;
-    hipGraphExec_t mGraph[2] = {nullptr, nullptr};
-    uint64_t mGraphShape[2] = {~0ull, ~0ull};   // MCS, colour slots, classes of the graph
-    uint64_t mGraphVersion[2] = {~0ull, ~0ull};
-    bool mGraphXcd[2] = {false, false};
+    struct CachedGraph {
+        hipGraphExec_t exec = nullptr;
+        uint64_t key[3] = {0, 0, 0};
+        uint64_t gen = 0;
+        uint64_t used = 0;
+    };
+    std::vector<CachedGraph> mGraphs;   // at most kGraphCache, least recently used evicted
+    static constexpr size_t kGraphCache = 8;
+    uint64_t mUse = 0;
     bool mXcdRanges = true;
     uint32_t mPerformGroup = kPerformGroup;
     uint64_t mGraphLaunches = 0, mKernelLaunches = 0;
     uint64_t mGraphUpdates = 0, mGraphInstantiations = 0;
-    uint64_t mCompVersion = 0;
-    uint64_t mRanVersion = ~0ull, mRanShape = ~0ull;   // the chunk before
 };
 
 }  // namespace pscbfm
