@@ -56,11 +56,20 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+TRAFFIC_SUMMARIES = {
+    "large_melt": "k_check_large_melt_hbm_traffic.md",
+    "small_melt": "k_check_small_melt_trace_and_traffic.md",
+    "network": "k_check_network_trace_and_traffic.md",
+}
+
+
 def measured_traffic(workload: str):
-    """HBM bytes per k_check launch from the committed rocprofv3 PMC summary."""
+    """HBM bytes per k_check launch from the committed rocprofv3 PMC summary
+    of the workload (FETCH_SIZE and WRITE_SIZE, scaled by the copy kernel)."""
     import re
-    f = ROOT / "profiles" / "k_check_large_melt_hbm_traffic.md"
-    if workload != "large_melt" or not f.exists():
+    name = TRAFFIC_SUMMARIES.get(workload)
+    f = ROOT / "profiles" / name if name else None
+    if f is None or not f.exists():
         return None
     m = re.search(r"```json\n(.*?)\n```", f.read_text(), flags=re.S)
     return json.loads(m.group(1))["traffic_bytes"] if m else None
@@ -116,7 +125,8 @@ def batched_leg(args) -> dict:
     log(f"batched: {len(built)} systems, {n_tot} monomers, built in {time.perf_counter() - t:.1f} s")
 
     # plain: one initialized updater per system, run one after the other
-    plain = [updater_from_system(x, seed=sp["seed"], gpu=0) for x, sp in zip(built, specs)]
+    # (--plain-mcs 0 skips it, for profiled runs of the batched kernels)
+    plain = [updater_from_system(x, seed=sp["seed"], gpu=0) for x, sp in zip(built, specs)] if args.plain_mcs else []
     for u in plain:
         u.getStats()
     t0 = time.perf_counter()
@@ -125,10 +135,12 @@ def batched_leg(args) -> dict:
     for u in plain:
         u.getStats()                  # synchronizes each updater's stream
     t_plain = time.perf_counter() - t0
-    plain_rate = n_tot * args.plain_mcs / t_plain
-    log(f"plain per-system: {args.plain_mcs} MCS each in {t_plain:.2f} s -> {plain_rate:.4g} attempts/s")
+    plain_rate = n_tot * args.plain_mcs / t_plain if args.plain_mcs else None
+    if plain:
+        log(f"plain per-system: {args.plain_mcs} MCS each in {t_plain:.2f} s -> {plain_rate:.4g} attempts/s")
 
     b = Batch(0)
+    b.set_graphs(not args.no_graphs)
     ids = [b.add(updater_from_system(x, seed=sp["seed"], validate_only=True), len(x["xyz"]))
            for x, sp in zip(built, specs)]
     b.synchronize()
@@ -151,8 +163,9 @@ def batched_leg(args) -> dict:
     log(f"batched: {args.batch_mcs} MCS in {t_batched:.2f} s -> {rate:.4g} attempts/s, "
         f"{exact}/{len(built)} states bit-exact vs plain")
     return {"systems": len(built), "monomers": n_tot, "mcs": args.batch_mcs, "chunk": args.chunk,
+            "graphs": not args.no_graphs,
             "value": rate, "unit": "attempts/s", "plain_value": plain_rate, "plain_mcs": args.plain_mcs,
-            "speedup_vs_plain": rate / plain_rate, "bit_exact_systems": exact}
+            "speedup_vs_plain": rate / plain_rate if plain_rate else None, "bit_exact_systems": exact}
 
 
 def served_leg(args, capacity: float) -> dict:
@@ -257,8 +270,10 @@ def main() -> int:
                     help="skip the PCIe-inclusive call after the timed region (for counter runs)")
     ap.add_argument("--cpu-seconds", type=float, default=24.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--time-every", type=int, default=8,
-                    help="time the kernels of every n-th substep with HIP events")
+    ap.add_argument("--time-every", type=int, default=0,
+                    help="time the kernels of every n-th substep with HIP events (0: 8 for systems of a million "
+                         "monomers or more, 128 for smaller ones, whose kernels last a few microseconds and whose "
+                         "launch pipeline a timed launch would stall)")
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-mcs", type=int, default=1000)
     ap.add_argument("--plain-mcs", type=int, default=200)
@@ -268,6 +283,8 @@ def main() -> int:
     ap.add_argument("--steady-jobs", type=int, default=1024,
                     help="jobs of one more served point at capacity (0 = skip)")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="batched leg without chunk graphs (kernel by kernel; the same kernels, for profilers)")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the plain copy kernel that calibrates HBM counters (0 = skip)")
@@ -309,7 +326,8 @@ def main() -> int:
     # HIP events around the kernels of every 8th substep of the timed region:
     # live kernel durations without slowing the launch pipeline of the rest
     # (short runs time every substep, so that some launches are timed)
-    every = args.time_every if args.steps * n_colors >= 4 * args.time_every else 1
+    every = args.time_every or (8 if n >= 1_000_000 else 128)
+    every = every if args.steps * n_colors >= 4 * every else 1
     t = time.perf_counter()
     u.runResident(args.equilibrate)
     u.getStats()
@@ -391,6 +409,7 @@ def main() -> int:
         log(f"runSimulationOnGPU({pcie_mcs}) with copy-back: {dt_pcie * 1e3:.1f} ms -> "
             f"{pcie_mcs * n / dt_pcie:.4g} attempts/s")
     u.cleanup()
+    out["roofline"]["timed_every_substeps"] = every
     if not args.no_batched:
         out["batched"] = batched_leg(args)
         if not args.no_served:

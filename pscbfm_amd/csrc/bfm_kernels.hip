@@ -64,6 +64,44 @@ __device__ __forceinline__ T &element(gptr<T> p)
     return *(T *)p;
 }
 
+// An empty asm statement that takes wave-uniform values in SGPRs: the loads
+// that produce them must have completed where it stands.  The batched kernels
+// place it before their exit tests, which the compiler would otherwise let the
+// loads of a member's buffers sink below (each sunk group is one more
+// dependent level of scalar loads before a block's first data load).
+__device__ __forceinline__ void in_sgprs(uint32_t v) { asm volatile("" ::"s"(v)); }
+template <class T>
+__device__ __forceinline__ void in_sgprs(T *p)
+{
+    asm volatile("" ::"s"(p));
+}
+
+// Everything check_body and perform_body read from a member's buffers.
+__device__ __forceinline__ void buffers_in_sgprs(const SweepBuffers &b)
+{
+    in_sgprs(b.ppos);
+    in_sgprs(b.ids);
+    in_sgprs(b.image);
+    in_sgprs(b.nbr);
+    in_sgprs(b.n_pad);
+    in_sgprs(b.lattice);
+    in_sgprs(b.lattice_words);
+    in_sgprs(b.claim0);
+    in_sgprs(b.claim1);
+    in_sgprs(b.surv);
+    in_sgprs(b.scount);
+    in_sgprs(b.accepted);
+    in_sgprs(b.proposals);
+    in_sgprs(b.tables);
+    in_sgprs(b.rng);
+    in_sgprs(uint32_t(b.seed));
+    in_sgprs(uint32_t(b.seed >> 32));
+    for (int a = 0; a < 3; ++a) {
+        in_sgprs(b.geom.mask[a]);
+        in_sgprs(b.geom.dmask[a]);
+    }
+}
+
 __device__ __forceinline__ int4 moved(const int4 &r, int dir)
 {
     const int32_t s = (dir & 1) ? 1 : -1;
@@ -396,10 +434,17 @@ __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__res
 {
     const uint32_t lb = xcd_ranges ? xcd_logical(blockIdx.x, n_blocks) : blockIdx.x;
     const BlockEntry be = blk_tab[lb];
-    const SchedEntry e = sched[size_t(slot) * n_sys + be.sys];
     // the system's buffers by value: the body's stores cannot alias them, so
-    // they stay in registers as a plain launch's arguments do
+    // they stay in registers as a plain launch's arguments do; loaded before
+    // the slot entry, so that both are in flight together
     const SweepBuffers buf = sys[be.sys].buf;
+    const SchedEntry e = sched[size_t(slot) * n_sys + be.sys];
+    buffers_in_sgprs(buf);
+    in_sgprs(e.begin);
+    in_sgprs(e.sub_seed);
+    in_sgprs(e.parity);
+    in_sgprs(uint32_t(e.age));
+    in_sgprs(uint32_t(e.age >> 32));
     const uint32_t nb = check_blocks(e.count);
     if (e.colour == kNoColour || be.blk >= nb)
         return;
@@ -419,8 +464,11 @@ __global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__r
 {
     const uint32_t lb = xcd_ranges ? xcd_logical(blockIdx.x, n_blocks) : blockIdx.x;
     const BlockEntry be = blk_tab[lb];
-    const SchedEntry e = sched[size_t(slot) * n_sys + be.sys];
     const SweepBuffers buf = sys[be.sys].buf;
+    const SchedEntry e = sched[size_t(slot) * n_sys + be.sys];
+    buffers_in_sgprs(buf);
+    in_sgprs(e.parity);
+    in_sgprs(e.count);
     const uint32_t cb = check_blocks(e.count);
     if (e.colour == kNoColour || be.blk >= perform_blocks(cb, kGroup))
         return;
