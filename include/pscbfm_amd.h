@@ -306,7 +306,9 @@ int bfm_batch_run_members(bfm_batch *b, const uint32_t *members, uint32_t n_memb
 int bfm_batch_get_positions(bfm_batch *b, uint32_t member, int32_t *xyz, uint32_t n);
 int bfm_batch_get_age(bfm_batch *b, uint32_t member, int64_t *age);
 /* attempts = 0 here (the batch does not track them); accepted and proposals
- * are the member's device counters. */
+ * are the member's device counters; sorts its on-device re-sorts.  A member
+ * keeps its updater's setStepsBetweenSortings period and re-sorts at the same
+ * ages as the updater would alone (a chunk is split where one falls due). */
 int bfm_batch_get_stats(bfm_batch *b, uint32_t member, bfm_stats *out);
 /* bfm_observe for a member (its resident state; waits for the member's queued
  * work). */

@@ -231,7 +231,7 @@ uint32_t BatchEngine::liveMembers() const
     return k;
 }
 
-uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age, bool keep_topology)
+uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age, bool keep_topology, bool may_sort)
 {
     hip_check(hipSetDevice(mGpu), "hipSetDevice");
     auto m = std::make_unique<Member>();
@@ -269,6 +269,11 @@ uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age, bool kee
     drop(m->sys.attribute);
     drop(m->sys.xyz);
     drop(m->sys.colour);
+    if (m->sys.spatial_order && m->sys.steps_between_sortings > 0 && may_sort) {
+        m->sort_period = m->sys.steps_between_sortings;
+        m->last_sort_age = age;   // laid out in spatial order by plan()
+        mResorter.reserve(m->sys, L);
+    }
     m->dev.seg_begin = static_cast<const uint32_t *>(slab_part(m->slab, L, kPartSegBegin));
     m->dev.seg_count = static_cast<const uint32_t *>(slab_part(m->slab, L, kPartSegCount));
     m->dev.n_colours = m->sys.n_colours;
@@ -629,10 +634,32 @@ void BatchEngine::runMembers(const std::vector<uint32_t> &ids, uint32_t n_mcs, u
     if (ids != mCompIds)
         rebuildTables(ids);
     for (uint32_t done = 0; done < n_mcs;) {
-        const uint32_t m = std::min(chunk, n_mcs - done);
+        // a chunk ends where a member's next re-sort falls (rarely: every
+        // steps_between_sortings MCS), so each member sorts at the same ages
+        // as it would alone
+        const uint32_t m = sortDue(std::min(chunk, n_mcs - done));
         runChunk(m);
         done += m;
     }
+}
+
+uint32_t BatchEngine::sortDue(uint32_t limit)
+{
+    uint32_t until = limit;
+    for (const uint32_t id : mCompIds) {
+        Member &M = *mMembers[id];
+        if (M.sort_period <= 0)
+            continue;
+        const int64_t P = M.sort_period;
+        if (M.age % P == 0 && M.age != M.last_sort_age) {
+            mResorter.run(M.sys, M.slab, slab_layout(M.sys), M.dev.buf, mStream);
+            M.last_sort_age = M.age;
+            ++M.sorts;
+        }
+        const int64_t next = (M.age / P + 1) * P;   // the next multiple after its age
+        until = uint32_t(std::min<int64_t>(until, next - M.age));
+    }
+    return until;
 }
 
 void BatchEngine::run(uint32_t n_mcs, uint32_t chunk)

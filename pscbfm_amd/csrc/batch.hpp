@@ -26,6 +26,7 @@
 #include "analyzers.hpp"
 #include "bfm_kernels.hpp"
 #include "host_system.hpp"
+#include "resort.hpp"
 
 namespace pscbfm {
 
@@ -68,7 +69,12 @@ public:
     // Takes a planned system (HostSystem after plan()); allocates and uploads
     // its device slab at once.  Returns the member id.  Without
     // keep_topology the member drops its bond lists (no observe()).
-    uint32_t add(HostSystem &&sys, uint64_t seed, int64_t age, bool keep_topology = true);
+    // A member with spatial order re-sorts on the device at the start of
+    // every MCS whose age is a multiple of its sys.steps_between_sortings, as
+    // a plain updater does (never at the age it was added with, its layout is
+    // fresh); may_sort = false promises that it will not run over such an
+    // MCS (the engine then reserves no re-sort workspace for it).
+    uint32_t add(HostSystem &&sys, uint64_t seed, int64_t age, bool keep_topology = true, bool may_sort = true);
     // Device bytes a planned system would take as a member.
     static size_t bytesFor(const HostSystem &sys);
 
@@ -120,6 +126,7 @@ public:
     size_t size() const { return mMembers.size(); }
     size_t deviceBytes() const { return mBytes; }
     void counters(uint32_t id, uint64_t &accepted, uint64_t &proposals);
+    uint64_t sorts(uint32_t id) const { return member(id).sorts; }
     hipStream_t stream() const { return mStream; }
     void setUseGraphs(bool on) { mUseGraphs = on; }
     void setXcdRanges(bool on)
@@ -154,6 +161,9 @@ private:
         size_t bytes = 0;
         size_t slab_class = 0;   // bytes actually held (the size class)
         bool spent = false;      // its final download has overwritten its lattice
+        int64_t sort_period = 0; // re-sort every this many MCS (0: never)
+        int64_t last_sort_age = 0;
+        uint64_t sorts = 0;
         BatchSystem dev{};
         uint32_t check_blocks_max = 0;
         std::unique_ptr<DeviceAnalyzer> analyzer;   // made by the first observe()
@@ -163,6 +173,10 @@ private:
     const Member &member(uint32_t id) const;
     void rebuildTables(const std::vector<uint32_t> &ids);
     void runChunk(uint32_t mcs);
+    // re-sorts the members whose sort falls on their current age; returns
+    // the MCS until the next one falls due among them (at most `limit`)
+    uint32_t sortDue(uint32_t limit);
+    Resorter mResorter;   // shared by the members (stream order serialises them)
     // device memory in stream order (the pool of the device) when available
     void *allocate(size_t bytes, const char *what);
     void release(void *p);

@@ -590,11 +590,14 @@ static pscbfm::HostSystem system_for_batch(pscbfm::UpdaterGPUScBFM_AB_Type &u)
         u.copyBack();
         pscbfm::HostSystem s(u.system());
         pscbfm::plan(s);
+        s.steps_between_sortings = u.getStepsBetweenSortings();
         return s;
     }
     if (!u.validated())
         u.validate();
-    return pscbfm::HostSystem(u.system());
+    pscbfm::HostSystem s(u.system());
+    s.steps_between_sortings = u.getStepsBetweenSortings();
+    return s;
 }
 
 int bfm_batch_add(bfm_batch *b, bfm_updater *h, uint32_t *member)
@@ -677,6 +680,7 @@ int bfm_batch_get_stats(bfm_batch *b, uint32_t member, bfm_stats *out)
         b->e->counters(member, acc, prop);
         out->accepted = acc;
         out->proposals = prop;
+        out->sorts = b->e->sorts(member);
     });
 }
 
@@ -763,7 +767,9 @@ int bfm_server_submit_take(bfm_server *s, bfm_updater *h, uint32_t n_mcs, uint64
             return;
         }
         // validated here if needed, then moved: nothing is copied
+        const int64_t period = h->u.getStepsBetweenSortings();
         pscbfm::HostSystem sys = h->u.takeSystem();
+        sys.steps_between_sortings = period;
         h->u.setSeed(seed);
         h->u.setAge(age);
         *job = s->s->submit(std::move(sys), seed, age, n_mcs);

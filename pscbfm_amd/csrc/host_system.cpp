@@ -14,6 +14,7 @@ void HostSystem::reset()
     box[0] = box[1] = box[2] = 0;
     periodic[0] = periodic[1] = periodic[2] = true;
     n = 0;
+    steps_between_sortings = 5000;
     xyz.clear();
     attribute.clear();
     nnbr.clear();

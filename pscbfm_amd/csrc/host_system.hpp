@@ -26,6 +26,7 @@ struct HostSystem {
     uint8_t forbidden[512];          // 1 = forbidden (copyBondSet)
     bool spatial_order = true;       // lay colours out in Morton order of position
     uint32_t rng = kRngIntHash;      // setRng
+    int64_t steps_between_sortings = 5000;   // re-sort period a batch member keeps (setStepsBetweenSortings)
 
     // ---- derived by plan() --------------------------------------------------
     Geometry geom{};

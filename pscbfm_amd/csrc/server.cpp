@@ -102,7 +102,11 @@ uint32_t Server::step()
         if (mInUse + j.bytes > mBudget)
             break;
         try {
-            j.member = mEngine.add(std::move(j.sys), j.seed, j.age, false);   // the engine owns it now
+            // a re-sort falls inside the job only if one of its MCS after the
+            // first has an age that is a multiple of the sort period
+            const int64_t P = j.sys.steps_between_sortings;
+            const bool may_sort = j.sys.spatial_order && P > 0 && (j.age / P + 1) * P <= j.age + int64_t(j.n_mcs) - 1;
+            j.member = mEngine.add(std::move(j.sys), j.seed, j.age, false, may_sort);   // the engine owns it now
             j.sys = HostSystem();
             j.admitted = true;
             j.state = kRunning;

@@ -116,6 +116,30 @@ def test_batch_with_four_colour_members(gpu_ready):
         assert st["accepted"] > 0
 
 
+def test_batch_members_resort_on_their_schedule(gpu_ready):
+    """Members re-sorted on the device every 7, 11 and 30 MCS and one that never
+    sorts in the run, together in chunks of 25 MCS: each re-sorts at the ages a
+    plain updater would (the chunks are split there), keeps its trajectory and
+    ends where the oracle's replay of it alone ends."""
+    cases = [(systems.linear_melt(32, 100, 16, seed=71), 0x5EED0000000000F7, 0, 7),
+             (systems.cluster_gel(32, 0.8, 0.5, seed=72), 0x5EED0000000000F8, 3, 11),
+             (systems.diamond_network(64, (2, 2, 8), 32, seed=73), 0x5EED0000000000F9, 5, 30),
+             (systems.linear_melt(64, 400, 32, seed=74), 0x5EED0000000000FA, 0, None)]
+    print(f"case: batch with re-sorting members, {len(cases)} members", flush=True)
+    b = Batch(0)
+    ids = [b.add(updater_from_system(s, seed=seed, age=age, validate_only=True, steps_between_sortings=every),
+                 len(s["xyz"])) for s, seed, age, every in cases]
+    b.run(60, chunk=25)
+    for m, (s, seed, age, every) in zip(ids, cases):
+        want, acc, prop = oracle_final(s, seed, age, 60)
+        assert np.array_equal(b.positions(m), want), f"member {m} differs"
+        st = b.stats(m)
+        assert (st["accepted"], st["proposals"]) == (acc, prop)
+        period = every or 5000
+        assert st["sorts"] == sum(1 for a in range(age + 1, age + 60) if a % period == 0)
+    assert b.stats(ids[0])["sorts"] == 8 and b.stats(ids[3])["sorts"] == 0
+
+
 def test_batch_subset_and_removal(gpu_ready):
     cases = mixed_systems(4, seed=9)
     b = Batch(0)

@@ -124,3 +124,22 @@ def test_submit_take_hands_the_system_over(gpu_ready):
     assert np.array_equal(srv.result(a), srv.result(b))
     want, _, _ = orc.sweep(orc.System(s, 0x5EED0000000000B1), s["xyz"], 3, 150, 0)
     assert np.array_equal(srv.result(b), want)
+
+
+def test_served_jobs_resort_on_their_schedule(gpu_ready):
+    """A served job whose updater re-sorts every 40 MCS re-sorts inside its
+    run (chunks of 100 split at ages 40, 80, 120) and ends where the oracle
+    ends; a job beside it that never reaches a sort age is not given a
+    re-sort workspace."""
+    from pscbfm_amd import systems
+    a = systems.linear_melt(32, 100, 16, seed=62)
+    b = systems.cluster_gel(32, 0.8, 0.5, seed=63)
+    srv = Server(0, hbm_budget=0, chunk=100)
+    ja = srv.submit(updater_from_system(a, seed=0x5EED0000000000B2, validate_only=True, steps_between_sortings=40),
+                    150, len(a["xyz"]), take=True)
+    jb = srv.submit(updater_from_system(b, seed=0x5EED0000000000B3, age=7, validate_only=True), 150, len(b["xyz"]))
+    while srv.step() > 0:
+        pass
+    for j, s, seed, age in ((ja, a, 0x5EED0000000000B2, 0), (jb, b, 0x5EED0000000000B3, 7)):
+        want, _, _ = orc.sweep(orc.System(s, seed), s["xyz"], age, 150, 0)
+        assert np.array_equal(srv.result(j), want)
