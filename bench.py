@@ -258,6 +258,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000, help="timed Monte-Carlo steps")
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--autotune", type=int, default=20,
+                    help="MCS per autotuner probe after the relaxation (0: the default launch configuration)")
     ap.add_argument("--equilibrate", type=int, default=1000,
                     help="untimed MCS before the warm-up: the generated start (chains on the even sublattice, "
                          "every bond |b|^2 = 4) relaxes toward the melt's bond-vector distribution")
@@ -332,6 +334,14 @@ def main() -> int:
     u.runResident(args.equilibrate)
     u.getStats()
     log(f"relaxed for {args.equilibrate} MCS in {time.perf_counter() - t:.2f} s")
+    # launch configuration chosen by the autotuner (untimed; the system goes on
+    # relaxing through its probes)
+    if args.autotune > 0:
+        tuned = u.autotune(args.autotune, n_colors)
+    else:
+        tuned = {"perform_group": 4, "xcd_ranges": True, "check_block": [256] * n_colors,
+                 "perform_groups": [4] * n_colors, "morton_order": args.layout == "morton"}
+    log(f"launch configuration: {tuned}")
     u.setKernelTiming(every)
     u.runResident(args.warmup)
     s0 = u.getStats()  # synchronizes the updater's stream
@@ -393,7 +403,11 @@ def main() -> int:
         "config": {**workload_config(args.workload, w), "layout": args.layout, "monomers": n, "colours": n_colors,
                    "mcs": args.steps, "equilibrated_mcs": args.equilibrate,
                    "p_acc": round(p_acc, 4), "p_proposal": round(p_prop, 4),
-                   "rng": "int_hash (the reference's default generator)"},
+                   "rng": "int_hash (the reference's default generator)",
+                   "launch": {"check_workgroup": tuned["check_block"], "perform_group": tuned["perform_groups"],
+                              "xcd_ranges": tuned["xcd_ranges"],
+                              "order_within_colour": "morton" if tuned["morton_order"] else "id",
+                              "autotune_probe_mcs": args.autotune}},
         "roofline": roofline,
     }
     # the drop-in call as a LeMonADE host makes it: MCS plus the copy-back of

@@ -205,15 +205,29 @@ int bfm_set_kernel_timing(bfm_updater *h, int every);
  * is the same either way. */
 int bfm_set_xcd_ranges(bfm_updater *h, int on);
 
-/* Launch configuration (no counterpart in the reference; the trajectory is the
- * same under every one).  perform group: check waves whose survivors one
- * perform wave takes, 2, 4 (default) or 8.  bfm_autotune runs probe_mcs MCS
- * under each of the 6 configurations (perform group x XCD placement), timed
- * with HIP events, keeps the fastest and reports it; the system advances by
- * 7 * probe_mcs MCS meanwhile (one untimed probe first).  On an error the
+/* Launch configuration (the trajectory is the same under every one).
+ * perform group: check waves whose survivors one perform wave takes, 2, 4
+ * (default) or 8, for every colour segment.  bfm_autotune (below) reports the
+ * group it chose for colour 0 and the placement.  On an error the
  * configuration in force before the call stays.  After initialize. */
 int bfm_set_perform_group(bfm_updater *h, uint32_t group);
 int bfm_autotune(bfm_updater *h, uint32_t probe_mcs, uint32_t *group, int *xcd_ranges);
+/* Check workgroup size (128, 256 = default, or 512 threads) for every colour
+ * segment; the reference tunes threads per block per species
+ * (UpdaterGPUScBFM_AB_Type.cu:2940-2990).  Any time (after initialize it
+ * applies to every segment). */
+int bfm_set_check_block(bfm_updater *h, uint32_t threads);
+/* Launch configuration of one colour segment (after initialize): its check
+ * workgroup size and perform group; get also reports the block-to-XCD
+ * placement and whether the monomers of each colour lie in Morton order of
+ * position (1) or in id order (0).  bfm_autotune chooses all of them: the
+ * order within colours by timing both (re-sorting on the device), then per
+ * segment the fastest check workgroup size and perform group from the
+ * segment's own kernel durations, under the faster placement; the system
+ * advances by 9 * probe_mcs MCS (7 without spatial sorting). */
+int bfm_set_segment_config(bfm_updater *h, uint32_t colour, uint32_t threads, uint32_t group);
+int bfm_get_launch_config(const bfm_updater *h, uint32_t colour, uint32_t *threads, uint32_t *group, int *xcd_ranges,
+                          int *morton_order);
 
 /* Runs n_mcs steps without the copy-back of runSimulationOnGPU (positions
  * stay on the device; the age still advances).  For timing loops. */

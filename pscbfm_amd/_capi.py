@@ -139,6 +139,9 @@ SIGNATURES = {
     "bfm_set_kernel_timing": (c_int, [_H, c_int]),
     "bfm_set_spatial_sorting": (c_int, [_H, c_int]),
     "bfm_set_perform_group": (c_int, [_H, c_uint32]),
+    "bfm_set_check_block": (c_int, [_H, c_uint32]),
+    "bfm_set_segment_config": (c_int, [_H, c_uint32, c_uint32, c_uint32]),
+    "bfm_get_launch_config": (c_int, [_H, c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_int), POINTER(c_int)]),
     "bfm_autotune": (c_int, [_H, c_uint32, POINTER(c_uint32), POINTER(c_int)]),
     "bfm_batch_set_perform_group": (c_int, [_H, c_uint32]),
     "bfm_batch_autotune": (c_int, [_H, c_uint32, c_uint32, POINTER(c_uint32), POINTER(c_int)]),

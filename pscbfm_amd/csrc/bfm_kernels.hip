@@ -152,14 +152,14 @@ using DepositFor = typename std::conditional<kCubic, DepositCubic, DepositTable>
 
 // Phase 1 for check block `blk` of `nblk` covering the colour segment
 // [begin, begin+count).  Every wavefront compacts its own survivors into surv.
-template <bool kAllPeriodic, bool kCubic>
+template <bool kAllPeriodic, bool kCubic, uint32_t kThreads = kBlock>
 __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
                                            uint32_t sub_seed, uint32_t parity, uint32_t blk, uint32_t nblk)
 {
     const DepositFor<kCubic> dep{};
     const Geometry &g = buf.geom;
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t gwave = wave_uniform(blk * kWavesPerBlock + (threadIdx.x >> 6));
+    const uint32_t gwave = wave_uniform(blk * (kThreads / kWave) + (threadIdx.x >> 6));
     const gptr<const uint32_t> ppos = as_global<const uint32_t>(buf.ppos);
     const gptr<const uint32_t> ids = as_global(buf.ids);
     const gptr<const uint32_t> nbr = as_global(buf.nbr);
@@ -167,7 +167,7 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
     const gptr<uint32_t> claim_cur = as_global(pick(parity, buf.claim0, buf.claim1));
     const gptr<uint32_t> claim_prev = as_global(pick(parity, buf.claim1, buf.claim0));
 
-    const uint32_t t = blk * kBlock + threadIdx.x;
+    const uint32_t t = blk * kThreads + threadIdx.x;
     const WaveBondMask bonds(buf.tables);
     bool pass = false;
     uint2 entry = make_uint2(0u, 0u);
@@ -260,13 +260,13 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
     // stores; at the densities of a melt a survivor's mark shares every
     // 64-byte line with others, so clearing marks one by one writes more
     const gptr<uint4> cp = reinterpret_cast<gptr<uint4>>(claim_prev);
-    for (uint32_t k = t; k < buf.lattice_words / 4u; k += nblk * kBlock)
+    for (uint32_t k = t; k < buf.lattice_words / 4u; k += nblk * kThreads)
         element(cp + k) = make_uint4(0u, 0u, 0u, 0u);
 }
 
-template <bool kAllPeriodic, bool kCubic>
-__global__ __launch_bounds__(kBlock) void k_check(SweepBuffers buf, uint32_t begin, uint32_t count, uint64_t age,
-                                                  uint32_t sub_seed, uint32_t parity)
+template <bool kAllPeriodic, bool kCubic, uint32_t kThreads>
+__global__ __launch_bounds__(kThreads) void k_check(SweepBuffers buf, uint32_t begin, uint32_t count, uint64_t age,
+                                                    uint32_t sub_seed, uint32_t parity)
 {
     if (!kCubic) {
         load_dep(buf.tables);
@@ -274,22 +274,21 @@ __global__ __launch_bounds__(kBlock) void k_check(SweepBuffers buf, uint32_t beg
     }
     // the grid size from the arguments (launch_check sizes the grid so), not
     // from the launch's implicit arguments, which are read after a branch
-    const uint32_t nblk = check_blocks(count);
+    const uint32_t nblk = check_blocks(count, kThreads);
     const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, nblk) : blockIdx.x;
-    check_body<kAllPeriodic, kCubic>(buf, begin, count, age, sub_seed, parity, blk, nblk);
+    check_body<kAllPeriodic, kCubic, kThreads>(buf, begin, count, age, sub_seed, parity, blk, nblk);
 }
 
 // Phase 2: every wave of perform block `blk` takes the survivors of kGroup
-// consecutive check waves of a check launch of check_blocks blocks.
+// consecutive check waves of a check launch of check_waves wavefronts.
 template <bool kCubic, uint32_t kGroup>
-__device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t check_blocks_, uint32_t parity,
+__device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t check_waves, uint32_t parity,
                                              uint32_t blk)
 {
     const DepositFor<kCubic> dep{};
     const Geometry &g = buf.geom;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t pwave = wave_uniform(blk * kWavesPerBlock + (threadIdx.x >> 6));
-    const uint32_t check_waves = check_blocks_ * kWavesPerBlock;
     const uint32_t cw0 = pwave * kGroup;
     const gptr<const uint32_t> claim = as_global<const uint32_t>(pick(parity, buf.claim0, buf.claim1));
     const gptr<const uint2> surv = as_global<const uint2>(buf.surv);
@@ -374,15 +373,15 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
 }
 
 template <bool kCubic, uint32_t kGroup>
-__global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t check_blocks_, uint32_t parity)
+__global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t check_waves, uint32_t parity)
 {
     if (!kCubic) {
         load_dep(buf.tables);
         __syncthreads();
     }
     // the grid size from the arguments, as in k_check
-    const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, perform_blocks(check_blocks_, kGroup)) : blockIdx.x;
-    perform_body<kCubic, kGroup>(buf, check_blocks_, parity, blk);
+    const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, perform_blocks_for(check_waves, kGroup)) : blockIdx.x;
+    perform_body<kCubic, kGroup>(buf, check_waves, parity, blk);
 }
 
 // ---- many systems per launch ------------------------------------------------
@@ -477,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__r
         load_dep(buf.tables);
         __syncthreads();
     }
-    perform_body<kCubic, kGroup>(buf, cb, e.parity, blk);
+    perform_body<kCubic, kGroup>(buf, cb * kWavesPerBlock, e.parity, blk);
 }
 
 // Unwrapped coordinates of every monomer in original-id order: the
@@ -525,42 +524,57 @@ __global__ __launch_bounds__(kBlock) void k_copy(const uint4 *__restrict__ src, 
 // interval a kernel trace reports, instead of the span between two markers
 // queued around it (which adds the dispatch and the end-of-kernel flush).
 template <class... P, class... A>
-void dispatch(void (*kernel)(P...), dim3 grid, hipStream_t stream, hipEvent_t start, hipEvent_t stop, A... args)
+void dispatch(void (*kernel)(P...), dim3 grid, uint32_t threads, hipStream_t stream, hipEvent_t start,
+              hipEvent_t stop, A... args)
 {
     if (start && stop)
-        hipExtLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, stream, start, stop, 0, P(args)...);
+        hipExtLaunchKernelGGL(kernel, grid, dim3(threads), 0, stream, start, stop, 0, P(args)...);
     else
-        hipLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, stream, P(args)...);
+        hipLaunchKernelGGL(kernel, grid, dim3(threads), 0, stream, P(args)...);
+}
+
+template <uint32_t T>
+void launch_check_threads(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t sub_seed,
+                          uint32_t parity, hipStream_t stream, hipEvent_t start, hipEvent_t stop)
+{
+    const dim3 grid(check_blocks(count, T));
+    const bool periodic = buf.geom.periodic == 7u, cubic = buf.geom.cubic != 0u;
+    auto kernel = periodic ? (cubic ? k_check<true, true, T> : k_check<true, false, T>)
+                           : (cubic ? k_check<false, true, T> : k_check<false, false, T>);
+    dispatch(kernel, grid, T, stream, start, stop, buf, begin, count, age, sub_seed, parity);
 }
 
 hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t sub_seed,
-                        uint32_t parity, hipStream_t stream, hipEvent_t start, hipEvent_t stop)
+                        uint32_t parity, hipStream_t stream, hipEvent_t start, hipEvent_t stop, uint32_t block)
 {
-    const dim3 grid(check_blocks(count));
-    const bool periodic = buf.geom.periodic == 7u, cubic = buf.geom.cubic != 0u;
-    auto kernel = periodic ? (cubic ? k_check<true, true> : k_check<true, false>)
-                           : (cubic ? k_check<false, true> : k_check<false, false>);
-    dispatch(kernel, grid, stream, start, stop, buf, begin, count, age, sub_seed, parity);
+    switch (block) {
+    case 128: launch_check_threads<128>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
+    case 512: launch_check_threads<512>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
+    case 256: launch_check_threads<256>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 template <uint32_t G>
-void launch_perform_group(const SweepBuffers &buf, uint32_t cb, uint32_t parity, hipStream_t stream,
+void launch_perform_group(const SweepBuffers &buf, uint32_t waves, uint32_t parity, hipStream_t stream,
                           hipEvent_t start, hipEvent_t stop)
 {
-    const dim3 grid(perform_blocks(cb, G));
+    const dim3 grid(perform_blocks_for(waves, G));
     auto kernel = buf.geom.cubic != 0u ? k_perform<true, G> : k_perform<false, G>;
-    dispatch(kernel, grid, stream, start, stop, buf, cb, parity);
+    dispatch(kernel, grid, kBlock, stream, start, stop, buf, waves, parity);
 }
 
 hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t parity, hipStream_t stream,
-                          uint32_t group, hipEvent_t start, hipEvent_t stop)
+                          uint32_t group, hipEvent_t start, hipEvent_t stop, uint32_t check_block)
 {
-    const uint32_t cb = check_blocks(count);
+    if (!valid_check_block(check_block))
+        return hipErrorInvalidValue;
+    const uint32_t waves = check_waves(count, check_block);
     switch (group) {
-    case 2: launch_perform_group<2>(buf, cb, parity, stream, start, stop); break;
-    case 8: launch_perform_group<8>(buf, cb, parity, stream, start, stop); break;
-    default: launch_perform_group<4>(buf, cb, parity, stream, start, stop); break;
+    case 2: launch_perform_group<2>(buf, waves, parity, stream, start, stop); break;
+    case 8: launch_perform_group<8>(buf, waves, parity, stream, start, stop); break;
+    default: launch_perform_group<4>(buf, waves, parity, stream, start, stop); break;
     }
     return hipGetLastError();
 }

@@ -52,12 +52,34 @@ __host__ __device__ inline T pick(uint32_t parity, T even, T odd)
     return parity ? odd : even;
 }
 
-__host__ __device__ inline uint32_t check_blocks(uint32_t count) { return count == 0 ? 1u : (count + kBlock - 1) / kBlock; }
-// perform blocks that cover the survivors of a check launch of cb blocks
+// Workgroups of a check launch over `count` monomers with `block` threads each.
+__host__ __device__ inline uint32_t check_blocks(uint32_t count, uint32_t block = kBlock)
+{
+    return count == 0 ? 1u : (count + block - 1) / block;
+}
+// Wavefronts of that launch.  Wave w always covers monomers [64 w, 64 w + 64)
+// of the segment, whatever the workgroup size, so the survivors of a wave (and
+// the perform kernel's work) do not depend on it.
+__host__ __device__ inline uint32_t check_waves(uint32_t count, uint32_t block = kBlock)
+{
+    return check_blocks(count, block) * (block / kWave);
+}
+// Perform blocks (kBlock threads) covering `waves` check waves in groups.
+__host__ __device__ inline uint32_t perform_blocks_for(uint32_t waves, uint32_t group)
+{
+    return ((waves + group - 1) / group + kWavesPerBlock - 1) / kWavesPerBlock;
+}
+// perform blocks that cover the survivors of a check launch of cb blocks of kBlock
 __host__ __device__ inline uint32_t perform_blocks(uint32_t cb, uint32_t group = kPerformGroup)
 {
     return (cb + group - 1) / group;
 }
+// Workgroup sizes of the plain check kernel with a compiled variant (a launch
+// parameter the autotuner chooses among per colour segment); kBlock is the
+// default, and surv / scount are sized for the largest.
+constexpr uint32_t kCheckBlockSizes[] = {128, 256, 512};
+constexpr uint32_t kMaxCheckBlock = 512;
+inline bool valid_check_block(uint32_t b) { return b == 128 || b == 256 || b == 512; }
 // Perform group sizes with a compiled kernel (a launch parameter the
 // autotuners choose among); kPerformGroup is the default.
 constexpr uint32_t kPerformGroups[] = {2, 4, 8};
@@ -71,13 +93,17 @@ inline bool valid_perform_group(uint32_t g) { return g == 2 || g == 4 || g == 8;
 // sub_seed: substep_seed() of this substep (used by the integer-hash RNG).
 // With a start and a stop event, the events take the kernel's own dispatch
 // timestamps (its execution time, as a kernel trace reports it).
+// block: the check workgroup size (one of kCheckBlockSizes).
 hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t sub_seed,
-                        uint32_t parity, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+                        uint32_t parity, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
+                        uint32_t block = kBlock);
 
 // Phase 2: every survivor with no other moved corner in its front moves.
-// count is the colour segment size of the check launch.
+// count is the colour segment size of the check launch, check_block its
+// workgroup size.
 hipError_t launch_perform(const SweepBuffers &buf, uint32_t count, uint32_t parity, hipStream_t stream,
-                          uint32_t group = kPerformGroup, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+                          uint32_t group = kPerformGroup, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
+                          uint32_t check_block = kBlock);
 
 // ---- many systems per launch -------------------------------------------------
 

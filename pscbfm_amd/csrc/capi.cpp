@@ -545,6 +545,36 @@ int bfm_autotune(bfm_updater *h, uint32_t probe_mcs, uint32_t *group, int *xcd_r
     });
 }
 
+int bfm_set_check_block(bfm_updater *h, uint32_t threads)
+{
+    NEED_HANDLE(h);
+    return guarded([&] { h->u.setCheckBlock(threads); });
+}
+
+int bfm_set_segment_config(bfm_updater *h, uint32_t colour, uint32_t threads, uint32_t group)
+{
+    NEED_HANDLE(h);
+    return guarded([&] { h->u.setSegmentConfig(colour, threads, group); });
+}
+
+int bfm_get_launch_config(const bfm_updater *h, uint32_t colour, uint32_t *threads, uint32_t *group, int *xcd_ranges,
+                          int *morton_order)
+{
+    NEED_HANDLE(h);
+    return guarded([&] {
+        uint32_t t = 0, g = 0;
+        h->u.segmentConfig(colour, t, g);
+        if (threads)
+            *threads = t;
+        if (group)
+            *group = g;
+        if (xcd_ranges)
+            *xcd_ranges = h->u.xcdRanges() ? 1 : 0;
+        if (morton_order)
+            *morton_order = h->u.mortonLayout() ? 1 : 0;
+    });
+}
+
 int bfm_batch_create(bfm_batch **out, int gpu)
 {
     if (!out)

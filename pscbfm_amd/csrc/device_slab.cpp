@@ -20,7 +20,8 @@ static uint32_t used_partner_columns(const HostSystem &s)
 
 SlabLayout slab_layout(const HostSystem &s)
 {
-    const size_t blocks = check_blocks(std::max<uint32_t>(s.max_segment, 1));
+    // survivor slots for the check launch with the most (padded) wavefronts
+    const size_t waves = check_waves(std::max<uint32_t>(s.max_segment, 1), kMaxCheckBlock);
     SlabLayout L{};
     L.bytes[kPartPPos] = sizeof(uint32_t) * s.n_pad;
     L.bytes[kPartImage] = sizeof(int32_t) * 3 * size_t(s.n_pad);
@@ -32,8 +33,8 @@ SlabLayout slab_layout(const HostSystem &s)
     L.bytes[kPartNbr] = sizeof(uint32_t) * kMaxConnectivity * s.n_pad;
     L.bytes[kPartClaim0] = sizeof(uint32_t) * s.lattice_words;
     L.bytes[kPartClaim1] = sizeof(uint32_t) * s.lattice_words;
-    L.bytes[kPartSurv] = sizeof(uint2) * kBlock * blocks;
-    L.bytes[kPartScount] = sizeof(uint32_t) * kWavesPerBlock * blocks;
+    L.bytes[kPartSurv] = sizeof(uint2) * kWave * waves;
+    L.bytes[kPartScount] = sizeof(uint32_t) * waves;
     L.bytes[kPartAccepted] = sizeof(unsigned long long) * kStatSlots;
     L.bytes[kPartProposals] = sizeof(unsigned long long) * kStatSlots;
     size_t total = 0;

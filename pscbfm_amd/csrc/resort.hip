@@ -17,13 +17,20 @@ constexpr int kResortBlock = 256;
 // Site of the lower corner of every slot (padding slots included; they lie
 // outside the sorted ranges) and the identity permutation.
 __global__ __launch_bounds__(kResortBlock) void k_keys(const uint32_t *__restrict__ ppos,
+                                                       const uint32_t *__restrict__ ids,
                                                        const Tables *__restrict__ tables, uint32_t n_pad,
-                                                       uint32_t *__restrict__ keys, uint32_t *__restrict__ ident,
-                                                       uint32_t *__restrict__ perm)
+                                                       uint32_t by_id, uint32_t *__restrict__ keys,
+                                                       uint32_t *__restrict__ ident, uint32_t *__restrict__ perm)
 {
     const uint32_t t = blockIdx.x * kResortBlock + threadIdx.x;
     if (t >= n_pad)
         return;
+    if (by_id) {
+        keys[t] = ids[t] & kIdMask;
+        ident[t] = t;
+        perm[t] = t;
+        return;
+    }
     const int4 r = unpack_xyz(ppos[t]);
     uint32_t site = 0;
     const int32_t c[3] = {r.x, r.y, r.z};
@@ -132,7 +139,8 @@ void Resorter::reserve(const HostSystem &s, const SlabLayout &L)
     size_t temp = 0;
     uint32_t *none = nullptr;
     hip_check(rocprim::radix_sort_pairs(nullptr, temp, none, none, none, none,
-                                        size_t(std::max<uint32_t>(s.max_segment, 1)), 0u, site_bits(s)),
+                                        size_t(std::max<uint32_t>(s.max_segment, 1)), 0u,
+                                        std::max<uint32_t>(site_bits(s), kIdBits)),
               "radix sort (size query)");
     temp = std::max<size_t>(temp, 256);
     if (mSortBytes < temp) {
@@ -145,7 +153,8 @@ void Resorter::reserve(const HostSystem &s, const SlabLayout &L)
     }
 }
 
-void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const SweepBuffers &buf, hipStream_t stream)
+void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const SweepBuffers &buf, hipStream_t stream,
+                   bool by_id)
 {
     const uint32_t n_pad = s.n_pad;
     const ResortSpace sp = resort_space(s, L);
@@ -157,10 +166,11 @@ void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const S
     uint32_t *ident = reinterpret_cast<uint32_t *>(w + sp.keys + 2 * sp.words);
     uint32_t *perm = reinterpret_cast<uint32_t *>(w + sp.keys + 3 * sp.words);
     uint32_t *inv = reinterpret_cast<uint32_t *>(w + sp.keys + 4 * sp.words);
-    const uint32_t bits = site_bits(s);
+    const uint32_t bits = by_id ? kIdBits : site_bits(s);
 
     const dim3 grid((n_pad + kResortBlock - 1) / kResortBlock), block(kResortBlock);
-    hipLaunchKernelGGL(k_keys, grid, block, 0, stream, buf.ppos, buf.tables, n_pad, keys, ident, perm);
+    hipLaunchKernelGGL(k_keys, grid, block, 0, stream, buf.ppos, buf.ids, buf.tables, n_pad, by_id ? 1u : 0u, keys,
+                       ident, perm);
     hip_check(hipGetLastError(), "re-sort keys");
     for (uint32_t c = 0; c < s.n_colours; ++c) {
         const uint32_t b = s.seg_begin[c], n = s.seg_count[c];

@@ -32,8 +32,11 @@ public:
     // sweep drivers call it before their run loops, which then allocate nothing.
     void reserve(const HostSystem &s, const SlabLayout &L);
     // Re-sorts the system held in `slab` (layout L, kernels' view buf) on
-    // `stream`, in the workspace reserve() made (radix sort of rocPRIM).
-    void run(const HostSystem &s, void *slab, const SlabLayout &L, const SweepBuffers &buf, hipStream_t stream);
+    // `stream`, in the workspace reserve() made (radix sort of rocPRIM): by
+    // the lattice site of each monomer's corner, or with by_id by original id
+    // (the layout without spatial order).
+    void run(const HostSystem &s, void *slab, const SlabLayout &L, const SweepBuffers &buf, hipStream_t stream,
+             bool by_id = false);
     // Frees the workspace (call with the device current).
     void release();
     size_t workspaceBytes() const { return mWorkBytes + mSortBytes; }

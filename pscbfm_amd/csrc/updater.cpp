@@ -169,6 +169,66 @@ void UpdaterGPUScBFM_AB_Type::setPerformGroup(uint32_t group)
     if (!valid_perform_group(group))
         throw std::invalid_argument("perform group must be 2, 4 or 8");
     mPerformGroup = group;
+    std::fill(mSegGroup.begin(), mSegGroup.end(), group);
+}
+
+void UpdaterGPUScBFM_AB_Type::setCheckBlock(uint32_t block)
+{
+    if (!valid_check_block(block))
+        throw std::invalid_argument("check workgroup size must be 128, 256 or 512");
+    mCheckBlock = block;
+    std::fill(mSegBlock.begin(), mSegBlock.end(), block);
+}
+
+void UpdaterGPUScBFM_AB_Type::setSegmentConfig(uint32_t colour, uint32_t check_block, uint32_t group)
+{
+    if (!mInitialized)
+        throw StateError("segment configuration before initialize");
+    if (colour >= mSegBlock.size())
+        throw std::invalid_argument("no such colour");
+    if (!valid_check_block(check_block))
+        throw std::invalid_argument("check workgroup size must be 128, 256 or 512");
+    if (!valid_perform_group(group))
+        throw std::invalid_argument("perform group must be 2, 4 or 8");
+    mSegBlock[colour] = check_block;
+    mSegGroup[colour] = group;
+}
+
+void UpdaterGPUScBFM_AB_Type::segmentConfig(uint32_t colour, uint32_t &check_block, uint32_t &group) const
+{
+    if (!mInitialized)
+        throw StateError("segment configuration before initialize");
+    if (colour >= mSegBlock.size())
+        throw std::invalid_argument("no such colour");
+    check_block = mSegBlock[colour];
+    group = mSegGroup[colour];
+}
+
+void UpdaterGPUScBFM_AB_Type::resortLayout(bool morton)
+{
+    mResorter.run(mSys, mDevSlab, mLayout, mBuf, mStream, !morton);
+    mMorton = morton;
+    ++mSorts;
+}
+
+void UpdaterGPUScBFM_AB_Type::timedProbe(uint32_t n_mcs, std::vector<double> &check_ms,
+                                         std::vector<double> &perform_ms)
+{
+    collectTimings();
+    const uint32_t every = mTimingEvery;
+    mTimingEvery = 1;
+    std::fill(mColourCheckMs.begin(), mColourCheckMs.end(), 0.0);
+    std::fill(mColourPerformMs.begin(), mColourPerformMs.end(), 0.0);
+    try {
+        runResident(n_mcs, true);
+        collectTimings();
+    } catch (...) {
+        mTimingEvery = every;
+        throw;
+    }
+    mTimingEvery = every;
+    check_ms = mColourCheckMs;
+    perform_ms = mColourPerformMs;
 }
 
 void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
@@ -178,44 +238,76 @@ void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
     if (probe_mcs == 0)
         throw std::invalid_argument("autotune: probe_mcs must be positive");
     hip_check(hipSetDevice(mGpu), "hipSetDevice");
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hip_check(hipEventCreate(&e0), "hipEventCreate");
-    hip_check(hipEventCreate(&e1), "hipEventCreate");
-    const uint32_t group0 = mPerformGroup;
-    const bool xcd0 = mXcdRanges;
-    float best = -1.0f;
-    uint32_t best_group = group0;
-    bool best_xcd = xcd0;
+    const uint32_t nc = mSys.n_colours;
+    const std::vector<uint32_t> block0 = mSegBlock, group0 = mSegGroup;
+    const bool xcd0 = mXcdRanges, morton0 = mMorton;
+    auto restore = [&] {
+        mSegBlock = block0;
+        mSegGroup = group0;
+        setXcdRanges(xcd0);
+    };
     try {
+        std::vector<double> cms, pms;
         // one untimed probe first, so that no configuration pays for cold caches
         runResident(probe_mcs);
-        for (const bool xcd : {true, false})
-            for (const uint32_t g : kPerformGroups) {
-                setXcdRanges(xcd);
-                mPerformGroup = g;
-                hip_check(hipEventRecord(e0, mStream), "hipEventRecord");
-                runResident(probe_mcs);
-                hip_check(hipEventRecord(e1, mStream), "hipEventRecord");
-                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-                float ms = 0.0f;
-                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-                if (best < 0.0f || ms < best) {
-                    best = ms;
-                    best_group = g;
-                    best_xcd = xcd;
-                }
+        // 1. the monomer order within colours (only where spatial sorting is on:
+        // without it the layout is id order throughout)
+        if (mSys.spatial_order) {
+            timedProbe(probe_mcs, cms, pms);
+            double t_now = 0.0;
+            for (uint32_t c = 0; c < nc; ++c)
+                t_now += cms[c] + pms[c];
+            resortLayout(!mMorton);
+            timedProbe(probe_mcs, cms, pms);
+            double t_other = 0.0;
+            for (uint32_t c = 0; c < nc; ++c)
+                t_other += cms[c] + pms[c];
+            if (t_other >= t_now)
+                resortLayout(!mMorton);
+        }
+        // 2. per segment: check workgroup size and perform group, per placement
+        constexpr int kCand = 3;
+        double best_total[2] = {0.0, 0.0};
+        std::vector<uint32_t> best_block[2], best_group[2];
+        for (int x = 0; x < 2; ++x) {
+            setXcdRanges(x == 0);
+            std::vector<double> check_t[kCand], perform_t[kCand];
+            for (int k = 0; k < kCand; ++k) {
+                std::fill(mSegBlock.begin(), mSegBlock.end(), kCheckBlockSizes[k]);
+                std::fill(mSegGroup.begin(), mSegGroup.end(), kPerformGroups[k]);
+                timedProbe(probe_mcs, check_t[k], perform_t[k]);
             }
+            best_block[x].assign(nc, kBlock);
+            best_group[x].assign(nc, kPerformGroup);
+            for (uint32_t c = 0; c < nc; ++c) {
+                int bc = 0, bp = 0;
+                for (int k = 1; k < kCand; ++k) {
+                    if (check_t[k][c] < check_t[bc][c])
+                        bc = k;
+                    if (perform_t[k][c] < perform_t[bp][c])
+                        bp = k;
+                }
+                best_block[x][c] = kCheckBlockSizes[bc];
+                best_group[x][c] = kPerformGroups[bp];
+                best_total[x] += check_t[bc][c] + perform_t[bp][c];
+            }
+        }
+        const int x = best_total[1] < best_total[0] ? 1 : 0;
+        setXcdRanges(x == 0);
+        mSegBlock = best_block[x];
+        mSegGroup = best_group[x];
     } catch (...) {
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-        mPerformGroup = group0;
-        setXcdRanges(xcd0);
+        restore();
+        if (mMorton != morton0) {
+            try {
+                resortLayout(morton0);
+            } catch (...) {
+            }
+        }
         throw;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    mPerformGroup = best_group;
-    setXcdRanges(best_xcd);
+    mPerformGroup = mSegGroup.empty() ? kPerformGroup : mSegGroup[0];
+    mCheckBlock = mSegBlock.empty() ? kBlock : mSegBlock[0];
 }
 
 void UpdaterGPUScBFM_AB_Type::setSpatialSorting(bool on)
@@ -284,6 +376,11 @@ void UpdaterGPUScBFM_AB_Type::initialize()
     mTimedLaunches = 0;
     mLastSortAge = mAge;   // plan() laid the colours out in spatial order
     mSorts = 0;
+    mMorton = s.spatial_order;
+    mSegBlock.assign(s.n_colours, mCheckBlock);
+    mSegGroup.assign(s.n_colours, mPerformGroup);
+    mColourCheckMs.assign(s.n_colours, 0.0);
+    mColourPerformMs.assign(s.n_colours, 0.0);
     mInitialized = true;
 }
 
@@ -298,7 +395,7 @@ void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs, bool timing)
     for (uint32_t m = 0; m < n_mcs; ++m) {
         // periodic re-sort at the start of every mnStepsBetweenSortings-th MCS
         // (runSimulationOnGPU, UpdaterGPUScBFM_AB_Type.cu:3005-3012)
-        if (s.spatial_order && mStepsBetweenSortings > 0 && mAge % mStepsBetweenSortings == 0 &&
+        if (s.spatial_order && mMorton && mStepsBetweenSortings > 0 && mAge % mStepsBetweenSortings == 0 &&
             mAge != mLastSortAge) {
             mResorter.run(s, mDevSlab, mLayout, mBuf, mStream);
             mLastSortAge = mAge;
@@ -316,12 +413,14 @@ void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs, bool timing)
                 for (auto &x : ev)
                     x = takeEvent();
             hip_check(launch_check(mBuf, s.seg_begin[c], s.seg_count[c], age, substep_seed(mSeed, age, k), parity,
-                                   mStream, ev[0], ev[1]),
+                                   mStream, ev[0], ev[1], mSegBlock[c]),
                       "check kernel");
-            hip_check(launch_perform(mBuf, s.seg_count[c], parity, mStream, mPerformGroup, ev[2], ev[3]),
+            hip_check(launch_perform(mBuf, s.seg_count[c], parity, mStream, mSegGroup[c], ev[2], ev[3], mSegBlock[c]),
                       "perform kernel");
-            if (timed)
+            if (timed) {
                 mEvents.insert(mEvents.end(), ev, ev + 4);
+                mEventColours.push_back(c);
+            }
             ++mSubsteps;
         }
         mAttempts += s.n;
@@ -355,10 +454,16 @@ void UpdaterGPUScBFM_AB_Type::collectTimings() const
         mCheckMs += a;
         mPerformMs += b;
         ++mTimedLaunches;
+        const uint32_t c = mEventColours[k / 4];
+        if (c < mColourCheckMs.size()) {
+            mColourCheckMs[c] += a;
+            mColourPerformMs[c] += b;
+        }
     }
     auto &pool = const_cast<std::vector<hipEvent_t> &>(mEventPool);
     pool.insert(pool.end(), mEvents.begin(), mEvents.end());
     mEvents.clear();
+    mEventColours.clear();
 }
 
 void UpdaterGPUScBFM_AB_Type::copyBack()
@@ -451,6 +556,7 @@ void UpdaterGPUScBFM_AB_Type::freeDevice()
     for (auto e : mEvents)
         hipEventDestroy(e);
     mEvents.clear();
+    mEventColours.clear();
     for (auto e : mEventPool)
         hipEventDestroy(e);
     mEventPool.clear();

@@ -84,13 +84,28 @@ public:
     void setKernelTiming(uint32_t every) { mTimingEvery = every; }
     void setXcdRanges(bool on) { mBuf.xcd_ranges = on ? 1u : 0u; mXcdRanges = on; }
     bool xcdRanges() const { return mXcdRanges; }
-    // check waves per perform wave (2, 4 or 8)
+    // check waves per perform wave (2, 4 or 8), for every colour segment
     void setPerformGroup(uint32_t group);
     uint32_t performGroup() const { return mPerformGroup; }
-    // Times probe_mcs MCS under every launch configuration (perform group x
-    // XCD placement) and keeps the fastest.  The run goes on meanwhile (one
-    // untimed probe first, so the age advances by 7 * probe_mcs); the
-    // trajectory is the same under every configuration.
+    // check workgroup size (128, 256 or 512 threads), for every colour segment
+    void setCheckBlock(uint32_t block);
+    // the launch configuration of colour segment c (after initialize)
+    void setSegmentConfig(uint32_t colour, uint32_t check_block, uint32_t group);
+    void segmentConfig(uint32_t colour, uint32_t &check_block, uint32_t &group) const;
+    // monomers of a colour in Morton order of position (true) or in id order
+    bool mortonLayout() const { return mMorton; }
+    // Chooses the launch configuration by timing short probes (the run goes
+    // on meanwhile; the trajectory is the same under every configuration):
+    //  1. the monomer order within colours, Morton versus id (with spatial
+    //     sorting on: the monomers are re-sorted on the device for the id
+    //     probe, and back if Morton wins);
+    //  2. per colour segment, the check workgroup size (128, 256, 512) and
+    //     the perform group (2, 4, 8), from each segment's own kernel
+    //     durations (the check's do not depend on the group, the perform's
+    //     not on the workgroup size), under each XCD placement; the placement
+    //     with the smaller total wins.
+    // One untimed probe first; the age advances by 9 * probe_mcs (7 without
+    // spatial sorting).
     void autotune(uint32_t probe_mcs);
     const HostSystem &system() const { return mSys; }
     bool initialized() const { return mInitialized; }
@@ -117,6 +132,9 @@ private:
     int32_t at(T_Id i, int a) const;
     // the sweep driver; `timing` = honour setKernelTiming
     void runResident(uint32_t n_mcs, bool timing);
+    // every substep of n_mcs MCS timed, per colour
+    void timedProbe(uint32_t n_mcs, std::vector<double> &check_ms, std::vector<double> &perform_ms);
+    void resortLayout(bool morton);
     void freeDevice();
     void collectTimings() const;
 
@@ -145,9 +163,14 @@ private:
     uint32_t mTimingEvery = 0;   // time every n-th substep (0: off)
     bool mXcdRanges = true;
     uint32_t mPerformGroup = kPerformGroup;
+    uint32_t mCheckBlock = kBlock;
+    std::vector<uint32_t> mSegBlock, mSegGroup;   // per colour segment
+    bool mMorton = true;          // current order within colours (with spatial sorting)
     int64_t mStepsBetweenSortings = 5000;   // mnStepsBetweenSortings, UpdaterGPUScBFM_AB_Type.cu:1379
     std::vector<hipEvent_t> mEventPool;          // reusable events
     mutable std::vector<hipEvent_t> mEvents;     // pending (check start, stop, perform start, stop) quadruples
+    mutable std::vector<uint32_t> mEventColours; // colour of each pending quadruple
+    mutable std::vector<double> mColourCheckMs, mColourPerformMs;   // summed per colour
     mutable double mCheckMs = 0.0, mPerformMs = 0.0;
     mutable uint64_t mTimedLaunches = 0;
     hipEvent_t takeEvent();

@@ -170,13 +170,34 @@ class UpdaterGPUScBFM_AB_Type:
         """Check waves per perform wave: 2, 4 (default) or 8."""
         _capi.check(self._lib.bfm_set_perform_group(self._h, int(group)))
 
-    def autotune(self, probe_mcs: int) -> dict:
-        """Times probe_mcs MCS under each launch configuration and keeps the
-        fastest (the system advances by 7 * probe_mcs MCS: one untimed probe, then
-        one per configuration)."""
+    def autotune(self, probe_mcs: int, n_colors: int | None = None) -> dict:
+        """Chooses the order within colours (Morton or id), then per colour
+        segment the check workgroup size and perform group, and the XCD
+        placement, from short timed probes (the system advances by
+        9 * probe_mcs MCS, 7 without spatial sorting).  Returns the chosen
+        configuration (per colour with n_colors)."""
         g, x = c_uint32(), ctypes.c_int()
         _capi.check(self._lib.bfm_autotune(self._h, int(probe_mcs), byref(g), byref(x)))
-        return {"perform_group": g.value, "xcd_ranges": bool(x.value)}
+        out = {"perform_group": g.value, "xcd_ranges": bool(x.value)}
+        if n_colors:
+            cfg = [self.launchConfig(c) for c in range(n_colors)]
+            out.update({"check_block": [c["check_block"] for c in cfg],
+                        "perform_groups": [c["perform_group"] for c in cfg],
+                        "morton_order": cfg[0]["morton_order"]})
+        return out
+
+    def setCheckBlock(self, threads: int) -> None:
+        """Check workgroup size (128, 256 or 512 threads) for every colour segment."""
+        _capi.check(self._lib.bfm_set_check_block(self._h, int(threads)))
+
+    def setSegmentConfig(self, colour: int, threads: int, group: int) -> None:
+        _capi.check(self._lib.bfm_set_segment_config(self._h, int(colour), int(threads), int(group)))
+
+    def launchConfig(self, colour: int) -> dict:
+        t, g, x, m = c_uint32(), c_uint32(), ctypes.c_int(), ctypes.c_int()
+        _capi.check(self._lib.bfm_get_launch_config(self._h, int(colour), byref(t), byref(g), byref(x), byref(m)))
+        return {"check_block": t.value, "perform_group": g.value, "xcd_ranges": bool(x.value),
+                "morton_order": bool(m.value)}
 
     def setXcdRanges(self, on: bool) -> None:
         """Contiguous block range per XCD (default) or round-robin placement;

@@ -196,10 +196,12 @@ def test_autotune_keeps_the_trajectory(gpu_ready):
     seed = 0x5EED0000000000F3
     print(f"case: autotune n={n}", flush=True)
     u = updater_from_system(s, seed=seed, gpu=0)
-    cfg = u.autotune(5)                 # one untimed probe + 6 configurations, 5 MCS each
+    cfg = u.autotune(5, n_colors=2)     # one untimed probe, 2 layout probes, 6 configurations, 5 MCS each
     assert cfg["perform_group"] in (2, 4, 8)
-    assert u.getAge() == 35
-    u.runSimulationOnGPU(65)
+    assert all(b in (128, 256, 512) for b in cfg["check_block"])
+    assert all(g in (2, 4, 8) for g in cfg["perform_groups"])
+    assert u.getAge() == 45
+    u.runSimulationOnGPU(55)
     got = u.getMonomerPositions(n)
     colors, _ = u.getColors(n)
     st = u.getStats()
@@ -207,6 +209,45 @@ def test_autotune_keeps_the_trajectory(gpu_ready):
     want, acc, prop = orc.sweep(orc.System(s, seed, colors), s["xyz"], 0, 100, 0)
     assert np.array_equal(got, want)
     assert (st["accepted"], st["proposals"]) == (acc, prop)
+
+
+@pytest.mark.parametrize("system", ["melt", "network", "walls"])
+def test_every_launch_candidate_keeps_the_trajectory(gpu_ready, system):
+    """Every candidate the autotuner chooses among -- check workgroup size 128,
+    256 or 512, perform group 2, 4 or 8, XCD placement on or off, Morton or id
+    order within colours -- and different configurations per colour segment
+    give the oracle's trajectory bit-exactly."""
+    if system == "melt":
+        s = systems.linear_melt(32, 128, 16, seed=91)
+    elif system == "network":
+        s = diamond_small()
+    else:
+        s = dict(systems.cluster_gel(32, 0.8, 0.5, seed=92), periodic=(1, 0, 1))
+    n = len(s["xyz"])
+    seed = 0x5EED0000000000E9
+    mcs = 12
+    want, acc, prop = orc.sweep(orc.System(s, seed), s["xyz"], 0, mcs, 0)
+    configs = [(b, g, x, m) for b in (128, 256, 512) for g in (2, 4, 8) for x in (True, False) for m in (True, False)]
+    print(f"case: {system}, n={n}, {len(configs)} launch configurations + mixed per-segment ones", flush=True)
+    for b, g, x, m in configs:
+        u = updater_from_system(s, seed=seed, gpu=0, spatial_sorting=m)
+        u.setCheckBlock(b)
+        u.setPerformGroup(g)
+        u.setXcdRanges(x)
+        u.runSimulationOnGPU(mcs)
+        st = u.getStats()
+        got = u.getMonomerPositions(n)
+        u.cleanup()
+        assert np.array_equal(got, want), (b, g, x, m)
+        assert (st["accepted"], st["proposals"]) == (acc, prop), (b, g, x, m)
+    u = updater_from_system(s, seed=seed, gpu=0)
+    _, nc = u.getColors(n)
+    for c in range(nc):
+        u.setSegmentConfig(c, (128, 512, 256, 128)[c % 4], (8, 2, 4, 2)[c % 4])
+    assert [u.launchConfig(c)["check_block"] for c in range(nc)] == [(128, 512, 256, 128)[c % 4] for c in range(nc)]
+    u.runSimulationOnGPU(mcs)
+    assert np.array_equal(u.getMonomerPositions(n), want)
+    u.cleanup()
 
 
 def test_non_cubic_box(gpu_ready):
