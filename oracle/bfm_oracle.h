@@ -143,9 +143,14 @@ int orc_sweep_parallel(const orc_system *sys, int32_t *pos, uint64_t age0, uint3
 /* LeMonADE-style random-sequential updater: per MCS, n attempts; attempt k of
  * MCS `age` draws Philox(seed, (k, age_lo, age_hi, 2)): monomer = w0 % n,
  * direction = w1 % 6; a legal move is applied at once.  This is the CPU
- * baseline and the statistical reference.  Parity with LeMonADE's own
- * UpdaterSimpleSimulator is unpinned (LeMonADE is not available); its move
- * rules are those of the sweep above. */
+ * baseline and the statistical reference.  Pinned by closed-form
+ * trajectories of free and walled monomers from the Philox words
+ * (tests/test_oracle.py) and by its stationary laws, which detailed balance
+ * fixes: uniform over the 108 bond vectors for a free dimer and over the legal
+ * relative positions for two free monomers (tests/test_statistics.py).
+ * Trajectory parity with LeMonADE's own UpdaterSimpleSimulator is unpinned
+ * (LeMonADE is not available, and it draws from r250); its move rules are
+ * those of the sweep above. */
 int orc_random_sequential(const orc_system *sys, int32_t *pos, uint64_t age0,
                           uint32_t n_mcs, uint64_t *accepted);
 

@@ -196,6 +196,59 @@ def test_random_sequential_keeps_invariants():
     assert orc.check_system(sys_, x)[0] == 0
 
 
+_DIRS = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
+
+
+def _rs_words(seed, age, k):
+    # attempt k of MCS `age`: Philox4x32-10 (KAT-pinned above) of the counter
+    # (k, age_lo, age_hi, 2) under the key (seed_lo, seed_hi)
+    return orc.philox([k, age & 0xFFFFFFFF, age >> 32, 2], [seed & 0xFFFFFFFF, seed >> 32])
+
+
+def test_random_sequential_free_monomers_closed_form():
+    """Three free monomers far apart in a periodic 64^3 box never meet within
+    150 MCS, so every attempt is accepted: the trajectory is attempt k moving
+    monomer w0 % 3 along direction w1 % 6, n attempts per MCS, from age0 on."""
+    seed, age0, n_mcs = 0xC0FFEE0000000123, (1 << 32) - 40, 150     # ages cross 2^32
+    start = np.array([[0, 0, 0], [21, 22, 20], [42, 40, 43]], dtype=np.int32)
+    s = {"box": (64, 64, 64), "periodic": (1, 1, 1), "xyz": start,
+         "bonds": np.zeros((0, 2), dtype=np.uint32), "forbidden": orc.standard_bondset()}
+    want = start.astype(np.int64).copy()
+    for age in range(age0, age0 + n_mcs):
+        for k in range(3):
+            w = _rs_words(seed, age, k)
+            want[w[0] % 3] += _DIRS[w[1] % 6]
+        # the premise: no two cubes come near each other on the periodic box
+        for i in range(3):
+            for j in range(i + 1, 3):
+                d = np.abs((want[i] - want[j] + 32) % 64 - 32)
+                assert d.max() > 2
+    x, acc = orc.random_sequential(orc.System(s, seed), start, age0, n_mcs)
+    assert acc == 3 * n_mcs
+    assert np.array_equal(x, want)
+
+
+def test_random_sequential_walls_closed_form():
+    """One monomer in an 8^3 box with walls on x and y and periodic z: a move
+    is accepted iff the cube stays inside, 0 <= x, y <= 6; z runs free."""
+    seed, n_mcs = 31337, 400
+    start = np.array([[0, 6, 3]], dtype=np.int32)
+    s = {"box": (8, 8, 8), "periodic": (0, 0, 1), "xyz": start,
+         "bonds": np.zeros((0, 2), dtype=np.uint32), "forbidden": orc.standard_bondset()}
+    want, n_acc, blocked = start[0].astype(np.int64).copy(), 0, set()
+    for age in range(n_mcs):
+        w = _rs_words(seed, age, 0)
+        r1 = want + _DIRS[w[1] % 6]
+        if 0 <= r1[0] <= 6 and 0 <= r1[1] <= 6:
+            want, n_acc = r1, n_acc + 1
+        else:
+            blocked.add(w[1] % 6)
+    assert blocked == {0, 1, 2, 3}           # all four walls turned moves away
+    x, acc = orc.random_sequential(orc.System(s, seed), start, 0, n_mcs)
+    assert acc == n_acc
+    assert np.array_equal(x[0], want)
+
+
 # ---- invariant checker catches planted mistakes ---------------------------
 def test_check_system_catches_planted_errors():
     s = melt(16, 16, 16, 3)

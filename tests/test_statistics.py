@@ -91,36 +91,136 @@ def test_msd_agree(replicas):
         assert 0.95 < a / b < 1.12, (key, a, b)
 
 
-def test_parallel_substep_is_not_doubly_stochastic():
-    """Exact enumeration for two free monomers of one colour in a periodic 5^3
-    box: the substep's transition matrix (both propose at once, each checked
-    against the old state, both rejected if their new cubes overlap) has
-    column sums != 1, so the uniform distribution of the serial BFM is not
-    its stationary one.  This is the reference's rule; the engine keeps it."""
-    L = 5
-    D = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
+L2 = 8                                    # two-body box: periodic 8^3
+_DIRS = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
 
-    def overlap(a, b):
-        return all(min((a[k] - b[k]) % L, (b[k] - a[k]) % L) <= 1 for k in range(3))
 
-    def add(p, d):
-        return tuple((p[k] + d[k]) % L for k in range(3))
+def _overlap(r):
+    # cubes at relative position r overlap iff every axis is within 1 (periodic)
+    return all(min(v % L2, (-v) % L2) <= 1 for v in r)
 
-    a = (0, 0, 0)   # translation invariance: fix the first monomer's start
-    col = {}
-    for b in itertools.product(range(L), repeat=3):
-        if overlap(a, b):
-            continue
-        for da in D:
-            for db in D:
-                a1, b1 = add(a, da), add(b, db)
-                oka, okb = not overlap(a1, b), not overlap(b1, a)
-                if oka and okb and overlap(a1, b1):
+
+def _pair_states():
+    states = [r for r in itertools.product(range(L2), repeat=3) if not _overlap(r)]
+    return states, {r: i for i, r in enumerate(states)}
+
+
+def _pair_class(r):
+    # orbit of a relative position under the cube's symmetries (both laws below are symmetric)
+    return tuple(sorted(min(v % L2, (-v) % L2) for v in r))
+
+
+def _parallel_substep_law():
+    """Exact transition matrix of one substep of the parallel rule for two
+    free monomers of one colour (UpdaterGPUScBFM_AB_Type.cu:930-1028,
+    :1148-1190): both propose at once, each is checked against the old state,
+    and both are rejected if their new cubes overlap.  Translation is factored
+    out: the state is b's position relative to a.  Returns (P, stationary pi)."""
+    states, ix = _pair_states()
+    n = len(states)
+    P = np.zeros((n, n))
+    for r in states:
+        for da in _DIRS:
+            for db in _DIRS:
+                b1 = tuple(r[k] + db[k] for k in range(3))
+                oka = not _overlap(tuple(r[k] - da[k] for k in range(3)))
+                okb = not _overlap(b1)
+                if oka and okb and _overlap(tuple(b1[k] - da[k] for k in range(3))):
                     oka = okb = False
-                na, nb = (a1 if oka else a), (b1 if okb else b)
-                rel = tuple((nb[k] - na[k]) % L for k in range(3))
-                col[rel] = col.get(rel, 0.0) + 1.0 / 36
-    # summed over translations, the column sum of a relative position is the
-    # mass flowing into it; uniform stationarity needs exactly 1 for each
-    sums = np.array(list(col.values()))
+                na = da if oka else (0, 0, 0)
+                nb = b1 if okb else r
+                P[ix[r], ix[tuple((nb[k] - na[k]) % L2 for k in range(3))]] += 1.0 / 36
+    # stationary law: pi P = pi, sum(pi) = 1
+    A = np.vstack([P.T - np.eye(n), np.ones((1, n))])
+    pi = np.linalg.lstsq(A, np.r_[np.zeros(n), 1.0], rcond=None)[0]
+    return P, pi
+
+
+def _chi2_p(counts, probs):
+    from scipy import stats
+    e = probs * counts.sum()
+    chi = ((counts - e) ** 2 / e).sum()
+    return stats.chi2.sf(chi, len(counts) - 1)
+
+
+def _pair_histogram(algo, seed, n_samples, thin):
+    """Class histogram of b - a sampled every `thin` MCS from the oracle."""
+    states, _ = _pair_states()
+    classes = sorted({_pair_class(r) for r in states})
+    ci = {c: i for i, c in enumerate(classes)}
+    s = {"box": (L2, L2, L2), "periodic": (1, 1, 1), "xyz": np.array([[0, 0, 0], [3, 4, 5]], dtype=np.int32),
+         "bonds": np.zeros((0, 2), dtype=np.uint32), "forbidden": systems.standard_forbidden()}
+    o = orc.System(s, seed)
+    assert o.n_colors == 1
+    x, h = s["xyz"], np.zeros(len(classes))
+    for k in range(n_samples):
+        if algo == "sweep":
+            x, _, _ = orc.sweep(o, x, k * thin, thin, 0)
+        else:
+            x, _ = orc.random_sequential(o, x, k * thin, thin)
+        r = x[1] - x[0]
+        assert not _overlap(r)
+        h[ci[_pair_class(r)]] += 1
+    return h, classes, ci
+
+
+def test_parallel_substep_is_not_doubly_stochastic():
+    """The substep's transition matrix has column sums != 1, so the uniform
+    law of the serial BFM is not its stationary one.  This is the reference's
+    rule; the engine keeps it."""
+    P, pi = _parallel_substep_law()
+    sums = P.sum(axis=0)
+    assert np.allclose(P.sum(axis=1), 1.0)
     assert sums.min() < 0.95 and sums.max() > 1.05
+    n = len(pi)
+    assert 0.5 * np.abs(pi - 1.0 / n).sum() > 0.05       # total-variation distance to uniform
+
+
+def _class_probs(ci, weights):
+    states, _ = _pair_states()
+    q = np.zeros(len(ci))
+    for r, w in zip(states, weights):
+        q[ci[_pair_class(r)]] += w
+    return q
+
+
+def test_two_free_monomers_sample_their_stationary_laws():
+    """Two free monomers in a periodic 8^3 box, 40000 samples 16 MCS apart,
+    binned into the 31 symmetry classes of their relative position.  The
+    serial rule is symmetric with accept-if-legal, so orc_random_sequential
+    must sample the 485 legal relative positions uniformly; the oracle's
+    parallel sweep must instead sample the exact law of the substep above and
+    fail the uniform test by far.  Thresholds: p > 1e-3 where the law holds,
+    p < 1e-20 where it does not (seeded, so the outcome is fixed)."""
+    P, pi = _parallel_substep_law()
+    n = len(pi)
+    for algo, law, other in (("rs", np.full(n, 1.0 / n), pi), ("sweep", pi, np.full(n, 1.0 / n))):
+        h, classes, ci = _pair_histogram(algo, 4242, 40000, 16)
+        assert len(classes) == 31
+        assert _chi2_p(h, _class_probs(ci, law)) > 1e-3, algo
+        assert _chi2_p(h, _class_probs(ci, other)) < 1e-20, algo
+
+
+def test_free_dimer_visits_bond_vectors_uniformly():
+    """A bonded dimer in a periodic 32^3 box under orc_random_sequential:
+    detailed balance of the serial BFM makes every one of the 108 bond vectors
+    equally likely (and no other ever appears).  40000 samples 32 MCS apart;
+    chi^2 over the 108 vectors and over the 6 length/shape classes, p > 1e-3."""
+    f = systems.standard_forbidden()
+    allowed = [d for d in itertools.product(range(-4, 4), repeat=3)
+               if not f[(d[0] & 7) | ((d[1] & 7) << 3) | ((d[2] & 7) << 6)]]
+    ai = {d: i for i, d in enumerate(allowed)}
+    s = {"box": (32, 32, 32), "periodic": (1, 1, 1), "xyz": np.array([[4, 4, 4], [6, 4, 4]], dtype=np.int32),
+         "bonds": np.array([[0, 1]], dtype=np.uint32), "forbidden": f}
+    o = orc.System(s, 99)
+    x, h = s["xyz"], np.zeros(len(allowed))
+    for k in range(40000):
+        x, _ = orc.random_sequential(o, x, k * 32, 32)
+        h[ai[tuple(int(v) for v in x[1] - x[0])]] += 1
+    assert _chi2_p(h, np.full(108, 1 / 108)) > 1e-3
+    shape = [tuple(sorted(abs(v) for v in d)) for d in allowed]
+    kinds = sorted(set(shape))
+    assert len(kinds) == 6
+    hk = np.array([h[[i for i, c in enumerate(shape) if c == k]].sum() for k in kinds])
+    pk = np.array([shape.count(k) / 108 for k in kinds])
+    assert _chi2_p(hk, pk) > 1e-3
