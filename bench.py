@@ -119,7 +119,13 @@ def batched_leg(args) -> dict:
     import numpy as np
     from pscbfm_amd import Batch, updater_from_system, workloads
     t = time.perf_counter()
-    specs = workloads.mixed_batch(args.batch_systems, seed=0xBA7C4)
+    if args.batch_uniform:
+        count, box, chain_len, phi = args.batch_uniform.split(",")
+        specs = workloads.uniform_batch(int(count), int(box), int(chain_len), float(phi), seed=0xBA7C4)
+        composition = f"{int(count)} x (box {int(box)}, chain length {int(chain_len)}, phi {float(phi)})"
+    else:
+        specs = workloads.mixed_batch(args.batch_systems, seed=0xBA7C4)
+        composition = "mixed"
     built = [workloads.build(sp) for sp in specs]
     n_tot = sum(len(x["xyz"]) for x in built)
     log(f"batched: {len(built)} systems, {n_tot} monomers, built in {time.perf_counter() - t:.1f} s")
@@ -160,10 +166,10 @@ def batched_leg(args) -> dict:
     b.synchronize()
     t_batched = t_a + time.perf_counter() - t0
     rate = n_tot * args.batch_mcs / t_batched
-    log(f"batched: {args.batch_mcs} MCS in {t_batched:.2f} s -> {rate:.4g} attempts/s, "
-        f"{exact}/{len(built)} states bit-exact vs plain")
-    return {"systems": len(built), "monomers": n_tot, "mcs": args.batch_mcs, "chunk": args.chunk,
-            "graphs": not args.no_graphs,
+    check = f"{exact}/{len(built)} states bit-exact vs plain" if plain else "no plain comparison"
+    log(f"batched: {args.batch_mcs} MCS in {t_batched:.2f} s -> {rate:.4g} attempts/s, {check}")
+    return {"systems": len(built), "composition": composition, "monomers": n_tot, "mcs": args.batch_mcs,
+            "chunk": args.chunk, "graphs": not args.no_graphs,
             "value": rate, "unit": "attempts/s", "plain_value": plain_rate, "plain_mcs": args.plain_mcs,
             "speedup_vs_plain": rate / plain_rate if plain_rate else None, "bit_exact_systems": exact}
 
@@ -277,6 +283,9 @@ def main() -> int:
                          "monomers or more, 128 for smaller ones, whose kernels last a few microseconds and whose "
                          "launch pipeline a timed launch would stall)")
     ap.add_argument("--batch-systems", type=int, default=512)
+    ap.add_argument("--batch-uniform", metavar="COUNT,BOX,CHAIN_LEN,PHI", default=None,
+                    help="batch COUNT melts of one box edge, chain length and volume fraction "
+                         "instead of the mixed batch (for kernel traces of single compositions)")
     ap.add_argument("--batch-mcs", type=int, default=1000)
     ap.add_argument("--plain-mcs", type=int, default=200)
     ap.add_argument("--chunk", type=int, default=100)
@@ -426,7 +435,9 @@ def main() -> int:
     out["roofline"]["timed_every_substeps"] = every
     if not args.no_batched:
         out["batched"] = batched_leg(args)
-        if not args.no_served:
+        if args.batch_uniform:
+            log("served leg skipped: its jobs follow the mixed distribution, not --batch-uniform")
+        elif not args.no_served:
             out["served"] = served_leg(args, out["batched"]["value"])
     if not args.no_cpu:
         log("timing the CPU baseline")

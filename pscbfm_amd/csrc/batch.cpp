@@ -336,28 +336,36 @@ void BatchEngine::autotune(uint32_t probe_mcs, uint32_t chunk)
     hip_check(hipEventCreate(&e1), "hipEventCreate");
     const uint32_t group0 = mPerformGroup;
     const bool xcd0 = mXcdRanges;
-    float best = -1.0f;
     uint32_t best_group = group0;
     bool best_xcd = xcd0;
     try {
         // one untimed probe first, so that no configuration pays for cold caches
         run(probe_mcs, chunk);
-        for (const bool xcd : {true, false})
-            for (const uint32_t g : kPerformGroups) {
-                setXcdRanges(xcd);
-                setPerformGroup(g);
+        float ms[2][3] = {};
+        for (int x = 0; x < 2; ++x)
+            for (int k = 0; k < 3; ++k) {
+                setXcdRanges(x == 0);
+                setPerformGroup(kPerformGroups[k]);
                 hip_check(hipEventRecord(e0, mStream), "hipEventRecord");
                 run(probe_mcs, chunk);
                 hip_check(hipEventRecord(e1, mStream), "hipEventRecord");
                 hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-                float ms = 0.0f;
-                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-                if (best < 0.0f || ms < best) {
-                    best = ms;
-                    best_group = g;
-                    best_xcd = xcd;
-                }
+                hip_check(hipEventElapsedTime(&ms[x][k], e0, e1), "hipEventElapsedTime");
             }
+        // the configuration in force unless another is faster by more than 2 %
+        // (a choice made on run-to-run noise can be slower than the default)
+        int bx = xcd0 ? 0 : 1, bk = 0;
+        while (bk < 2 && kPerformGroups[bk] != group0)
+            ++bk;
+        const float incumbent = ms[bx][bk];
+        for (int x = 0; x < 2; ++x)
+            for (int k = 0; k < 3; ++k)
+                if (ms[x][k] < 0.98f * incumbent && ms[x][k] < ms[bx][bk]) {
+                    bx = x;
+                    bk = k;
+                }
+        best_group = kPerformGroups[bk];
+        best_xcd = bx == 0;
     } catch (...) {
         hipEventDestroy(e0);
         hipEventDestroy(e1);

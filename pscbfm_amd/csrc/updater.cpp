@@ -231,6 +231,39 @@ void UpdaterGPUScBFM_AB_Type::timedProbe(uint32_t n_mcs, std::vector<double> &ch
     perform_ms = mColourPerformMs;
 }
 
+double UpdaterGPUScBFM_AB_Type::wallProbe(uint32_t n_mcs)
+{
+    collectTimings();
+    hipEvent_t e0 = takeEvent(), e1 = takeEvent();
+    float ms = 0.0f;
+    hipError_t err = hipEventRecord(e0, mStream);
+    try {
+        if (err == hipSuccess) {
+            runResident(n_mcs, false);
+            err = hipEventRecord(e1, mStream);
+        }
+        if (err == hipSuccess)
+            err = hipEventSynchronize(e1);
+        if (err == hipSuccess)
+            err = hipEventElapsedTime(&ms, e0, e1);
+    } catch (...) {
+        mEventPool.push_back(e0);
+        mEventPool.push_back(e1);
+        throw;
+    }
+    mEventPool.push_back(e0);
+    mEventPool.push_back(e1);
+    hip_check(err, "probe timing");
+    return double(ms);
+}
+
+// A tuned choice replaces the default only when its probe is faster by more
+// than 2 %: short kernels time within a few per cent of each other from run
+// to run, and a choice made on noise can be slower than the default.
+static constexpr double kTuneMargin = 0.98;
+static_assert(kCheckBlockSizes[1] == kBlock && kPerformGroups[1] == kPerformGroup,
+              "candidate 1 is the default launch configuration");
+
 void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
 {
     if (!mInitialized)
@@ -246,26 +279,26 @@ void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
         mSegGroup = group0;
         setXcdRanges(xcd0);
     };
+    auto set_config = [&](const std::vector<uint32_t> &block, const std::vector<uint32_t> &group, bool xcd) {
+        mSegBlock = block;
+        mSegGroup = group;
+        setXcdRanges(xcd);
+    };
     try {
         std::vector<double> cms, pms;
         // one untimed probe first, so that no configuration pays for cold caches
         runResident(probe_mcs);
         // 1. the monomer order within colours (only where spatial sorting is on:
-        // without it the layout is id order throughout)
+        // without it the layout is id order throughout), by wall time
         if (mSys.spatial_order) {
-            timedProbe(probe_mcs, cms, pms);
-            double t_now = 0.0;
-            for (uint32_t c = 0; c < nc; ++c)
-                t_now += cms[c] + pms[c];
+            const double t_now = wallProbe(probe_mcs);
             resortLayout(!mMorton);
-            timedProbe(probe_mcs, cms, pms);
-            double t_other = 0.0;
-            for (uint32_t c = 0; c < nc; ++c)
-                t_other += cms[c] + pms[c];
-            if (t_other >= t_now)
+            const double t_other = wallProbe(probe_mcs);
+            if (!(t_other < kTuneMargin * t_now))
                 resortLayout(!mMorton);
         }
-        // 2. per segment: check workgroup size and perform group, per placement
+        // 2. per segment: check workgroup size and perform group, per placement,
+        // from each colour's own kernel durations
         constexpr int kCand = 3;
         double best_total[2] = {0.0, 0.0};
         std::vector<uint32_t> best_block[2], best_group[2];
@@ -280,11 +313,13 @@ void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
             best_block[x].assign(nc, kBlock);
             best_group[x].assign(nc, kPerformGroup);
             for (uint32_t c = 0; c < nc; ++c) {
-                int bc = 0, bp = 0;
-                for (int k = 1; k < kCand; ++k) {
-                    if (check_t[k][c] < check_t[bc][c])
+                // the default (candidate 1: 256 threads, group 4) unless another
+                // is faster by more than the margin
+                int bc = 1, bp = 1;
+                for (int k = 0; k < kCand; ++k) {
+                    if (check_t[k][c] < kTuneMargin * check_t[bc][c])
                         bc = k;
-                    if (perform_t[k][c] < perform_t[bp][c])
+                    if (perform_t[k][c] < kTuneMargin * perform_t[bp][c])
                         bp = k;
                 }
                 best_block[x][c] = kCheckBlockSizes[bc];
@@ -292,10 +327,29 @@ void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
                 best_total[x] += check_t[bc][c] + perform_t[bp][c];
             }
         }
-        const int x = best_total[1] < best_total[0] ? 1 : 0;
-        setXcdRanges(x == 0);
-        mSegBlock = best_block[x];
-        mSegGroup = best_group[x];
+        const int x = best_total[1] < kTuneMargin * best_total[0] ? 1 : 0;
+        // 3. the pick against the defaults by wall time, which also sees what
+        // per-kernel durations do not (the boundaries between dependent
+        // kernels): two alternating probes each, the faster of each pair
+        const std::vector<uint32_t> def_block(nc, kBlock), def_group(nc, kPerformGroup);
+        const bool tuned_is_default = best_block[x] == def_block && best_group[x] == def_group && x == 0;
+        bool keep_tuned = false;
+        if (!tuned_is_default) {
+            double t_def = 0.0, t_tuned = 0.0;
+            for (int rep = 0; rep < 2; ++rep) {
+                set_config(def_block, def_group, true);
+                const double a = wallProbe(probe_mcs);
+                set_config(best_block[x], best_group[x], x == 0);
+                const double b = wallProbe(probe_mcs);
+                t_def = rep == 0 ? a : std::min(t_def, a);
+                t_tuned = rep == 0 ? b : std::min(t_tuned, b);
+            }
+            keep_tuned = t_tuned < kTuneMargin * t_def;
+        }
+        if (keep_tuned)
+            set_config(best_block[x], best_group[x], x == 0);
+        else
+            set_config(def_block, def_group, true);
     } catch (...) {
         restore();
         if (mMorton != morton0) {

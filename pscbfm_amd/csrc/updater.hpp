@@ -134,6 +134,7 @@ private:
     void runResident(uint32_t n_mcs, bool timing);
     // every substep of n_mcs MCS timed, per colour
     void timedProbe(uint32_t n_mcs, std::vector<double> &check_ms, std::vector<double> &perform_ms);
+    double wallProbe(uint32_t n_mcs);   // ms of n_mcs MCS on the stream, no per-kernel timing
     void resortLayout(bool morton);
     void freeDevice();
     void collectTimings() const;

@@ -6,6 +6,10 @@ floor(phi * L^3 / (8 N)), each with its own seed.
 
 job_stream: Poisson arrivals of jobs whose systems follow the same
 distribution, each asking for 100..2000 MCS in steps of 100.
+
+uniform_batch: n melts of one box edge, chain length and volume fraction
+(chains as above), each with its own seed: the compositions that separate
+member size from density in the batched sweep's cost.
 """
 from __future__ import annotations
 
@@ -17,13 +21,22 @@ BOXES = (32, 64, 128)
 CHAIN_LENGTHS = (16, 32, 64, 128)
 
 
+def _spec(L: int, N: int, phi: float, seed: int) -> dict:
+    chains = int(phi * L ** 3 / (8 * N))
+    chains = max(1, min(chains, L ** 3 // (16 * N)))
+    return {"box": L, "chain_len": N, "phi": phi, "chains": chains, "seed": seed}
+
+
 def mixed_system_spec(rng: np.random.Generator) -> dict:
     L = int(rng.choice(BOXES))
     N = int(rng.choice(CHAIN_LENGTHS))
     phi = float(rng.uniform(0.3, 0.5))
-    chains = int(phi * L ** 3 / (8 * N))
-    chains = max(1, min(chains, L ** 3 // (16 * N)))
-    return {"box": L, "chain_len": N, "phi": phi, "chains": chains, "seed": int(rng.integers(1, 2 ** 63))}
+    return _spec(L, N, phi, int(rng.integers(1, 2 ** 63)))
+
+
+def uniform_batch(n_systems: int, box: int, chain_len: int, phi: float, seed: int) -> list[dict]:
+    rng = np.random.default_rng(seed)
+    return [_spec(box, chain_len, phi, int(rng.integers(1, 2 ** 63))) for _ in range(n_systems)]
 
 
 def build(spec: dict) -> dict:
