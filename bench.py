@@ -265,7 +265,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1000, help="timed Monte-Carlo steps")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--autotune", type=int, default=20,
-                    help="MCS per autotuner probe after the relaxation (0: the default launch configuration)")
+                    help="least MCS per autotuner probe after the relaxation, more where a probe would last "
+                         "under 4 ms (0: the default launch configuration)")
     ap.add_argument("--equilibrate", type=int, default=1000,
                     help="untimed MCS before the warm-up: the generated start (chains on the even sublattice, "
                          "every bond |b|^2 = 4) relaxes toward the melt's bond-vector distribution")
@@ -342,11 +343,17 @@ def main() -> int:
     t = time.perf_counter()
     u.runResident(args.equilibrate)
     u.getStats()
-    log(f"relaxed for {args.equilibrate} MCS in {time.perf_counter() - t:.2f} s")
+    t_relax = time.perf_counter() - t
+    log(f"relaxed for {args.equilibrate} MCS in {t_relax:.2f} s")
     # launch configuration chosen by the autotuner (untimed; the system goes on
-    # relaxing through its probes)
+    # relaxing through its probes).  Probes of at least --autotune MCS, and
+    # long enough to last about 4 ms, so that run-to-run noise of short
+    # kernels does not decide
+    probe_mcs = args.autotune
+    if args.autotune > 0 and args.equilibrate > 0:
+        probe_mcs = max(args.autotune, -int(-0.004 // (t_relax / args.equilibrate)))
     if args.autotune > 0:
-        tuned = u.autotune(args.autotune, n_colors)
+        tuned = u.autotune(probe_mcs, n_colors)
     else:
         tuned = {"perform_group": 4, "xcd_ranges": True, "check_block": [256] * n_colors,
                  "perform_groups": [4] * n_colors, "morton_order": args.layout == "morton"}
@@ -416,7 +423,7 @@ def main() -> int:
                    "launch": {"check_workgroup": tuned["check_block"], "perform_group": tuned["perform_groups"],
                               "xcd_ranges": tuned["xcd_ranges"],
                               "order_within_colour": "morton" if tuned["morton_order"] else "id",
-                              "autotune_probe_mcs": args.autotune}},
+                              "autotune_probe_mcs": probe_mcs}},
         "roofline": roofline,
     }
     # the drop-in call as a LeMonADE host makes it: MCS plus the copy-back of

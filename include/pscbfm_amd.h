@@ -212,7 +212,7 @@ int bfm_set_xcd_ranges(bfm_updater *h, int on);
  * configuration in force before the call stays.  After initialize. */
 int bfm_set_perform_group(bfm_updater *h, uint32_t group);
 int bfm_autotune(bfm_updater *h, uint32_t probe_mcs, uint32_t *group, int *xcd_ranges);
-/* Check workgroup size (128, 256 = default, or 512 threads) for every colour
+/* Check workgroup size (64, 128, 256 = default, 512 or 1024 threads) for every colour
  * segment; the reference tunes threads per block per species
  * (UpdaterGPUScBFM_AB_Type.cu:2940-2990).  Any time (after initialize it
  * applies to every segment). */
@@ -223,8 +223,11 @@ int bfm_set_check_block(bfm_updater *h, uint32_t threads);
  * position (1) or in id order (0).  bfm_autotune chooses all of them: the
  * order within colours by timing both (re-sorting on the device), then per
  * segment the fastest check workgroup size and perform group from the
- * segment's own kernel durations, under the faster placement; the system
- * advances by 9 * probe_mcs MCS (7 without spatial sorting). */
+ * segment's own kernel durations, under the faster placement, and keeps that
+ * pick only if it also beats the defaults by wall time; each choice needs a
+ * 2 % lead over the default.  The system advances by 15 * probe_mcs MCS when
+ * the pick is the default, else 19 * probe_mcs (four fewer without spatial
+ * sorting). */
 int bfm_set_segment_config(bfm_updater *h, uint32_t colour, uint32_t threads, uint32_t group);
 int bfm_get_launch_config(const bfm_updater *h, uint32_t colour, uint32_t *threads, uint32_t *group, int *xcd_ranges,
                           int *morton_order);

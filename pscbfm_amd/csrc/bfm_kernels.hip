@@ -548,9 +548,11 @@ hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count,
                         uint32_t parity, hipStream_t stream, hipEvent_t start, hipEvent_t stop, uint32_t block)
 {
     switch (block) {
+    case 64: launch_check_threads<64>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
     case 128: launch_check_threads<128>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
-    case 512: launch_check_threads<512>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
     case 256: launch_check_threads<256>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
+    case 512: launch_check_threads<512>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
+    case 1024: launch_check_threads<1024>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

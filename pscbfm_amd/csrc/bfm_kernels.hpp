@@ -77,9 +77,9 @@ __host__ __device__ inline uint32_t perform_blocks(uint32_t cb, uint32_t group =
 // Workgroup sizes of the plain check kernel with a compiled variant (a launch
 // parameter the autotuner chooses among per colour segment); kBlock is the
 // default, and surv / scount are sized for the largest.
-constexpr uint32_t kCheckBlockSizes[] = {128, 256, 512};
-constexpr uint32_t kMaxCheckBlock = 512;
-inline bool valid_check_block(uint32_t b) { return b == 128 || b == 256 || b == 512; }
+constexpr uint32_t kCheckBlockSizes[] = {64, 128, 256, 512, 1024};
+constexpr uint32_t kMaxCheckBlock = 1024;
+inline bool valid_check_block(uint32_t b) { return b == 64 || b == 128 || b == 256 || b == 512 || b == 1024; }
 // Perform group sizes with a compiled kernel (a launch parameter the
 // autotuners choose among); kPerformGroup is the default.
 constexpr uint32_t kPerformGroups[] = {2, 4, 8};

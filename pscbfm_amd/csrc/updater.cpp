@@ -261,8 +261,16 @@ double UpdaterGPUScBFM_AB_Type::wallProbe(uint32_t n_mcs)
 // than 2 %: short kernels time within a few per cent of each other from run
 // to run, and a choice made on noise can be slower than the default.
 static constexpr double kTuneMargin = 0.98;
-static_assert(kCheckBlockSizes[1] == kBlock && kPerformGroups[1] == kPerformGroup,
-              "candidate 1 is the default launch configuration");
+// The probes of the per-segment search: every check workgroup size once, each
+// paired with a perform group so that every group is timed; probe kDefaultProbe
+// is the default launch configuration.
+constexpr int kProbes = 5;
+constexpr uint32_t kProbeBlock[kProbes] = {64, 128, 256, 512, 1024};
+constexpr uint32_t kProbeGroup[kProbes] = {4, 2, 4, 8, 4};
+constexpr int kDefaultProbe = 2;
+static_assert(kProbeBlock[kDefaultProbe] == kBlock && kProbeGroup[kDefaultProbe] == kPerformGroup,
+              "the default probe is the default launch configuration");
+static_assert(kPerformGroups[1] == kPerformGroup, "group 1 is the default perform group");
 
 void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
 {
@@ -289,42 +297,59 @@ void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
         // one untimed probe first, so that no configuration pays for cold caches
         runResident(probe_mcs);
         // 1. the monomer order within colours (only where spatial sorting is on:
-        // without it the layout is id order throughout), by wall time
+        // without it the layout is id order throughout), by wall time: two
+        // alternating probes per order, the faster of each pair
         if (mSys.spatial_order) {
-            const double t_now = wallProbe(probe_mcs);
-            resortLayout(!mMorton);
-            const double t_other = wallProbe(probe_mcs);
-            if (!(t_other < kTuneMargin * t_now))
+            double t_now = 0.0, t_other = 0.0;
+            for (int rep = 0; rep < 2; ++rep) {
+                const double a = wallProbe(probe_mcs);
+                resortLayout(!mMorton);
+                const double b = wallProbe(probe_mcs);
+                resortLayout(!mMorton);
+                t_now = rep == 0 ? a : std::min(t_now, a);
+                t_other = rep == 0 ? b : std::min(t_other, b);
+            }
+            if (t_other < kTuneMargin * t_now)
                 resortLayout(!mMorton);
         }
         // 2. per segment: check workgroup size and perform group, per placement,
-        // from each colour's own kernel durations
-        constexpr int kCand = 3;
+        // from each colour's own kernel durations (the perform kernel's work
+        // does not depend on the check workgroup size: wave w of a check launch
+        // covers monomers [64 w, 64 w + 64) whatever the size)
         double best_total[2] = {0.0, 0.0};
         std::vector<uint32_t> best_block[2], best_group[2];
         for (int x = 0; x < 2; ++x) {
             setXcdRanges(x == 0);
-            std::vector<double> check_t[kCand], perform_t[kCand];
-            for (int k = 0; k < kCand; ++k) {
-                std::fill(mSegBlock.begin(), mSegBlock.end(), kCheckBlockSizes[k]);
-                std::fill(mSegGroup.begin(), mSegGroup.end(), kPerformGroups[k]);
+            std::vector<double> check_t[kProbes], perform_t[kProbes];
+            for (int k = 0; k < kProbes; ++k) {
+                std::fill(mSegBlock.begin(), mSegBlock.end(), kProbeBlock[k]);
+                std::fill(mSegGroup.begin(), mSegGroup.end(), kProbeGroup[k]);
                 timedProbe(probe_mcs, check_t[k], perform_t[k]);
             }
             best_block[x].assign(nc, kBlock);
             best_group[x].assign(nc, kPerformGroup);
             for (uint32_t c = 0; c < nc; ++c) {
-                // the default (candidate 1: 256 threads, group 4) unless another
-                // is faster by more than the margin
-                int bc = 1, bp = 1;
-                for (int k = 0; k < kCand; ++k) {
+                // the default unless another is faster by more than the margin;
+                // a group timed in several probes counts with its best time
+                int bc = kDefaultProbe;
+                for (int k = 0; k < kProbes; ++k)
                     if (check_t[k][c] < kTuneMargin * check_t[bc][c])
                         bc = k;
-                    if (perform_t[k][c] < kTuneMargin * perform_t[bp][c])
-                        bp = k;
+                double group_t[3] = {-1.0, -1.0, -1.0};
+                for (int k = 0; k < kProbes; ++k) {
+                    int gi = 0;
+                    while (gi < 2 && kPerformGroups[gi] != kProbeGroup[k])
+                        ++gi;
+                    if (group_t[gi] < 0.0 || perform_t[k][c] < group_t[gi])
+                        group_t[gi] = perform_t[k][c];
                 }
-                best_block[x][c] = kCheckBlockSizes[bc];
+                int bp = 1;   // kPerformGroups[1] is the default group
+                for (int gi = 0; gi < 3; ++gi)
+                    if (group_t[gi] >= 0.0 && group_t[gi] < kTuneMargin * group_t[bp])
+                        bp = gi;
+                best_block[x][c] = kProbeBlock[bc];
                 best_group[x][c] = kPerformGroups[bp];
-                best_total[x] += check_t[bc][c] + perform_t[bp][c];
+                best_total[x] += check_t[bc][c] + group_t[bp];
             }
         }
         const int x = best_total[1] < kTuneMargin * best_total[0] ? 1 : 0;

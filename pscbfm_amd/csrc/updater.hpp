@@ -87,7 +87,7 @@ public:
     // check waves per perform wave (2, 4 or 8), for every colour segment
     void setPerformGroup(uint32_t group);
     uint32_t performGroup() const { return mPerformGroup; }
-    // check workgroup size (128, 256 or 512 threads), for every colour segment
+    // check workgroup size (64, 128, 256, 512 or 1024 threads), for every colour segment
     void setCheckBlock(uint32_t block);
     // the launch configuration of colour segment c (after initialize)
     void setSegmentConfig(uint32_t colour, uint32_t check_block, uint32_t group);
@@ -97,14 +97,18 @@ public:
     // Chooses the launch configuration by timing short probes (the run goes
     // on meanwhile; the trajectory is the same under every configuration):
     //  1. the monomer order within colours, Morton versus id (with spatial
-    //     sorting on: the monomers are re-sorted on the device for the id
-    //     probe, and back if Morton wins);
-    //  2. per colour segment, the check workgroup size (128, 256, 512) and
+    //     sorting on: the monomers are re-sorted on the device between two
+    //     alternating wall-time probes of each order);
+    //  2. per colour segment, the check workgroup size (64 ... 1024) and
     //     the perform group (2, 4, 8), from each segment's own kernel
     //     durations (the check's do not depend on the group, the perform's
     //     not on the workgroup size), under each XCD placement; the placement
-    //     with the smaller total wins.
-    // One untimed probe first; the age advances by 9 * probe_mcs (7 without
+    //     with the smaller total wins;
+    //  3. that pick against the defaults by wall time (two probes each),
+    //     kept only if faster there too.
+    // Every step keeps the default unless another choice is faster by more
+    // than 2 %.  One untimed probe first; the age advances by 15 * probe_mcs
+    // when the pick is the default, else 19 * probe_mcs (four fewer without
     // spatial sorting).
     void autotune(uint32_t probe_mcs);
     const HostSystem &system() const { return mSys; }

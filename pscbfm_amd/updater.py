@@ -173,9 +173,11 @@ class UpdaterGPUScBFM_AB_Type:
     def autotune(self, probe_mcs: int, n_colors: int | None = None) -> dict:
         """Chooses the order within colours (Morton or id), then per colour
         segment the check workgroup size and perform group, and the XCD
-        placement, from short timed probes (the system advances by
-        9 * probe_mcs MCS, 7 without spatial sorting).  Returns the chosen
-        configuration (per colour with n_colors)."""
+        placement, from short timed probes, keeping the pick only if it also
+        beats the defaults by wall time (the system advances by 15 * probe_mcs
+        MCS when the pick is the default, else 19 * probe_mcs; four fewer
+        without spatial sorting).  Returns the chosen configuration (per
+        colour with n_colors)."""
         g, x = c_uint32(), ctypes.c_int()
         _capi.check(self._lib.bfm_autotune(self._h, int(probe_mcs), byref(g), byref(x)))
         out = {"perform_group": g.value, "xcd_ranges": bool(x.value)}
@@ -187,7 +189,7 @@ class UpdaterGPUScBFM_AB_Type:
         return out
 
     def setCheckBlock(self, threads: int) -> None:
-        """Check workgroup size (128, 256 or 512 threads) for every colour segment."""
+        """Check workgroup size (64, 128, 256, 512 or 1024 threads) for every colour segment."""
         _capi.check(self._lib.bfm_set_check_block(self._h, int(threads)))
 
     def setSegmentConfig(self, colour: int, threads: int, group: int) -> None:

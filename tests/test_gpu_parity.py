@@ -196,14 +196,14 @@ def test_autotune_keeps_the_trajectory(gpu_ready):
     seed = 0x5EED0000000000F3
     print(f"case: autotune n={n}", flush=True)
     u = updater_from_system(s, seed=seed, gpu=0)
-    # one untimed probe, 2 layout probes, 6 configurations, and 4 more when the
-    # pick is not the default (compared by wall time): 5 MCS each
+    # one untimed probe, 4 layout probes, 10 configurations, and 4 more when
+    # the pick is not the default (compared by wall time): 5 MCS each
     cfg = u.autotune(5, n_colors=2)
     assert cfg["perform_group"] in (2, 4, 8)
-    assert all(b in (128, 256, 512) for b in cfg["check_block"])
+    assert all(b in (64, 128, 256, 512, 1024) for b in cfg["check_block"])
     assert all(g in (2, 4, 8) for g in cfg["perform_groups"])
     age = u.getAge()
-    assert age in (45, 65)
+    assert age in (75, 95)
     u.runSimulationOnGPU(100 - age)
     got = u.getMonomerPositions(n)
     colors, _ = u.getColors(n)
@@ -216,8 +216,8 @@ def test_autotune_keeps_the_trajectory(gpu_ready):
 
 @pytest.mark.parametrize("system", ["melt", "network", "walls"])
 def test_every_launch_candidate_keeps_the_trajectory(gpu_ready, system):
-    """Every candidate the autotuner chooses among -- check workgroup size 128,
-    256 or 512, perform group 2, 4 or 8, XCD placement on or off, Morton or id
+    """Every candidate the autotuner chooses among -- check workgroup size 64,
+    128, 256, 512 or 1024, perform group 2, 4 or 8, XCD placement on or off, Morton or id
     order within colours -- and different configurations per colour segment
     give the oracle's trajectory bit-exactly."""
     if system == "melt":
@@ -230,7 +230,8 @@ def test_every_launch_candidate_keeps_the_trajectory(gpu_ready, system):
     seed = 0x5EED0000000000E9
     mcs = 12
     want, acc, prop = orc.sweep(orc.System(s, seed), s["xyz"], 0, mcs, 0)
-    configs = [(b, g, x, m) for b in (128, 256, 512) for g in (2, 4, 8) for x in (True, False) for m in (True, False)]
+    configs = [(b, g, x, m) for b in (64, 128, 256, 512, 1024) for g in (2, 4, 8) for x in (True, False)
+               for m in (True, False)]
     print(f"case: {system}, n={n}, {len(configs)} launch configurations + mixed per-segment ones", flush=True)
     for b, g, x, m in configs:
         u = updater_from_system(s, seed=seed, gpu=0, spatial_sorting=m)
@@ -245,9 +246,11 @@ def test_every_launch_candidate_keeps_the_trajectory(gpu_ready, system):
         assert (st["accepted"], st["proposals"]) == (acc, prop), (b, g, x, m)
     u = updater_from_system(s, seed=seed, gpu=0)
     _, nc = u.getColors(n)
+    blocks, groups = (1024, 64, 512, 128), (8, 2, 4, 2)
     for c in range(nc):
-        u.setSegmentConfig(c, (128, 512, 256, 128)[c % 4], (8, 2, 4, 2)[c % 4])
-    assert [u.launchConfig(c)["check_block"] for c in range(nc)] == [(128, 512, 256, 128)[c % 4] for c in range(nc)]
+        u.setSegmentConfig(c, blocks[c % 4], groups[c % 4])
+    assert [u.launchConfig(c)["check_block"] for c in range(nc)] == [blocks[c % 4] for c in range(nc)]
+    assert [u.launchConfig(c)["perform_group"] for c in range(nc)] == [groups[c % 4] for c in range(nc)]
     u.runSimulationOnGPU(mcs)
     assert np.array_equal(u.getMonomerPositions(n), want)
     u.cleanup()

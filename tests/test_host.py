@@ -293,3 +293,14 @@ def test_perform_group_must_be_compiled():
     for g in (0, 1, 3, 16):
         with pytest.raises(_capi.InvalidArgument):
             u.setPerformGroup(g)
+
+
+def test_check_block_must_be_compiled():
+    # the check kernel is built for 64 ... 1024 threads per workgroup
+    from pscbfm_amd import _capi, systems, updater_from_system
+    u = updater_from_system(systems.linear_melt(16, 8, 8, seed=1), seed=1, gpu=None, validate_only=True)
+    for b in (64, 128, 256, 512, 1024):
+        u.setCheckBlock(b)
+    for b in (0, 32, 96, 192, 2048):
+        with pytest.raises(_capi.InvalidArgument):
+            u.setCheckBlock(b)
