@@ -355,8 +355,10 @@ def main() -> int:
     if args.autotune > 0:
         tuned = u.autotune(probe_mcs, n_colors)
     else:
-        tuned = {"perform_group": 4, "xcd_ranges": True, "check_block": [256] * n_colors,
-                 "perform_groups": [4] * n_colors, "morton_order": args.layout == "morton"}
+        cfg = [u.launchConfig(c) for c in range(n_colors)]
+        tuned = {"perform_group": cfg[0]["perform_group"], "xcd_ranges": cfg[0]["xcd_ranges"],
+                 "check_block": [c["check_block"] for c in cfg], "perform_groups": [c["perform_group"] for c in cfg],
+                 "morton_order": cfg[0]["morton_order"]}
     log(f"launch configuration: {tuned}")
     u.setKernelTiming(every)
     u.runResident(args.warmup)

@@ -313,9 +313,9 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
         ends[k] = wave_uniform(__shfl(incl, int(k)));
     const uint32_t total = ends[kGroup - 1];
 
-    uint32_t acc = 0;
-    for (uint32_t t = lane; t < total; t += kWave) {
-        // check wave k of the group holds survivors [ends[k-1], ends[k])
+    // slot in surv of the group's survivor t: check wave k of the group holds
+    // survivors [ends[k-1], ends[k])
+    auto slot = [&](uint32_t t) {
         uint32_t k = 0, base = 0;
 #pragma unroll
         for (uint32_t q = 0; q + 1 < kGroup; ++q)
@@ -323,47 +323,88 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
                 k = q + 1;
                 base = ends[q];
             }
-        const uint2 e = element(surv + size_t(cw0 + k) * kWave + (t - base));
+        return size_t(cw0 + k) * kWave + (t - base);
+    };
+    // another survivor's moved corner in the front: the moved cubes overlap
+    // and both stay (the reference's second front check on the tentative
+    // lattice, :1148-1190)
+    auto clashes = [&](const Front &f, const uint32_t (&k4)[4]) {
+        return ((k4[0] & f.m[0]) | (k4[1] & f.m[1]) | (k4[2] & f.m[2]) | (k4[3] & f.m[3])) != 0u;
+    };
+    auto apply = [&](const uint2 &e, const Front &f) {
+        // Vacate the old corner, occupy the new one.  Byte j of the lattice
+        // is the aligned 2x2x2 block of sites j*8 .. j*8+7 (the lowest Morton
+        // bits are x0, y0, z0 in every geometry), and two corners in one
+        // block would be overlapping cubes: a valid state has at most one
+        // corner per byte.  An accepted move's old block holds only its own
+        // corner, its new block holds none, and no other accepted move of the
+        // substep touches either (the front tests reject every pair whose
+        // cubes would overlap).  So each move owns its two bytes and writes
+        // their new values with plain byte stores: no atomics, no reads.
         const uint32_t i = e.x & kSortedIndexMask;
         const int dir = int(e.x >> kSortedIndexBits);
         const int4 r0 = unpack_xyz(e.y);
-        // another survivor's moved corner in the front: the moved cubes
-        // overlap and both stay (the reference's second front check on the
-        // tentative lattice, :1148-1190)
-        const Front f = front_sites(g, dep, r0, dir);
-        uint32_t k4[4];
+        const gptr<uint8_t> blocks = (gptr<uint8_t>)lattice;
+        const uint32_t nb = f.new_site >> 3;
+        if ((f.old_site >> 3) != nb)
+            blocks[f.old_site >> 3] = uint8_t(0);
+        blocks[nb] = uint8_t(1u << (f.new_site & 7u));
+        // new wrapped coordinate on the move axis; crossing the box boundary
+        // changes the periodic image (only on periodic axes: walls keep the
+        // others inside)
+        const int a = dir >> 1;
+        const uint32_t am = sel3(g.mask, a);
+        const uint32_t v = uint32_t(comp(r0, a) + ((dir & 1) ? 1 : -1)) & am;
+        const uint32_t shift = kCoordBits * uint32_t(a);
+        ppos[i] = (e.y & ~(kCoordMask << shift)) | (v << shift);
+        if ((dir & 1) ? v == 0u : v == am)
+            as_global(buf.image)[size_t(a) * buf.n_pad + i] += (dir & 1) ? 1 : -1;
+    };
+    auto front_of = [&](const uint2 &e) {
+        return front_sites(g, dep, unpack_xyz(e.y), int(e.x >> kSortedIndexBits));
+    };
+
+    // The survivors in rounds of 64 lanes.  The first kRounds rounds (enough
+    // for a group's usual count) issue all their loads before any store: a
+    // store may alias a later round's loads, so in a plain loop every round
+    // would wait for the one before.
+    constexpr uint32_t kRounds = kGroup >= 8 ? kGroup / 4 : 1;
+    uint32_t acc = 0;
+    uint2 e[kRounds];
+    bool live[kRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kRounds; ++r) {
+        live[r] = lane + r * kWave < total;
+        e[r] = live[r] ? element(surv + slot(lane + r * kWave)) : make_uint2(0u, 0u);
+    }
+    Front f[kRounds];
+    uint32_t k4[kRounds][4];
+#pragma unroll
+    for (uint32_t r = 0; r < kRounds; ++r) {
+        f[r] = front_of(e[r]);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            k4[q] = f.m[q] ? claim[f.w[q]] : 0u;
-        const bool clash = ((k4[0] & f.m[0]) | (k4[1] & f.m[1]) | (k4[2] & f.m[2]) | (k4[3] & f.m[3])) != 0u;
-        if (!clash) {
-            // Vacate the old corner, occupy the new one.  Byte j of the
-            // lattice is the aligned 2x2x2 block of sites j*8 .. j*8+7 (the
-            // lowest Morton bits are x0, y0, z0 in every geometry), and two
-            // corners in one block would be overlapping cubes: a valid state
-            // has at most one corner per byte.  An accepted move's old block
-            // holds only its own corner, its new block holds none, and no
-            // other accepted move of the substep touches either (the front
-            // tests reject every pair whose cubes would overlap).  So each
-            // move owns its two bytes and writes their new values with plain
-            // byte stores: no atomics, no reads.
-            const gptr<uint8_t> blocks = (gptr<uint8_t>)lattice;
-            const uint32_t nb = f.new_site >> 3;
-            if ((f.old_site >> 3) != nb)
-                blocks[f.old_site >> 3] = uint8_t(0);
-            blocks[nb] = uint8_t(1u << (f.new_site & 7u));
-            // new wrapped coordinate on the move axis; crossing the box
-            // boundary changes the periodic image (only on periodic axes:
-            // walls keep the others inside)
-            const int a = dir >> 1;
-            const uint32_t am = sel3(g.mask, a);
-            const uint32_t v = uint32_t(comp(r0, a) + ((dir & 1) ? 1 : -1)) & am;
-            const uint32_t shift = kCoordBits * uint32_t(a);
-            ppos[i] = (e.y & ~(kCoordMask << shift)) | (v << shift);
-            if ((dir & 1) ? v == 0u : v == am)
-                as_global(buf.image)[size_t(a) * buf.n_pad + i] += (dir & 1) ? 1 : -1;
-        }
-        acc += uint32_t(__popcll(__ballot(!clash)));
+            k4[r][q] = live[r] && f[r].m[q] ? claim[f[r].w[q]] : 0u;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kRounds; ++r) {
+        const bool go = live[r] && !clashes(f[r], k4[r]);
+        if (go)
+            apply(e[r], f[r]);
+        acc += uint32_t(__popcll(__ballot(go)));
+    }
+    // the rest, one round at a time (more survivors than kRounds rounds hold)
+    for (uint32_t t = lane + kRounds * kWave; t < total; t += kWave) {
+        const uint2 e1 = element(surv + slot(t));
+        const Front f1 = front_of(e1);
+        uint32_t c4[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            c4[q] = f1.m[q] ? claim[f1.w[q]] : 0u;
+        const bool go = !clashes(f1, c4);
+        if (go)
+            apply(e1, f1);
+        acc += uint32_t(__popcll(__ballot(go)));
     }
     if (lane == 0 && total) {
         atomicAdd((unsigned long long *)&as_global(buf.accepted)[pwave & (kStatSlots - 1)], (unsigned long long)acc);

@@ -169,13 +169,19 @@ void UpdaterGPUScBFM_AB_Type::setPerformGroup(uint32_t group)
     if (!valid_perform_group(group))
         throw std::invalid_argument("perform group must be 2, 4 or 8");
     mPerformGroup = group;
+    mPerformGroupSet = true;
     std::fill(mSegGroup.begin(), mSegGroup.end(), group);
+}
+
+uint32_t UpdaterGPUScBFM_AB_Type::defaultGroup(uint32_t colour) const
+{
+    return default_perform_group(check_waves(mSys.seg_count[colour], kBlock), mResidentWaves);
 }
 
 void UpdaterGPUScBFM_AB_Type::setCheckBlock(uint32_t block)
 {
     if (!valid_check_block(block))
-        throw std::invalid_argument("check workgroup size must be 128, 256 or 512");
+        throw std::invalid_argument("check workgroup size must be 64, 128, 256, 512 or 1024");
     mCheckBlock = block;
     std::fill(mSegBlock.begin(), mSegBlock.end(), block);
 }
@@ -187,7 +193,7 @@ void UpdaterGPUScBFM_AB_Type::setSegmentConfig(uint32_t colour, uint32_t check_b
     if (colour >= mSegBlock.size())
         throw std::invalid_argument("no such colour");
     if (!valid_check_block(check_block))
-        throw std::invalid_argument("check workgroup size must be 128, 256 or 512");
+        throw std::invalid_argument("check workgroup size must be 64, 128, 256, 512 or 1024");
     if (!valid_perform_group(group))
         throw std::invalid_argument("perform group must be 2, 4 or 8");
     mSegBlock[colour] = check_block;
@@ -263,14 +269,13 @@ double UpdaterGPUScBFM_AB_Type::wallProbe(uint32_t n_mcs)
 static constexpr double kTuneMargin = 0.98;
 // The probes of the per-segment search: every check workgroup size once, each
 // paired with a perform group so that every group is timed; probe kDefaultProbe
-// is the default launch configuration.
+// has the default workgroup size (each segment's default group is
+// default_perform_group of its size).
 constexpr int kProbes = 5;
 constexpr uint32_t kProbeBlock[kProbes] = {64, 128, 256, 512, 1024};
 constexpr uint32_t kProbeGroup[kProbes] = {4, 2, 4, 8, 4};
 constexpr int kDefaultProbe = 2;
-static_assert(kProbeBlock[kDefaultProbe] == kBlock && kProbeGroup[kDefaultProbe] == kPerformGroup,
-              "the default probe is the default launch configuration");
-static_assert(kPerformGroups[1] == kPerformGroup, "group 1 is the default perform group");
+static_assert(kProbeBlock[kDefaultProbe] == kBlock, "the default probe has the default workgroup size");
 
 void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
 {
@@ -327,7 +332,7 @@ void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
                 timedProbe(probe_mcs, check_t[k], perform_t[k]);
             }
             best_block[x].assign(nc, kBlock);
-            best_group[x].assign(nc, kPerformGroup);
+            best_group[x].resize(nc);
             for (uint32_t c = 0; c < nc; ++c) {
                 // the default unless another is faster by more than the margin;
                 // a group timed in several probes counts with its best time
@@ -343,7 +348,9 @@ void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
                     if (group_t[gi] < 0.0 || perform_t[k][c] < group_t[gi])
                         group_t[gi] = perform_t[k][c];
                 }
-                int bp = 1;   // kPerformGroups[1] is the default group
+                int bp = 0;   // the segment's default group
+                while (bp < 2 && kPerformGroups[bp] != defaultGroup(c))
+                    ++bp;
                 for (int gi = 0; gi < 3; ++gi)
                     if (group_t[gi] >= 0.0 && group_t[gi] < kTuneMargin * group_t[bp])
                         bp = gi;
@@ -356,7 +363,10 @@ void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
         // 3. the pick against the defaults by wall time, which also sees what
         // per-kernel durations do not (the boundaries between dependent
         // kernels): two alternating probes each, the faster of each pair
-        const std::vector<uint32_t> def_block(nc, kBlock), def_group(nc, kPerformGroup);
+        const std::vector<uint32_t> def_block(nc, kBlock);
+        std::vector<uint32_t> def_group(nc);
+        for (uint32_t c = 0; c < nc; ++c)
+            def_group[c] = defaultGroup(c);
         const bool tuned_is_default = best_block[x] == def_block && best_group[x] == def_group && x == 0;
         bool keep_tuned = false;
         if (!tuned_is_default) {
@@ -456,8 +466,15 @@ void UpdaterGPUScBFM_AB_Type::initialize()
     mLastSortAge = mAge;   // plan() laid the colours out in spatial order
     mSorts = 0;
     mMorton = s.spatial_order;
+    // resident wavefronts of the device at the sweep kernels' occupancy (8
+    // per SIMD, 4 SIMDs per CU): the default perform group depends on it
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, mGpu) == hipSuccess && cus > 0)
+        mResidentWaves = uint32_t(cus) * 4u * 8u;
     mSegBlock.assign(s.n_colours, mCheckBlock);
-    mSegGroup.assign(s.n_colours, mPerformGroup);
+    mSegGroup.resize(s.n_colours);
+    for (uint32_t c = 0; c < s.n_colours; ++c)
+        mSegGroup[c] = mPerformGroupSet ? mPerformGroup : defaultGroup(c);
     mColourCheckMs.assign(s.n_colours, 0.0);
     mColourPerformMs.assign(s.n_colours, 0.0);
     mInitialized = true;

@@ -84,8 +84,10 @@ public:
     void setKernelTiming(uint32_t every) { mTimingEvery = every; }
     void setXcdRanges(bool on) { mBuf.xcd_ranges = on ? 1u : 0u; mXcdRanges = on; }
     bool xcdRanges() const { return mXcdRanges; }
-    // check waves per perform wave (2, 4 or 8), for every colour segment
+    // check waves per perform wave (2, 4 or 8), for every colour segment;
+    // without it each segment starts with default_perform_group of its size
     void setPerformGroup(uint32_t group);
+    uint32_t defaultGroup(uint32_t colour) const;
     uint32_t performGroup() const { return mPerformGroup; }
     // check workgroup size (64, 128, 256, 512 or 1024 threads), for every colour segment
     void setCheckBlock(uint32_t block);
@@ -168,6 +170,8 @@ private:
     uint32_t mTimingEvery = 0;   // time every n-th substep (0: off)
     bool mXcdRanges = true;
     uint32_t mPerformGroup = kPerformGroup;
+    bool mPerformGroupSet = false;    // setPerformGroup called: every segment uses it
+    uint32_t mResidentWaves = 8192;   // 256 CUs x 4 SIMDs x 8 waves; queried at initialize
     uint32_t mCheckBlock = kBlock;
     std::vector<uint32_t> mSegBlock, mSegGroup;   // per colour segment
     bool mMorton = true;          // current order within colours (with spatial sorting)
