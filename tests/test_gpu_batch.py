@@ -53,6 +53,26 @@ def test_batch_members_match_oracle(gpu_ready, graphs, xcd_ranges):
         assert b.age(m) == age + 150
 
 
+@pytest.mark.parametrize("group", [2, 4, 8])
+def test_batch_dilute_members_fill_every_perform_round(gpu_ready, group):
+    """Members whose groups of check waves hold more survivors than a perform
+    wave's first rounds: free monomers and dimers at 6 % volume fraction,
+    beside a dense melt, under every perform group."""
+    cases = [(systems.linear_melt(64, 2000, 1, seed=5), 0x5EED0000000000D3, 0),
+             (systems.linear_melt(64, 1000, 2, seed=6), 0x5EED0000000000D4, 7),
+             (systems.linear_melt(32, 128, 16, seed=7), 0x5EED0000000000D5, 3)]
+    print(f"case: dilute batch members, perform group {group}", flush=True)
+    b = Batch(0)
+    b.set_perform_group(group)
+    ids = [b.add(updater_from_system(s, seed=seed, age=age, validate_only=True), len(s["xyz"]))
+           for s, seed, age in cases]
+    b.run(50, chunk=20)
+    for m, (s, seed, age) in zip(ids, cases):
+        want, acc, prop = oracle_final(s, seed, age, 50)
+        assert np.array_equal(b.positions(m), want), f"member {m} differs"
+        assert (b.stats(m)["accepted"], b.stats(m)["proposals"]) == (acc, prop)
+
+
 def four_melts_in_a_slab(periodic):
     """Four 16^3 melts (coordinates within [0, 14]) side by side in a 64 x 16 x 32 box."""
     s = systems.linear_melt(16, 16, 16, seed=31)

@@ -256,6 +256,20 @@ def test_every_launch_candidate_keeps_the_trajectory(gpu_ready, system):
     u.cleanup()
 
 
+@pytest.mark.parametrize("group", [2, 4, 8])
+def test_dilute_systems_fill_every_perform_round(gpu_ready, group):
+    """Free monomers (93 % of proposals survive) and dimers (53 %) at 6 %
+    volume fraction: a group of g check waves holds far more survivors than
+    the perform wave's first rounds (64 for groups of 2 and 4, 128 for 8), so
+    the rounds after them run too."""
+    free = systems.linear_melt(64, 2000, 1, seed=5)
+    st = assert_parity(free, seed=0x5EED0000000000D1, chunks=[40, 25], perform_group=group)
+    assert st["proposals"] > 0.9 * st["attempts"]
+    dimers = systems.linear_melt(64, 1000, 2, seed=6)
+    st = assert_parity(dimers, seed=0x5EED0000000000D2, chunks=[40, 25], perform_group=group)
+    assert st["proposals"] > 0.5 * st["attempts"]
+
+
 def test_non_cubic_box(gpu_ready):
     # compact Morton order for a 64 x 16 x 32 box
     assert_parity(non_cubic_box_system(), seed=32, chunks=[250])
