@@ -35,6 +35,9 @@
 namespace pscbfm {
 
 constexpr int kMaxConnectivity = 8;      // MAX_CONNECTIVITY, UpdaterGPUScBFM_AB_Type.h:85
+// Partner columns the check kernel reads together with a slot's own words,
+// before the partner counts arrive (a linear chain's monomer has at most 2).
+constexpr int kEarlyPartners = 2;
 constexpr uint32_t kSegmentAlign = 64;   // monomers; 64 x 16 B = 1 KiB
 constexpr int kBlock = 256;              // threads per workgroup (4 wave64)
 constexpr int kWave = 64;                // lanes per wavefront

@@ -42,6 +42,15 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
 
 __device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Value of lane i - D of the same row of 16 lanes (DPP row_shr:D), 0 where
+// that lane lies before the row's start.
+template <int D>
+__device__ __forceinline__ uint32_t row_shr(uint32_t v)
+{
+    static_assert(D >= 1 && D <= 15, "row_shr shifts by 1..15 lanes");
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x110 + D, 0xF, 0xF, false));
+}
+
 // Pointers into global memory.  The kernel bodies access their buffers through
 // these: a batched kernel reads its member's pointers from memory, and without
 // the address space the compiler would use flat instructions for them (slower,
@@ -177,20 +186,26 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
         // source lane active; only lanes inside the segment may pass.
         const bool inside = t < count;
         const uint32_t i = begin + (inside ? t : count - 1u);
-        const uint32_t p0 = ppos[i];
+        // Two levels of dependent loads per wave: the slot's own words and
+        // its first kEarlyPartners partner indices, then the front words and
+        // the partner positions.  Every load of the first two stages is
+        // unconditional, so the compiler knows how many are in flight and
+        // waits for exactly the ones each stage needs (a load under a branch
+        // makes it wait for all of them).  The partner table has
+        // kMaxConnectivity columns, zero beyond a slot's own partners, so
+        // these reads stay in bounds; a lane uses only its first nn.
         const uint32_t idn = ids[i];
+        const uint32_t p0 = ppos[i];
+        uint32_t partner[kMaxConnectivity];
+#pragma unroll
+        for (int j = 0; j < kEarlyPartners; ++j)
+            partner[j] = nbr[size_t(j) * buf.n_pad + i];
         const uint32_t id = idn & kIdMask, nn = idn >> kIdBits;
-        // partners of the wave's busiest lane; lanes with fewer read their
-        // own slot instead (in bounds, and the result is unused)
+        // partners of the wave's busiest lane
         uint32_t nn_wave = 0;
 #pragma unroll
         for (int j = 0; j < kMaxConnectivity; ++j)
             nn_wave = __ballot(nn > uint32_t(j)) ? uint32_t(j + 1) : nn_wave;
-        uint32_t partner[kMaxConnectivity];
-#pragma unroll
-        for (int j = 0; j < kMaxConnectivity; ++j)
-            if (uint32_t(j) < nn_wave)
-                partner[j] = nbr[size_t(j) * buf.n_pad + i];
 
         const int4 r0 = unpack_xyz(p0);
         // direction = rn % 6 (.cu:964-967); the generator is uniform per launch
@@ -211,18 +226,29 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
              (!(g.periodic & 4u) && (r1.z < 0 || r1.z > g.wall_hi[2]))))
             ok = false;
 
-        // excluded volume: no corner in the 9-site front
+        // excluded volume: no corner in the 9-site front (all four words are
+        // read: each is a valid word of the lattice, and a word outside the
+        // front has mask 0)
         const Front f = front_sites(g, dep, r0, dir);
         uint32_t l[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            l[q] = f.m[q] ? lat[f.w[q]] : 0u;
+            l[q] = lat[f.w[q]];
 
         // bond partners belong to other colours and do not move this substep;
-        // every position load is issued before the first is used
+        // every position load is issued before the first is used (a lane
+        // with fewer partners reads its own slot instead)
         uint32_t qpos[kMaxConnectivity];
 #pragma unroll
-        for (int j = 0; j < kMaxConnectivity; ++j)
+        for (int j = 0; j < kEarlyPartners; ++j)
+            qpos[j] = ppos[uint32_t(j) < nn ? partner[j] : i];
+        // the rest of the partners, after the counts (networks' crosslinkers)
+#pragma unroll
+        for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
+            if (uint32_t(j) < nn_wave)
+                partner[j] = nbr[size_t(j) * buf.n_pad + i];
+#pragma unroll
+        for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
             if (uint32_t(j) < nn_wave)
                 qpos[j] = ppos[uint32_t(j) < nn ? partner[j] : i];
         // (nn_wave is wave-uniform: the lookups run on every lane; a lane
@@ -297,20 +323,24 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
     const gptr<uint32_t> ppos = as_global(buf.ppos);
 
     // survivor counts of the group's check waves and their inclusive prefix
-    // sums, held by lanes 0..kGroup-1
+    // sums, held by lanes 0..kGroup-1 (within the first row of 16 lanes, so
+    // the scan is row shifts between lanes, not a trip through LDS; a lane
+    // whose source lies before the row start adds 0)
+    static_assert(kGroup <= 16, "the survivor-count scan runs within one DPP row");
     uint32_t incl = 0;
     if (lane < kGroup && cw0 + lane < check_waves)
         incl = scount[cw0 + lane];
-#pragma unroll
-    for (uint32_t d = 1; d < kGroup; d <<= 1) {
-        const uint32_t up = __shfl_up(incl, d);
-        if (lane >= d)
-            incl += up;
-    }
+    incl += row_shr<1>(incl);
+    if (kGroup > 2)
+        incl += row_shr<2>(incl);
+    if (kGroup > 4)
+        incl += row_shr<4>(incl);
+    if (kGroup > 8)
+        incl += row_shr<8>(incl);
     uint32_t ends[kGroup];
 #pragma unroll
     for (uint32_t k = 0; k < kGroup; ++k)
-        ends[k] = wave_uniform(__shfl(incl, int(k)));
+        ends[k] = uint32_t(__builtin_amdgcn_readlane(int(incl), int(k)));
     const uint32_t total = ends[kGroup - 1];
 
     // slot in surv of the group's survivor t: check wave k of the group holds
@@ -384,7 +414,7 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
         f[r] = front_of(e[r]);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            k4[r][q] = live[r] && f[r].m[q] ? claim[f[r].w[q]] : 0u;
+            k4[r][q] = claim[f[r].w[q]];
     }
 #pragma unroll
     for (uint32_t r = 0; r < kRounds; ++r) {
@@ -400,7 +430,7 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
         uint32_t c4[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            c4[q] = f1.m[q] ? claim[f1.w[q]] : 0u;
+            c4[q] = claim[f1.w[q]];
         const bool go = !clashes(f1, c4);
         if (go)
             apply(e1, f1);
