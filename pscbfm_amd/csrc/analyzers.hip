@@ -42,7 +42,8 @@ __global__ __launch_bounds__(kHistBins) void k_clear_result(DeviceResult *r)
 __global__ __launch_bounds__(kAnaBlock) void k_positions(const uint32_t *__restrict__ ppos,
                                                          const uint32_t *__restrict__ ids,
                                                          const int32_t *__restrict__ image, uint32_t n_pad,
-                                                         int64_t bx, int64_t by, int64_t bz, int4 *__restrict__ pos)
+                                                         Geometry g, int64_t bx, int64_t by, int64_t bz,
+                                                         int4 *__restrict__ pos)
 {
     const uint32_t t = blockIdx.x * kAnaBlock + threadIdx.x;
     if (t >= n_pad)
@@ -50,10 +51,10 @@ __global__ __launch_bounds__(kAnaBlock) void k_positions(const uint32_t *__restr
     const uint32_t id = ids[t];
     if (id == 0xFFFFFFFFu)
         return;
-    const int4 w = unpack_xyz(ppos[t]);
-    const int64_t x = int64_t(w.x) + int64_t(image[t]) * bx;
-    const int64_t y = int64_t(w.y) + int64_t(image[size_t(n_pad) + t]) * by;
-    const int64_t z = int64_t(w.z) + int64_t(image[2 * size_t(n_pad) + t]) * bz;
+    const uint32_t site = ppos[t];
+    const int64_t x = int64_t(site_coord(site, g.dmask[0])) + int64_t(image[t]) * bx;
+    const int64_t y = int64_t(site_coord(site, g.dmask[1])) + int64_t(image[size_t(n_pad) + t]) * by;
+    const int64_t z = int64_t(site_coord(site, g.dmask[2])) + int64_t(image[2 * size_t(n_pad) + t]) * bz;
     pos[id & kIdMask] = make_int4(int32_t(x), int32_t(y), int32_t(z), 0);
 }
 
@@ -319,7 +320,8 @@ void DeviceAnalyzer::computeChains(const HostSystem &s, const SweepBuffers &buf,
 {
     const dim3 block(kAnaBlock);
     hipLaunchKernelGGL(k_positions, dim3((s.n_pad + kAnaBlock - 1) / kAnaBlock), block, 0, stream, buf.ppos,
-                       buf.ids, buf.image, s.n_pad, int64_t(s.box[0]), int64_t(s.box[1]), int64_t(s.box[2]), mPos);
+                       buf.ids, buf.image, s.n_pad, buf.geom, int64_t(s.box[0]), int64_t(s.box[1]),
+                       int64_t(s.box[2]), mPos);
     hip_check(hipGetLastError(), "analyzer positions");
     const uint32_t C = mChains.linear ? mChains.n_chains : 0;
     if (C) {

@@ -3,7 +3,9 @@
 // Layout of one system in HBM (all arrays are indexed by the *sorted* monomer
 // index: monomers grouped by colour, each colour segment padded to a multiple
 // of kSegmentAlign so that every colour starts on an aligned boundary):
-//   uint32_t ppos[nPad]        wrapped x | y << 10 | z << 20 (box edges <= 1024)
+//   uint32_t ppos[nPad]        lattice site of the monomer's lower corner
+//                              (compact Morton index, the bit of the corner
+//                              in the occupancy lattice)
 //   uint32_t ids[nPad]         original monomer id (keys the RNG) | number of
 //                              bond partners << 28
 //   int32_t  image[3][nPad]    periodic image of each axis: unwrapped = wrapped
@@ -11,22 +13,22 @@
 //                              the box boundary
 //   uint32_t nbr[8][nPad]      sorted index of the j-th bond partner (slot-major,
 //                              so lane i reads nbr[j*nPad+i] coalesced)
-//   uint32_t lattice[W]        occupancy bit lattice: the 8 sites of every
-//                              monomer cube, compact Morton order
-//   uint32_t claim[W]          sites entered by this substep's proposals
-//   uint32_t coll[2][W]        sites entered by more than one proposal
-//   uint2    surv[blocks*256]  compacted proposals: (sorted index | dir << 29,
-//                              packed position), per check workgroup
-//   uint32_t clist[2][...]     per-workgroup lists of coll[] words to clear
+//   uint32_t lattice[W]        occupancy bit lattice: one bit per monomer, at
+//                              its lower corner, compact Morton order
+//   uint32_t claim[2][W]       corners entered by a substep's proposals,
+//                              double-buffered by substep parity
+//   uint2    surv[waves*64]    compacted proposals: (sorted index | dir << 29,
+//                              corner site), per check wavefront
+//   uint32_t scount[waves]     proposals per check wavefront
 //
-// Positions are wrapped on the device: a bond vector is only needed modulo 8
-// (the bond-table index (d & 7), UpdaterGPUScBFM_AB_Type.cu:871-889) and every
-// box edge is a multiple of 8, so wrapped coordinates give the same index.
+// A position is kept as its site index and moved with dilated (Morton)
+// arithmetic on one axis's bits: a bond vector is only needed modulo 8 (the
+// bond-table index (d & 7), UpdaterGPUScBFM_AB_Type.cu:871-889), and the low
+// 3 bits of every axis are the low 9 bits of the site in every geometry.
 //
-// The reference keeps 1 lattice byte per monomer (its lower corner) and tests
-// a 9-site front (UpdaterGPUScBFM_AB_Type.cu:420-594); this engine stores the
-// full cube, 1 bit per site, so the same excluded-volume rule needs only the
-// 4 sites of the face the cube enters.
+// The lattice keeps the reference's 1 site per monomer (its lower corner) and
+// tests the reference's 9-site front (UpdaterGPUScBFM_AB_Type.cu:420-594), in
+// at most 4 words of 32 sites.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -49,18 +51,6 @@ constexpr uint32_t kSortedIndexMask = (1u << kSortedIndexBits) - 1u;
 constexpr uint32_t kIdBits = 28;
 constexpr uint32_t kIdMask = (1u << kIdBits) - 1u;
 constexpr uint32_t kMaxBox = 1024;       // 10 bits per axis, 30-bit site index
-constexpr uint32_t kCoordBits = 10;      // bits per axis of a packed position
-constexpr uint32_t kCoordMask = (1u << kCoordBits) - 1u;
-
-__host__ __device__ __forceinline__ uint32_t pack_xyz(uint32_t x, uint32_t y, uint32_t z)
-{
-    return x | (y << kCoordBits) | (z << (2 * kCoordBits));
-}
-__host__ __device__ __forceinline__ int4 unpack_xyz(uint32_t p)
-{
-    return make_int4(int32_t(p & kCoordMask), int32_t((p >> kCoordBits) & kCoordMask),
-                     int32_t((p >> (2 * kCoordBits)) & kCoordMask), 0);
-}
 
 // Geometry of one periodic (or walled) power-of-two box.
 struct Geometry {
@@ -71,12 +61,14 @@ struct Geometry {
     uint32_t cubic;      // 1: all edges equal, the index is the classic Z-curve
 };
 
-// Small per-system tables; each workgroup copies them to LDS.
+// Small per-system tables.
 struct Tables {
-    uint32_t dep[3][64];  // deposit of coordinate bits: [a][v & 31] | [a][32 + (v >> 5)]
-    uint32_t bond[16];    // 512-bit forbidden-bond mask, bit (dx&7)|(dy&7)<<3|(dz&7)<<6
+    uint32_t dep[3][64];  // deposit of coordinate bits: [a][v & 31] | [a][32 + (v >> 5)] (host side)
+    // 512-bit forbidden-bond mask in the site's bit order: bit bond_key() of
+    // the vector (dx & 7, dy & 7, dz & 7), i.e. those three 3-bit numbers
+    // interleaved x, y, z from bit 0 (see bond_key)
+    uint32_t bond[16];
 };
-constexpr int kTableWords = 3 * 64 + 16;
 
 // ---- Philox4x32-10 (Salmon et al., SC'11) -----------------------------------
 __host__ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
@@ -171,36 +163,48 @@ __host__ __device__ __forceinline__ uint32_t spread3(uint32_t v)
     return v;
 }
 
-// Index bits of coordinate v (already masked) on axis a for a cubic box: the
-// compact Morton order is then the classic Z-curve, x in bit 0, computed in
-// registers.  Non-cubic boxes use per-axis tables in LDS (bfm_kernels.hip).
-struct DepositCubic {
-    __host__ __device__ __forceinline__ uint32_t operator()(int a, uint32_t v) const { return spread3(v) << a; }
-};
-__host__ __device__ __forceinline__ uint32_t dilated_inc(uint32_t d, uint32_t m) { return ((d | ~m) + 1u) & m; }
-__host__ __device__ __forceinline__ uint32_t dilated_dec(uint32_t d, uint32_t m) { return (d - 1u) & m; }
+// Arithmetic on one axis of a site index.  x holds only bits of the axis's
+// mask m (dmask), k is a dilated constant of the axis; the result wraps modulo
+// the box edge.  Adding sets every other bit so that carries run through them.
+__host__ __device__ __forceinline__ uint32_t dil_add(uint32_t x, uint32_t k, uint32_t m) { return ((x | ~m) + k) & m; }
+__host__ __device__ __forceinline__ uint32_t dil_sub(uint32_t x, uint32_t k, uint32_t m) { return (x - k) & m; }
+
+// Coordinate of axis `dmask` of a site: the axis's bits gathered downward.
+// (For the copy-back and the analyzers; the sweep never needs coordinates.)
+__host__ __device__ inline uint32_t site_coord(uint32_t site, uint32_t dmask)
+{
+    uint32_t v = 0, bit = 1;
+    for (uint32_t m = dmask; m != 0u; m &= m - 1u, bit <<= 1)
+        if (site & m & (0u - m))
+            v |= bit;
+    return v;
+}
 
 __host__ __device__ __forceinline__ int32_t comp(const int4 &r, int a) { return a == 0 ? r.x : (a == 1 ? r.y : r.z); }
 // v[a] for a per-lane axis without indexing memory (kernel arguments live in
 // SGPRs; a dynamic index would turn them into scratch or vector loads).
 __host__ __device__ __forceinline__ uint32_t sel3(const uint32_t v[3], int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); }
 
+// ---- the 9-site front ---------------------------------------------------------
 // The lattice holds one bit per monomer, at its lower corner.  A cube with
 // corner r moving in direction dir (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z, as the
 // reference's DXTable_d/DYTable_d/DZTable_d, UpdaterGPUScBFM_AB_Type.cu:
 // 1537-1541) can only overlap a cube whose corner lies in the 3x3 "front":
-// the plane two sites ahead of r on the move axis, one site either side of r
-// on the other two (checkFront, UpdaterGPUScBFM_AB_Type.cu:420-594).  The
-// same 9 sites are where another moved corner makes two moved cubes overlap.
+// the plane two sites ahead of r on the move axis a, one site either side of
+// r on the other two axes b and c (checkFront, UpdaterGPUScBFM_AB_Type.cu:
+// 420-594).  The same 9 sites are where another moved corner makes two moved
+// cubes overlap.
 //
-// Index bits 0..4 of a site are x0 y0 z0 x1 y1 for every box (edges >= 8), so
-// a 32-bit word spans 4 x 4 x 2 sites and 3 consecutive coordinates of one
-// axis fall into at most 2 words: the 9 sites lie in at most 4 words.  w[q]
-// are those words, m[q] the bits of the front in each (m[q] == 0: not needed).
+// Index bits 0..4 of a site are x0 y0 z0 x1 y1 for every box (edges >= 8,
+// the interleave takes one bit of each axis per round while the axis has
+// bits), so a 32-bit word spans 4 x 4 x 2 sites.  The three b values of the
+// front fall into at most 2 words, as do the three c values: the 9 sites lie
+// in the words of (plane, b-1 or b+1, c-1 or c+1), w[0..3] below, and m[q]
+// are the front's bits in w[q] (0 where w[q] repeats an earlier word).
 struct Front {
     uint32_t w[4], m[4];
     uint32_t new_site;   // corner after the move
-    uint32_t old_site;   // corner before the move
+    uint32_t ma;         // dmask of the move axis
 };
 
 // Product of two factors below 2^24 (one v_mul_u32_u24 on the device).
@@ -209,57 +213,86 @@ __host__ __device__ __forceinline__ uint32_t mul24(uint32_t a, uint32_t b)
     return (a & 0xFFFFFFu) * (b & 0xFFFFFFu);
 }
 
-template <class Dep>
-__host__ __device__ __forceinline__ Front front_sites(const Geometry &g, const Dep &dep, const int4 &r, int dir)
+// Dilated masks of the move axis a and the two front axes b, c of direction dir.
+template <bool kCubic>
+__host__ __device__ __forceinline__ void front_axes(const Geometry &g, int dir, uint32_t &ma, uint32_t &mb,
+                                                    uint32_t &mc)
 {
     const int a = dir >> 1;
-    const int b = a == 0 ? 1 : 0;
-    const int c = a == 2 ? 1 : 2;
+    if (kCubic) {
+        // every axis's bits are the x bits shifted by the axis number
+        ma = g.dmask[0] << a;
+        mb = g.dmask[0] << (a == 0 ? 1 : 0);
+        mc = g.dmask[0] << (a == 2 ? 1 : 2);
+    } else {
+        ma = sel3(g.dmask, a);
+        mb = a == 0 ? g.dmask[1] : g.dmask[0];
+        mc = a == 2 ? g.dmask[1] : g.dmask[2];
+    }
+}
+
+template <bool kCubic>
+__host__ __device__ __forceinline__ Front front_of(const Geometry &g, uint32_t site, int dir)
+{
+    uint32_t ma, mb, mc;
+    front_axes<kCubic>(g, dir, ma, mb, mc);
+    const int a = dir >> 1;
     const bool up = (dir & 1) != 0;
-    const uint32_t dma = sel3(g.dmask, a), dmb = sel3(g.dmask, b), dmc = sel3(g.dmask, c);
-    const uint32_t da = dep(a, uint32_t(up ? comp(r, a) + 2 : comp(r, a) - 2) & sel3(g.mask, a));
-    const uint32_t b0 = dep(b, uint32_t(comp(r, b) - 1) & sel3(g.mask, b));
-    const uint32_t c0 = dep(c, uint32_t(comp(r, c) - 1) & sel3(g.mask, c));
-    const uint32_t b1 = dilated_inc(b0, dmb), b2 = dilated_inc(b1, dmb);
-    const uint32_t c1 = dilated_inc(c0, dmc), c2 = dilated_inc(c1, dmc);
-    // the moved corner is one step behind the front plane, the old one two
-    const uint32_t a1 = up ? dilated_dec(da, dma) : dilated_inc(da, dma);
-    const uint32_t a0 = up ? dilated_dec(a1, dma) : dilated_inc(a1, dma);
+    // dilated 1 and 2 of an axis are its first two bits: bits a and a + 3
+    // for the move axis, the lowest bit of the mask for b and c
+    const uint32_t one_a = 1u << a, two_a = 1u << (a + 3);
+    const uint32_t one_b = mb & (0u - mb), one_c = mc & (0u - mc);
+    const uint32_t sa = site & ma, sb = site & mb, sc = site & mc;
+    // a - 2 is a + (L - 2), whose dilated form is the mask without its lowest
+    // bit; a - 1 is a + (L - 1), the mask itself
+    const uint32_t plane = dil_add(sa, up ? two_a : (ma ^ one_a), ma);
+    const uint32_t a1 = dil_add(sa, up ? one_a : ma, ma);
+    const uint32_t b0 = dil_sub(sb, one_b, mb), b2 = dil_add(sb, one_b, mb);
+    const uint32_t c0 = dil_sub(sc, one_c, mc), c2 = dil_add(sc, one_c, mc);
     Front f;
-    f.new_site = a1 | b1 | c1;
-    f.old_site = a0 | b1 | c1;
-    // Site bit = 2^(la + lb + lc) with la, lb, lc the in-word parts of the
-    // three axes (disjoint bit positions, so | is +).  Summing 2^lb over the
-    // b-rows that share a word (sb_*) and 2^lc likewise (sc_*), the mask of a
-    // word is sb * sc << la: the products of distinct powers never carry.
-    // Every factor is < 2^19 (y1 is index bit 4), so 24-bit multiplies do.
-    const uint32_t hb0 = b0 >> 5, hc0 = c0 >> 5;
-    const bool b1_hi = (b1 >> 5) != hb0, b2_hi = (b2 >> 5) != hb0;
-    const bool c1_hi = (c1 >> 5) != hc0, c2_hi = (c2 >> 5) != hc0;
-    const uint32_t pb0 = 1u << (b0 & 31u), pb1 = 1u << (b1 & 31u), pb2 = 1u << (b2 & 31u);
-    const uint32_t pc0 = 1u << (c0 & 31u), pc1 = 1u << (c1 & 31u), pc2 = 1u << (c2 & 31u);
+    f.w[0] = (plane | b0 | c0) >> 5;
+    f.w[1] = (plane | b2 | c0) >> 5;
+    f.w[2] = (plane | b0 | c2) >> 5;
+    f.w[3] = (plane | b2 | c2) >> 5;
+    // The bit of a site in its word is the sum of its three axes' in-word
+    // index bits (disjoint positions), so it is 2^la * 2^lb * 2^lc.  Summing
+    // 2^lb over the b values that share a word (sb_*) and 2^lc likewise
+    // (sc_*), the mask of a word is sb * sc << la: the products of distinct
+    // powers never carry.  Every factor is < 2^19 (y1 is index bit 4), so
+    // 24-bit multiplies do.  A value has left the word of b - 1 when its
+    // index bits above the word differ.
+    const bool b1_hi = ((sb ^ b0) >> 5) != 0u, b2_hi = ((b2 ^ b0) >> 5) != 0u;
+    const bool c1_hi = ((sc ^ c0) >> 5) != 0u, c2_hi = ((c2 ^ c0) >> 5) != 0u;
+    const uint32_t pb0 = 1u << (b0 & 31u), pb1 = 1u << (sb & 31u), pb2 = 1u << (b2 & 31u);
+    const uint32_t pc0 = 1u << (c0 & 31u), pc1 = 1u << (sc & 31u), pc2 = 1u << (c2 & 31u);
     const uint32_t sb_lo = pb0 | (b1_hi ? 0u : pb1) | (b2_hi ? 0u : pb2);
     const uint32_t sb_hi = (b1_hi ? pb1 : 0u) | (b2_hi ? pb2 : 0u);
     const uint32_t sc_lo = pc0 | (c1_hi ? 0u : pc1) | (c2_hi ? 0u : pc2);
     const uint32_t sc_hi = (c1_hi ? pc1 : 0u) | (c2_hi ? pc2 : 0u);
-    const uint32_t la = da & 31u, ha = da >> 5;
-    const uint32_t hb2 = b2 >> 5, hc2 = c2 >> 5;
-    f.w[0] = ha | hb0 | hc0;
-    f.w[1] = ha | hb2 | hc0;
-    f.w[2] = ha | hb0 | hc2;
-    f.w[3] = ha | hb2 | hc2;
+    const uint32_t la = plane & 31u;
     f.m[0] = mul24(sb_lo, sc_lo) << la;
     f.m[1] = mul24(sb_hi, sc_lo) << la;
     f.m[2] = mul24(sb_lo, sc_hi) << la;
     f.m[3] = mul24(sb_hi, sc_hi) << la;
+    f.new_site = a1 | sb | sc;
+    f.ma = ma;
     return f;
 }
 
-// Site of corner (x, y, z) (wrapped coordinates).
-template <class Dep>
-__host__ __device__ __forceinline__ uint32_t corner_site(const Dep &dep, const int4 &r)
+// Key of a bond vector from the partner's site and the moved corner's site:
+// the three per-axis differences modulo 8, taken on the low 9 site bits
+// (x0 y0 z0 x1 y1 z1 x2 y2 z2 in every geometry), interleaved as there.
+// r1x/r1y/r1z are the moved corner's site masked with kLow9[axis].
+constexpr uint32_t kLow9[3] = {0x049u, 0x092u, 0x124u};
+__host__ __device__ __forceinline__ uint32_t bond_key(uint32_t partner_site, uint32_t r1x, uint32_t r1y, uint32_t r1z)
 {
-    return dep(0, uint32_t(r.x)) | dep(1, uint32_t(r.y)) | dep(2, uint32_t(r.z));
+    return dil_sub(partner_site & kLow9[0], r1x, kLow9[0]) | dil_sub(partner_site & kLow9[1], r1y, kLow9[1]) |
+           dil_sub(partner_site & kLow9[2], r1z, kLow9[2]);
+}
+// The same key from a bond vector.
+__host__ __device__ __forceinline__ uint32_t bond_key_of(int dx, int dy, int dz)
+{
+    return spread3(uint32_t(dx) & 7u) | (spread3(uint32_t(dy) & 7u) << 1) | (spread3(uint32_t(dz) & 7u) << 2);
 }
 
 // Bond-table index of a bond vector: (dx & 7) | (dy & 7) << 3 | (dz & 7) << 6.

@@ -111,28 +111,6 @@ __device__ __forceinline__ void buffers_in_sgprs(const SweepBuffers &b)
     }
 }
 
-__device__ __forceinline__ int4 moved(const int4 &r, int dir)
-{
-    const int32_t s = (dir & 1) ? 1 : -1;
-    int4 q = r;
-    const int a = dir >> 1;
-    q.x += a == 0 ? s : 0;
-    q.y += a == 1 ? s : 0;
-    q.z += a == 2 ? s : 0;
-    return q;
-}
-
-// The Morton tables of a non-cubic box, in LDS (a file-scope __shared__ array
-// keeps the LDS address space visible to the compiler).
-__shared__ uint32_t s_dep[3 * 64];
-
-__device__ __forceinline__ void load_dep(const Tables *t)
-{
-    const gptr<const uint32_t> src = as_global(t)->dep[0];
-    for (int k = threadIdx.x; k < 3 * 64; k += blockDim.x)
-        s_dep[k] = src[k];
-}
-
 // The 512-bit forbidden-bond mask held across a wavefront: lane k holds word
 // k & 15, and a lookup is one cross-lane read.  Unlike a table in LDS it needs
 // no workgroup barrier before the first data load, and its own load overlaps
@@ -142,22 +120,12 @@ struct WaveBondMask {
     uint32_t word;
     __device__ __forceinline__ explicit WaveBondMask(const Tables *t)
         : word(as_global(t)->bond[threadIdx.x & 15u]) {}
-    __device__ __forceinline__ bool forbidden(int dx, int dy, int dz) const
+    // key: bond_key() of the bond vector (a 9-bit index)
+    __device__ __forceinline__ bool forbidden(uint32_t key) const
     {
-        const uint32_t b = bond_index(dx, dy, dz);
-        return (uint32_t(__shfl(int(word), int(b >> 5))) >> (b & 31u)) & 1u;
+        return (uint32_t(__shfl(int(word), int(key >> 5))) >> (key & 31u)) & 1u;
     }
 };
-
-struct DepositTable {
-    __device__ __forceinline__ uint32_t operator()(int a, uint32_t v) const
-    {
-        return s_dep[a * 64 + (v & 31u)] | s_dep[a * 64 + 32 + (v >> 5)];
-    }
-};
-
-template <bool kCubic>
-using DepositFor = typename std::conditional<kCubic, DepositCubic, DepositTable>::type;
 
 // Phase 1 for check block `blk` of `nblk` covering the colour segment
 // [begin, begin+count).  Every wavefront compacts its own survivors into surv.
@@ -165,7 +133,6 @@ template <bool kAllPeriodic, bool kCubic, uint32_t kThreads = kBlock>
 __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
                                            uint32_t sub_seed, uint32_t parity, uint32_t blk, uint32_t nblk)
 {
-    const DepositFor<kCubic> dep{};
     const Geometry &g = buf.geom;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t gwave = wave_uniform(blk * (kThreads / kWave) + (threadIdx.x >> 6));
@@ -187,27 +154,24 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
         const bool inside = t < count;
         const uint32_t i = begin + (inside ? t : count - 1u);
         // Two levels of dependent loads per wave: the slot's own words and
-        // its first kEarlyPartners partner indices, then the front words and
-        // the partner positions.  Every load of the first two stages is
-        // unconditional, so the compiler knows how many are in flight and
-        // waits for exactly the ones each stage needs (a load under a branch
-        // makes it wait for all of them).  The partner table has
+        // its first kEarlyPartners partner indices, then the front words,
+        // the front masks and the partner sites.  Every load of the first two
+        // stages is unconditional, so the compiler knows how many are in
+        // flight and waits for exactly the ones each stage needs (a load under
+        // a branch makes it wait for all of them).  The partner table has
         // kMaxConnectivity columns, zero beyond a slot's own partners, so
         // these reads stay in bounds; a lane uses only its first nn.
         const uint32_t idn = ids[i];
-        const uint32_t p0 = ppos[i];
+        const uint32_t s0 = ppos[i];
         uint32_t partner[kMaxConnectivity];
 #pragma unroll
         for (int j = 0; j < kEarlyPartners; ++j)
             partner[j] = nbr[size_t(j) * buf.n_pad + i];
         const uint32_t id = idn & kIdMask, nn = idn >> kIdBits;
-        // partners of the wave's busiest lane
-        uint32_t nn_wave = 0;
-#pragma unroll
-        for (int j = 0; j < kMaxConnectivity; ++j)
-            nn_wave = __ballot(nn > uint32_t(j)) ? uint32_t(j + 1) : nn_wave;
+        // a lane of the wave with partners beyond the early columns (a
+        // network's crosslinker)
+        const bool more = __ballot(nn > uint32_t(kEarlyPartners)) != 0ull;
 
-        const int4 r0 = unpack_xyz(p0);
         // direction = rn % 6 (.cu:964-967); the generator is uniform per launch
         uint32_t rn;
         if (buf.rng == kRngPhilox)
@@ -215,49 +179,58 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
         else
             rn = int_hash(int_hash(id) ^ sub_seed);
         const int dir = int(rn % 6u);
-        const int4 r1 = moved(r0, dir);
-
-        bool ok = true;
-        // walls: every axis against its walls (the axes not moved along are
-        // inside already), so that no per-lane index reaches the buffers struct
-        if (!kAllPeriodic &&
-            ((!(g.periodic & 1u) && (r1.x < 0 || r1.x > g.wall_hi[0])) ||
-             (!(g.periodic & 2u) && (r1.y < 0 || r1.y > g.wall_hi[1])) ||
-             (!(g.periodic & 4u) && (r1.z < 0 || r1.z > g.wall_hi[2]))))
-            ok = false;
 
         // excluded volume: no corner in the 9-site front (all four words are
-        // read: each is a valid word of the lattice, and a word outside the
-        // front has mask 0)
-        const Front f = front_sites(g, dep, r0, dir);
+        // read: each is a word of the lattice, and a repeated word has mask 0)
+        const Front f = front_of<kCubic>(g, s0, dir);
         uint32_t l[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             l[q] = lat[f.w[q]];
 
         // bond partners belong to other colours and do not move this substep;
-        // every position load is issued before the first is used (a lane
-        // with fewer partners reads its own slot instead)
+        // every site load is issued before the first is used (a lane with
+        // fewer partners reads its own slot instead)
         uint32_t qpos[kMaxConnectivity];
 #pragma unroll
         for (int j = 0; j < kEarlyPartners; ++j)
             qpos[j] = ppos[uint32_t(j) < nn ? partner[j] : i];
-        // the rest of the partners, after the counts (networks' crosslinkers)
+        uint32_t nn_wave = uint32_t(kEarlyPartners);
+        if (more) {
+            // the rest of the partners, after the counts
+#pragma unroll
+            for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
+                nn_wave = __ballot(nn > uint32_t(j)) ? uint32_t(j + 1) : nn_wave;
+#pragma unroll
+            for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
+                if (uint32_t(j) < nn_wave)
+                    partner[j] = nbr[size_t(j) * buf.n_pad + i];
+#pragma unroll
+            for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
+                if (uint32_t(j) < nn_wave)
+                    qpos[j] = ppos[uint32_t(j) < nn ? partner[j] : i];
+        }
+
+        bool ok = true;
+        // walls: a move along a walled axis may not leave [0, L - 2]; the
+        // moved corner's coordinate is then L - 1 (from L - 2 upward, or from
+        // 0 downward modulo L), whose dilated form is the axis's mask
+        if (!kAllPeriodic && !((g.periodic >> (dir >> 1)) & 1u) && (f.new_site & f.ma) == f.ma)
+            ok = false;
+
+        // bonds from the moved corner (the lookups run on every lane; a lane
+        // with fewer partners tests its own slot and ignores the result)
+        const uint32_t r1x = f.new_site & kLow9[0], r1y = f.new_site & kLow9[1], r1z = f.new_site & kLow9[2];
+#pragma unroll
+        for (int j = 0; j < kEarlyPartners; ++j) {
+            const bool bad = bonds.forbidden(bond_key(qpos[j], r1x, r1y, r1z));
+            if (uint32_t(j) < nn && bad)
+                ok = false;
+        }
 #pragma unroll
         for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
-            if (uint32_t(j) < nn_wave)
-                partner[j] = nbr[size_t(j) * buf.n_pad + i];
-#pragma unroll
-        for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
-            if (uint32_t(j) < nn_wave)
-                qpos[j] = ppos[uint32_t(j) < nn ? partner[j] : i];
-        // (nn_wave is wave-uniform: the lookups run on every lane; a lane
-        // with fewer partners tests its own position and ignores the result)
-#pragma unroll
-        for (int j = 0; j < kMaxConnectivity; ++j)
             if (uint32_t(j) < nn_wave) {
-                const int4 q = unpack_xyz(qpos[j]);
-                const bool bad = bonds.forbidden(q.x - r1.x, q.y - r1.y, q.z - r1.z);
+                const bool bad = bonds.forbidden(bond_key(qpos[j], r1x, r1y, r1z));
                 if (uint32_t(j) < nn && bad)
                     ok = false;
             }
@@ -269,7 +242,7 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
             // one bit per moved corner
             atomicOr((uint32_t *)&claim_cur[f.new_site >> 5], 1u << (f.new_site & 31u));
             pass = true;
-            entry = make_uint2(i | (uint32_t(dir) << kSortedIndexBits), p0);
+            entry = make_uint2(i | (uint32_t(dir) << kSortedIndexBits), s0);
         }
     }
 
@@ -294,10 +267,6 @@ template <bool kAllPeriodic, bool kCubic, uint32_t kThreads>
 __global__ __launch_bounds__(kThreads) void k_check(SweepBuffers buf, uint32_t begin, uint32_t count, uint64_t age,
                                                     uint32_t sub_seed, uint32_t parity)
 {
-    if (!kCubic) {
-        load_dep(buf.tables);
-        __syncthreads();
-    }
     // the grid size from the arguments (launch_check sizes the grid so), not
     // from the launch's implicit arguments, which are read after a branch
     const uint32_t nblk = check_blocks(count, kThreads);
@@ -311,7 +280,6 @@ template <bool kCubic, uint32_t kGroup>
 __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t check_waves, uint32_t parity,
                                              uint32_t blk)
 {
-    const DepositFor<kCubic> dep{};
     const Geometry &g = buf.geom;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t pwave = wave_uniform(blk * kWavesPerBlock + (threadIdx.x >> 6));
@@ -373,25 +341,22 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
         // their new values with plain byte stores: no atomics, no reads.
         const uint32_t i = e.x & kSortedIndexMask;
         const int dir = int(e.x >> kSortedIndexBits);
-        const int4 r0 = unpack_xyz(e.y);
+        const uint32_t old_site = e.y;
         const gptr<uint8_t> blocks = (gptr<uint8_t>)lattice;
         const uint32_t nb = f.new_site >> 3;
-        if ((f.old_site >> 3) != nb)
-            blocks[f.old_site >> 3] = uint8_t(0);
+        if ((old_site >> 3) != nb)
+            blocks[old_site >> 3] = uint8_t(0);
         blocks[nb] = uint8_t(1u << (f.new_site & 7u));
-        // new wrapped coordinate on the move axis; crossing the box boundary
-        // changes the periodic image (only on periodic axes: walls keep the
-        // others inside)
-        const int a = dir >> 1;
-        const uint32_t am = sel3(g.mask, a);
-        const uint32_t v = uint32_t(comp(r0, a) + ((dir & 1) ? 1 : -1)) & am;
-        const uint32_t shift = kCoordBits * uint32_t(a);
-        ppos[i] = (e.y & ~(kCoordMask << shift)) | (v << shift);
-        if ((dir & 1) ? v == 0u : v == am)
-            as_global(buf.image)[size_t(a) * buf.n_pad + i] += (dir & 1) ? 1 : -1;
+        ppos[i] = f.new_site;
+        // crossing the box boundary changes the periodic image (only on
+        // periodic axes: walls keep the others inside): the moved corner's
+        // coordinate on the move axis wrapped to 0 upward or to L - 1 downward
+        const uint32_t a1 = f.new_site & f.ma;
+        if ((dir & 1) ? a1 == 0u : a1 == f.ma)
+            as_global(buf.image)[size_t(dir >> 1) * buf.n_pad + i] += (dir & 1) ? 1 : -1;
     };
-    auto front_of = [&](const uint2 &e) {
-        return front_sites(g, dep, unpack_xyz(e.y), int(e.x >> kSortedIndexBits));
+    auto front_of_entry = [&](const uint2 &e) {
+        return front_of<kCubic>(g, e.y, int(e.x >> kSortedIndexBits));
     };
 
     // The survivors in rounds of 64 lanes.  The first kRounds rounds (enough
@@ -411,7 +376,7 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
     uint32_t k4[kRounds][4];
 #pragma unroll
     for (uint32_t r = 0; r < kRounds; ++r) {
-        f[r] = front_of(e[r]);
+        f[r] = front_of_entry(e[r]);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             k4[r][q] = claim[f[r].w[q]];
@@ -426,7 +391,7 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
     // the rest, one round at a time (more survivors than kRounds rounds hold)
     for (uint32_t t = lane + kRounds * kWave; t < total; t += kWave) {
         const uint2 e1 = element(surv + slot(t));
-        const Front f1 = front_of(e1);
+        const Front f1 = front_of_entry(e1);
         uint32_t c4[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -446,10 +411,6 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
 template <bool kCubic, uint32_t kGroup>
 __global__ __launch_bounds__(kBlock) void k_perform(SweepBuffers buf, uint32_t check_waves, uint32_t parity)
 {
-    if (!kCubic) {
-        load_dep(buf.tables);
-        __syncthreads();
-    }
     // the grid size from the arguments, as in k_check
     const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, perform_blocks_for(check_waves, kGroup)) : blockIdx.x;
     perform_body<kCubic, kGroup>(buf, check_waves, parity, blk);
@@ -519,10 +480,6 @@ __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__res
     if (e.colour == kNoColour || be.blk >= nb)
         return;
     const uint32_t blk = be.blk;
-    if (!kCubic) {
-        load_dep(buf.tables);
-        __syncthreads();
-    }
     check_body<kAllPeriodic, kCubic>(buf, e.begin, e.count, e.age, e.sub_seed, e.parity, blk, nb);
 }
 
@@ -543,10 +500,6 @@ __global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__r
     if (e.colour == kNoColour || be.blk >= perform_blocks(cb, kGroup))
         return;
     const uint32_t blk = be.blk;
-    if (!kCubic) {
-        load_dep(buf.tables);
-        __syncthreads();
-    }
     perform_body<kCubic, kGroup>(buf, cb * kWavesPerBlock, e.parity, blk);
 }
 
@@ -554,11 +507,12 @@ __global__ __launch_bounds__(kBlock) void k_perform_batch(const BatchSystem *__r
 // reference's doCopyBack (UpdaterGPUScBFM_AB_Type.cu:3402-3484: unsort, and
 // position + image * box from :1765-1769) done on the device, so that one
 // 12-byte record per monomer crosses to the host.  One lane per sorted slot;
-// padding slots (id word ~0) write nothing.  The sum is taken modulo 2^32, as
-// the int32 result of the 64-bit sum.
+// padding slots (id word ~0) write nothing.  The coordinates are gathered
+// from the corner's site; the sum is taken modulo 2^32, as the int32 result
+// of the 64-bit sum.
 __global__ __launch_bounds__(kBlock) void k_unwrap(const uint32_t *__restrict__ ppos, const uint32_t *__restrict__ ids,
-                                                   const int32_t *__restrict__ image, uint32_t n_pad, uint32_t bx,
-                                                   uint32_t by, uint32_t bz, int32_t *__restrict__ xyz)
+                                                   const int32_t *__restrict__ image, uint32_t n_pad, Geometry g,
+                                                   uint32_t bx, uint32_t by, uint32_t bz, int32_t *__restrict__ xyz)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n_pad)
@@ -571,7 +525,7 @@ __global__ __launch_bounds__(kBlock) void k_unwrap(const uint32_t *__restrict__ 
     const uint32_t box[3] = {bx, by, bz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const uint32_t w = (p >> (kCoordBits * uint32_t(a))) & kCoordMask;
+        const uint32_t w = site_coord(p, g.dmask[a]);
         const uint32_t img = uint32_t(image[size_t(a) * n_pad + t]);
         xyz[3 * id + size_t(a)] = int32_t(w + img * box[a]);
     }
@@ -719,7 +673,7 @@ hipError_t launch_unwrap(const SweepBuffers &buf, const uint32_t box[3], int32_t
     if (buf.n_pad == 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_unwrap, dim3((buf.n_pad + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, buf.ppos, buf.ids,
-                       buf.image, buf.n_pad, box[0], box[1], box[2], xyz);
+                       buf.image, buf.n_pad, buf.geom, box[0], box[1], box[2], xyz);
     return hipGetLastError();
 }
 

@@ -12,7 +12,7 @@ constexpr uint32_t kStatSlots = 4096;          // spread of the accepted / propo
 
 // Everything one substep of one system needs on the device.
 struct SweepBuffers {
-    uint32_t *ppos;           // packed wrapped positions
+    uint32_t *ppos;           // lattice sites of the lower corners
     const uint32_t *ids;      // original id | partner count << kIdBits
     int32_t *image;           // [3][n_pad] periodic images
     const uint32_t *nbr;

@@ -281,9 +281,13 @@ void plan(HostSystem &s)
     make_geometry(s.box, s.periodic, s.geom, s.tables, s.lattice_words);
     if (s.n > kIdMask)
         throw std::runtime_error("too many monomers for the 28-bit monomer id");
+    // the forbidden mask in the kernels' order (bond_key: the three 3-bit
+    // differences interleaved as in a site index)
     for (int i = 0; i < 512; ++i)
-        if (s.forbidden[i])
-            s.tables.bond[i >> 5] |= 1u << (i & 31);
+        if (s.forbidden[i]) {
+            const uint32_t k = bond_key_of(i & 7, (i >> 3) & 7, (i >> 6) & 7);
+            s.tables.bond[k >> 5] |= 1u << (k & 31);
+        }
 
     // walls
     for (uint32_t i = 0; i < s.n; ++i)
@@ -424,7 +428,7 @@ void plan(HostSystem &s)
             w[a] = uint32_t(v - img * L);
             s.image_sorted[size_t(a) * s.n_pad + t] = int32_t(img);
         }
-        s.ppos_sorted[t] = pack_xyz(w[0], w[1], w[2]);
+        s.ppos_sorted[t] = host_site(s.geom, s.tables, int32_t(w[0]), int32_t(w[1]), int32_t(w[2]));
         s.ids_sorted[t] = i | (uint32_t(s.nnbr[i]) << kIdBits);
         for (uint32_t j = 0; j < s.nnbr[i]; ++j)
             s.nbr_sorted[size_t(j) * s.n_pad + t] = s.to_sorted[s.nbr[size_t(i) * kMaxConnectivity + j]];
@@ -447,7 +451,7 @@ void unpack_positions(HostSystem &s)
         const uint32_t t = s.to_sorted[i];
         const uint32_t p = s.ppos_sorted[t];
         for (int a = 0; a < 3; ++a) {
-            const int64_t w = (p >> (kCoordBits * uint32_t(a))) & kCoordMask;
+            const int64_t w = site_coord(p, s.geom.dmask[a]);
             const int64_t img = s.image_sorted[size_t(a) * s.n_pad + t];
             s.xyz[3 * size_t(i) + a] = int32_t(w + img * int64_t(s.box[a]));
         }

@@ -14,11 +14,11 @@ namespace {
 
 constexpr int kResortBlock = 256;
 
-// Site of the lower corner of every slot (padding slots included; they lie
-// outside the sorted ranges) and the identity permutation.
+// Sort key of every slot (padding slots included; they lie outside the
+// sorted ranges): the site of its lower corner, or its id; and the identity
+// permutation.
 __global__ __launch_bounds__(kResortBlock) void k_keys(const uint32_t *__restrict__ ppos,
-                                                       const uint32_t *__restrict__ ids,
-                                                       const Tables *__restrict__ tables, uint32_t n_pad,
+                                                       const uint32_t *__restrict__ ids, uint32_t n_pad,
                                                        uint32_t by_id, uint32_t *__restrict__ keys,
                                                        uint32_t *__restrict__ ident, uint32_t *__restrict__ perm)
 {
@@ -31,15 +31,7 @@ __global__ __launch_bounds__(kResortBlock) void k_keys(const uint32_t *__restric
         perm[t] = t;
         return;
     }
-    const int4 r = unpack_xyz(ppos[t]);
-    uint32_t site = 0;
-    const int32_t c[3] = {r.x, r.y, r.z};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const uint32_t v = uint32_t(c[a]);
-        site |= tables->dep[a][v & 31u] | tables->dep[a][32 + (v >> 5)];
-    }
-    keys[t] = site;
+    keys[t] = ppos[t];   // the corner's site: Morton order
     ident[t] = t;
     perm[t] = t;
 }
@@ -169,8 +161,8 @@ void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const S
     const uint32_t bits = by_id ? kIdBits : site_bits(s);
 
     const dim3 grid((n_pad + kResortBlock - 1) / kResortBlock), block(kResortBlock);
-    hipLaunchKernelGGL(k_keys, grid, block, 0, stream, buf.ppos, buf.ids, buf.tables, n_pad, by_id ? 1u : 0u, keys,
-                       ident, perm);
+    hipLaunchKernelGGL(k_keys, grid, block, 0, stream, buf.ppos, buf.ids, n_pad, by_id ? 1u : 0u, keys, ident,
+                       perm);
     hip_check(hipGetLastError(), "re-sort keys");
     for (uint32_t c = 0; c < s.n_colours; ++c) {
         const uint32_t b = s.seg_begin[c], n = s.seg_count[c];

@@ -1,9 +1,13 @@
 // Host checks of device helpers in bfm_device.hpp:
-//  * the 9-site front arithmetic (front_sites) against a direct enumeration
-//    with the lattice's site function: for every box, corner and direction
-//    tested, the at most 4 (word, mask) pairs must hold exactly the bits of
-//    the 9 front sites, and new_site / old_site must be the sites of the moved
-//    and the old corner (exhaustive for small boxes, random corners for large);
+//  * the 9-site front (front_of) against a direct
+//    enumeration with the lattice's site function: for every box, corner and
+//    direction tested, the 4 (word, mask) pairs must hold exactly the bits of
+//    the 9 front sites, and new_site must be the site of the moved corner
+//    (exhaustive for small boxes, random corners for large);
+//  * the wall and periodic-crossing tests on the moved corner's site;
+//  * bond_key (per-axis differences modulo 8 on the low 9 site bits) against
+//    the bond vector of the two corners;
+//  * site_coord (coordinates gathered from a site) against the coordinates;
 //  * colour_at (one entry of the per-MCS colour permutation, computed without
 //    the array) against the full shuffle colour_order.
 // Prints "ok <cases>" and exits 0 on success.
@@ -18,11 +22,6 @@ using namespace pscbfm;
 
 namespace {
 
-struct HostDepositTable {
-    const Tables *t;
-    uint32_t operator()(int a, uint32_t v) const { return t->dep[a][v & 31u] | t->dep[a][32 + (v >> 5)]; }
-};
-
 uint64_t g_state = 0x9E3779B97F4A7C15ull;
 uint32_t next_u32()
 {
@@ -32,11 +31,11 @@ uint32_t next_u32()
     return uint32_t(g_state >> 32);
 }
 
-template <class Dep>
-bool check_one(const uint64_t box[3], const Geometry &g, const Tables &t, const Dep &dep, int x, int y, int z, int dir)
+template <bool kCubic>
+bool check_one(const uint64_t box[3], const Geometry &g, const Tables &t, int x, int y, int z, int dir)
 {
-    const int4 r = make_int4(x, y, z, 0);
-    const Front f = front_sites(g, dep, r, dir);
+    const uint32_t site = host_site(g, t, x, y, z);
+    const Front f = front_of<kCubic>(g, site, dir);
     const int a = dir >> 1, s = (dir & 1) ? 1 : -1;
     int d[3] = {0, 0, 0};
     d[a] = s;
@@ -52,20 +51,51 @@ bool check_one(const uint64_t box[3], const Geometry &g, const Tables &t, const 
         }
     std::set<uint32_t> got;
     int bits = 0;
+    const uint32_t *m = f.m;
     for (int q = 0; q < 4; ++q)
         for (int bit = 0; bit < 32; ++bit)
-            if ((f.m[q] >> bit) & 1u) {
+            if ((m[q] >> bit) & 1u) {
                 got.insert((f.w[q] << 5) | uint32_t(bit));
                 ++bits;
             }
-    const uint32_t new_site = host_site(g, t, x + d[0], y + d[1], z + d[2]);
-    const uint32_t old_site = host_site(g, t, x, y, z);
-    if (got != want || bits != 9 || f.new_site != new_site || f.old_site != old_site) {
-        std::printf("FAIL box (%llu,%llu,%llu) r (%d,%d,%d) dir %d: %zu/%d bits, new %u/%u old %u/%u\n",
+    const int r1[3] = {x + d[0], y + d[1], z + d[2]};
+    const uint32_t new_site = host_site(g, t, r1[0], r1[1], r1[2]);
+    // the moved corner's coordinate on the move axis is L - 1 exactly when the
+    // move leaves [0, L - 2] (the wall test) and 0 / L - 1 when an upward /
+    // downward move crosses the periodic boundary
+    const int L = int(box[a]), ra = a == 0 ? x : (a == 1 ? y : z);
+    const uint32_t a1 = f.new_site & f.ma;
+    // (a corner on a walled axis lies in [0, L - 2])
+    const bool wall_hit = ra <= L - 2 && a1 == f.ma, want_wall = ra <= L - 2 && (ra + s < 0 || ra + s > L - 2);
+    const bool crossed = (dir & 1) ? a1 == 0u : a1 == f.ma, want_cross = ra + s < 0 || ra + s >= L;
+    if (got != want || bits != 9 || f.new_site != new_site || f.ma != g.dmask[a] || wall_hit != want_wall ||
+        crossed != want_cross) {
+        std::printf("FAIL box (%llu,%llu,%llu) r (%d,%d,%d) dir %d: %zu/%d bits, new %u/%u wall %d/%d cross %d/%d\n",
                     (unsigned long long)box[0], (unsigned long long)box[1], (unsigned long long)box[2], x, y, z,
-                    dir, got.size(), bits, f.new_site, new_site, f.old_site, old_site);
+                    dir, got.size(), bits, f.new_site, new_site, int(wall_hit), int(want_wall), int(crossed),
+                    int(want_cross));
         return false;
     }
+    // bonds from the moved corner to corners at every vector within reach
+    // (the key depends only on the vector modulo 8)
+    const uint32_t r1x = f.new_site & kLow9[0], r1y = f.new_site & kLow9[1], r1z = f.new_site & kLow9[2];
+    for (int k = 0; k < 4; ++k) {
+        const int v[3] = {int(next_u32() % 9) - 4, int(next_u32() % 9) - 4, int(next_u32() % 9) - 4};
+        const uint32_t q = host_site(g, t, r1[0] + v[0], r1[1] + v[1], r1[2] + v[2]);
+        if (bond_key(q, r1x, r1y, r1z) != bond_key_of(v[0], v[1], v[2])) {
+            std::printf("FAIL bond_key box (%llu,%llu,%llu) r1 (%d,%d,%d) v (%d,%d,%d)\n",
+                        (unsigned long long)box[0], (unsigned long long)box[1], (unsigned long long)box[2], r1[0],
+                        r1[1], r1[2], v[0], v[1], v[2]);
+            return false;
+        }
+    }
+    const int r[3] = {x, y, z};
+    for (int k = 0; k < 3; ++k)
+        if (site_coord(site, g.dmask[k]) != uint32_t(r[k])) {
+            std::printf("FAIL site_coord box (%llu,%llu,%llu) r (%d,%d,%d) axis %d\n", (unsigned long long)box[0],
+                        (unsigned long long)box[1], (unsigned long long)box[2], x, y, z, k);
+            return false;
+        }
     return true;
 }
 
@@ -113,8 +143,6 @@ int main(int argc, char **argv)
         Tables t{};
         uint64_t words = 0;
         make_geometry(box, periodic, g, t, words);
-        const HostDepositTable table{&t};
-        const DepositCubic cubic{};
         const uint64_t sites = box[0] * box[1] * box[2];
         const bool exhaustive = sites <= 16384;
         const long samples = exhaustive ? long(sites) : 20000;
@@ -130,9 +158,9 @@ int main(int argc, char **argv)
                 z = int(next_u32() % box[2]);
             }
             for (int dir = 0; dir < 6; ++dir) {
-                if (!check_one(box, g, t, table, x, y, z, dir))
+                if (!check_one<false>(box, g, t, x, y, z, dir))
                     return 1;
-                if (g.cubic && !check_one(box, g, t, cubic, x, y, z, dir))
+                if (g.cubic && !check_one<true>(box, g, t, x, y, z, dir))
                     return 1;
                 ++cases;
             }
