@@ -14,11 +14,13 @@
 // Differences in mechanism (same result):
 //  * 1 bit per site instead of 1 byte, in compact Morton order; the 9 front
 //    sites fall into at most 4 words;
+//  * a position is the site of its corner, moved and compared by dilated
+//    arithmetic on one axis's bits, so neither kernel spreads coordinates;
 //  * a survivor marks its moved corner with one atomic OR (no return value);
 //  * survivors are compacted per wavefront (wave64 ballot + mbcnt), so phase 2
 //    only runs over moves that can still happen (the reference's TODO,
-//    :1222-1228) and no global counter is contended; the only workgroup
-//    barrier is the one after the LDS tables are filled at kernel start;
+//    :1222-1228) and no global counter is contended; neither kernel uses LDS
+//    or a workgroup barrier;
 //  * no memset: the marks of a substep are cleared by the next check launch
 //    (the claim lattice is double-buffered by substep parity);
 //  * an accepted move writes its old and new lattice bytes with plain byte
@@ -27,9 +29,6 @@
 #include "bfm_kernels.hpp"
 
 #include <hip/hip_ext.h>
-
-#include <algorithm>
-#include <type_traits>
 
 namespace pscbfm {
 namespace {
