@@ -298,6 +298,9 @@ def main() -> int:
     ap.add_argument("--no-graphs", action="store_true",
                     help="batched leg without chunk graphs (kernel by kernel; the same kernels, for profilers)")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--sort-every", type=int, default=0,
+                    help="setStepsBetweenSortings of the plain leg (0 = the updater's default, 5000 as the "
+                         "reference's); trajectories do not depend on it, the generator being keyed on the id")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the plain copy kernel that calibrates HBM counters (0 = skip)")
     args = ap.parse_args()
@@ -331,7 +334,8 @@ def main() -> int:
         system = dict(system, xyz=xyz, bonds=perm[system["bonds"]])
     log(f"generated {args.workload}: {n} monomers in {time.perf_counter() - t:.1f} s")
     t = time.perf_counter()
-    u = updater_from_system(system, seed=w["seed"], gpu=0, spatial_sorting=args.layout == "morton")
+    u = updater_from_system(system, seed=w["seed"], gpu=0, spatial_sorting=args.layout == "morton",
+                            steps_between_sortings=args.sort_every or None)
     colors, n_colors = u.getColors(n)
     log(f"initialized in {time.perf_counter() - t:.1f} s, {n_colors} colours")
 
@@ -420,6 +424,7 @@ def main() -> int:
                  "relaxed for the equilibrate MCS before timing"),
         "config": {**workload_config(args.workload, w), "layout": args.layout, "monomers": n, "colours": n_colors,
                    "mcs": args.steps, "equilibrated_mcs": args.equilibrate,
+                   "steps_between_sortings": u.getStepsBetweenSortings(),
                    "p_acc": round(p_acc, 4), "p_proposal": round(p_prop, 4),
                    "rng": "int_hash (the reference's default generator)",
                    "launch": {"check_workgroup": tuned["check_block"], "perform_group": tuned["perform_groups"],
