@@ -6,7 +6,9 @@
 //    (exhaustive for small boxes, random corners for large);
 //  * the wall and periodic-crossing tests on the moved corner's site;
 //  * bond_key (per-axis differences modulo 8 on the low 9 site bits) against
-//    the bond vector of the two corners;
+//    the bond vector of the two corners, and the forbidden mask plan() builds in
+//    that order against the reference-order table (copyBondSet's index) for
+//    every vector in [-4,3]^3;
 //  * site_coord (coordinates gathered from a site) against the coordinates;
 //  * colour_at (one entry of the per-MCS colour permutation, computed without
 //    the array) against the full shuffle colour_order.
@@ -164,6 +166,37 @@ int main(int argc, char **argv)
                     return 1;
                 ++cases;
             }
+        }
+    }
+    {
+        // plan() on one free monomer: the device bond mask in bond_key order
+        HostSystem h;
+        h.box[0] = h.box[1] = h.box[2] = 16;
+        h.n = 1;
+        h.xyz = {3, 4, 5};
+        h.attribute = {0};
+        h.nnbr = {0};
+        h.nbr.assign(kMaxConnectivity, 0);
+        standard_bond_set(h.forbidden);
+        plan(h);
+        int allowed = 0;
+        for (int dx = -4; dx <= 3; ++dx)
+            for (int dy = -4; dy <= 3; ++dy)
+                for (int dz = -4; dz <= 3; ++dz) {
+                    const uint32_t k = bond_key_of(dx, dy, dz);
+                    const bool dev = (h.tables.bond[k >> 5] >> (k & 31u)) & 1u;
+                    const bool ref = h.forbidden[bond_table_index(dx, dy, dz)] != 0;
+                    if (dev != ref) {
+                        std::printf("FAIL bond mask (%d,%d,%d): device %d reference %d\n", dx, dy, dz, int(dev),
+                                    int(ref));
+                        return 1;
+                    }
+                    allowed += !dev;
+                    ++cases;
+                }
+        if (allowed != 108) {
+            std::printf("FAIL bond mask: %d allowed vectors\n", allowed);
+            return 1;
         }
     }
     for (uint32_t n = 1; n <= 40; ++n)
