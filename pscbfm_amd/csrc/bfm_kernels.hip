@@ -179,13 +179,16 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
             rn = int_hash(int_hash(id) ^ sub_seed);
         const int dir = int(rn % 6u);
 
-        // excluded volume: no corner in the 9-site front (all four words are
-        // read: each is a word of the lattice, and a repeated word has mask 0)
+        // excluded volume: no corner in the 9-site front.  All four words are
+        // read; a lane whose word q holds no front site (mask 0: the front
+        // stays within one word along b or c) reads word 0 instead, so that
+        // such lanes share one cache line rather than touching lines of their
+        // own (the vector L1's line accesses bound this kernel)
         const Front f = front_of<kCubic>(g, s0, dir);
         uint32_t l[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            l[q] = lat[f.w[q]];
+            l[q] = lat[f.m[q] != 0u ? f.w[q] : 0u];
 
         // bond partners belong to other colours and do not move this substep;
         // every site load is issued before the first is used (a lane with
@@ -378,7 +381,7 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
         f[r] = front_of_entry(e[r]);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            k4[r][q] = claim[f[r].w[q]];
+            k4[r][q] = claim[f[r].m[q] != 0u ? f[r].w[q] : 0u];   // shared line where unused
     }
 #pragma unroll
     for (uint32_t r = 0; r < kRounds; ++r) {
@@ -394,7 +397,7 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
         uint32_t c4[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            c4[q] = claim[f1.w[q]];
+            c4[q] = claim[f1.m[q] != 0u ? f1.w[q] : 0u];
         const bool go = !clashes(f1, c4);
         if (go)
             apply(e1, f1);
