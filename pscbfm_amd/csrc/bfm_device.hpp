@@ -180,7 +180,6 @@ __host__ __device__ inline uint32_t site_coord(uint32_t site, uint32_t dmask)
     return v;
 }
 
-__host__ __device__ __forceinline__ int32_t comp(const int4 &r, int a) { return a == 0 ? r.x : (a == 1 ? r.y : r.z); }
 // v[a] for a per-lane axis without indexing memory (kernel arguments live in
 // SGPRs; a dynamic index would turn them into scratch or vector loads).
 __host__ __device__ __forceinline__ uint32_t sel3(const uint32_t v[3], int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); }
