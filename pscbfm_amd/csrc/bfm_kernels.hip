@@ -153,11 +153,11 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
         const bool inside = t < count;
         const uint32_t i = begin + (inside ? t : count - 1u);
         // Two levels of dependent loads per wave: the slot's own words and
-        // its first kEarlyPartners partner indices, then the front words,
-        // the front masks and the partner sites.  Every load of the first two
-        // stages is unconditional, so the compiler knows how many are in
-        // flight and waits for exactly the ones each stage needs (a load under
-        // a branch makes it wait for all of them).  The partner table has
+        // its first kEarlyPartners partner indices, then the front words and
+        // the partner sites.  Every load of the first two stages is
+        // unconditional, so the compiler knows how many are in flight and
+        // waits for exactly the ones each stage needs (a load under a branch
+        // makes it wait for all of them).  The partner table has
         // kMaxConnectivity columns, zero beyond a slot's own partners, so
         // these reads stay in bounds; a lane uses only its first nn.
         const uint32_t idn = ids[i];
