@@ -62,6 +62,31 @@ def test_small_melt_1000_mcs(gpu_ready):
     assert_parity(s, seed=0x5EED000000000001, chunks=[1000])
 
 
+def test_small_melt_full_run_10000_mcs(gpu_ready):
+    # the small-melt configuration at its full length (10^4 MCS, 1.6e8
+    # attempts), in the drop-in's save intervals; the oracle's reference is
+    # its OpenMP colour sweep, equal to the serial replay (tests/test_oracle.py)
+    s = systems.linear_melt(64, 512, 32, seed=0x5EED000000000001)
+    n = len(s["xyz"])
+    seed = 0x5EED000000000001
+    print(f"case: small melt n={n} seed={seed:#x} 10 x 1000 MCS", flush=True)
+    u = updater_from_system(s, seed=seed, gpu=0)
+    for _ in range(10):
+        u.runSimulationOnGPU(1000)
+    got = u.getMonomerPositions(n)
+    colors, _ = u.getColors(n)
+    stats = u.getStats()
+    u.cleanup()
+    osys = orc.System(s, seed, colors)
+    want, acc, prop = orc.sweep_parallel(osys, s["xyz"], 0, 10000, 0)
+    diff = np.nonzero((got != want).any(axis=1))[0]
+    assert diff.size == 0, f"{diff.size} monomers differ, first {diff[:5]}"
+    assert stats["accepted"] == acc and stats["proposals"] == prop
+    assert stats["attempts"] == 10000 * n
+    rc, msg = orc.check_system(osys, got)
+    assert rc == 0, msg
+
+
 def test_round_robin_block_placement(gpu_ready):
     # the XCD placement changes which hardware runs a block, never the result
     s = systems.linear_melt(64, 512, 32, seed=21)
