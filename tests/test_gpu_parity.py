@@ -350,6 +350,31 @@ def test_network_full_size_20_mcs(gpu_ready):
     assert_parity(s, seed=0x5EED000000000003, chunks=[20])
 
 
+def test_network_full_run_1000_mcs(gpu_ready):
+    # the network configuration at its full length (10^3 MCS, 1.06e9
+    # attempts) in 100-MCS drop-in calls, against the oracle's OpenMP colour
+    # sweep (equal to the serial replay, tests/test_oracle.py)
+    s = systems.diamond_network(256, (8, 8, 32), 32, seed=0x5EED000000000003)
+    n = len(s["xyz"])
+    seed = 0x5EED000000000003
+    print(f"case: network n={n} seed={seed:#x} 10 x 100 MCS", flush=True)
+    u = updater_from_system(s, seed=seed, gpu=0)
+    for _ in range(10):
+        u.runSimulationOnGPU(100)
+    got = u.getMonomerPositions(n)
+    colors, nc = u.getColors(n)
+    stats = u.getStats()
+    u.cleanup()
+    assert nc == 2
+    osys = orc.System(s, seed, colors)
+    want, acc, prop = orc.sweep_parallel(osys, s["xyz"], 0, 1000, 0)
+    diff = np.nonzero((got != want).any(axis=1))[0]
+    assert diff.size == 0, f"{diff.size} monomers differ, first {diff[:5]}"
+    assert stats["accepted"] == acc and stats["proposals"] == prop
+    rc, msg = orc.check_system(osys, got)
+    assert rc == 0, msg
+
+
 @pytest.mark.parametrize("rng", [orc.RNG_INT_HASH, orc.RNG_PHILOX])
 def test_four_colour_cluster_gel(gpu_ready, rng):
     """Tetrahedral clusters (every pair of a cluster bonded: the bond graph
