@@ -300,3 +300,33 @@ def test_mixed_batch_full_size_matches_plain(gpu_ready):
             bad.append(m)
         u.cleanup()
     assert not bad, f"{len(bad)} of {len(ids)} members differ from their plain runs"
+
+
+def test_mixed_batch_full_run_1000_mcs(gpu_ready):
+    """The 512-system mixed workload at its full size and length (20.3 M monomers, 10^3 MCS
+    each, chunks of 100 MCS, 2.0e10 attempts): every member ends where it ends alone, and
+    the three smallest members where the oracle's replay ends."""
+    from pscbfm_amd import workloads
+    specs = workloads.mixed_batch(512, seed=0xBA7C4)
+    built = [workloads.build(sp) for sp in specs]
+    print(f"case: mixed batch at full size, {len(built)} systems, "
+          f"{sum(len(x['xyz']) for x in built)} monomers, 1000 MCS in chunks of 100", flush=True)
+    b = Batch(0)
+    ids = [b.add(updater_from_system(x, seed=sp["seed"], validate_only=True), len(x["xyz"]))
+           for x, sp in zip(built, specs)]
+    b.run(1000, 100)
+    b.synchronize()
+    bad = []
+    for x, sp, m in zip(built, specs, ids):
+        u = updater_from_system(x, seed=sp["seed"], gpu=0)
+        u.runResident(1000)
+        u.copyBack()
+        if not np.array_equal(b.positions(m), u.getMonomerPositions(len(x["xyz"]))):
+            bad.append(m)
+        u.cleanup()
+    assert not bad, f"{len(bad)} of {len(ids)} members differ from their plain runs"
+    for j in sorted(range(len(built)), key=lambda j: len(built[j]["xyz"]))[:3]:
+        x, sp = built[j], specs[j]
+        print(f"case: member {j} ({len(x['xyz'])} monomers) against the oracle", flush=True)
+        want, _, _ = orc.sweep(orc.System(x, sp["seed"]), x["xyz"], 0, 1000, 0)
+        assert np.array_equal(b.positions(ids[j]), want), f"member {j} differs from the oracle"
