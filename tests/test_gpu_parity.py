@@ -306,6 +306,32 @@ def test_large_melt_full_size_10_mcs(gpu_ready):
     assert_parity(s, seed=0x5EED000000000002, chunks=[10])
 
 
+def test_large_melt_full_run_1000_mcs(gpu_ready):
+    # the large-melt configuration at its full length (10^3 MCS, 8.4e9
+    # attempts) in 100-MCS drop-in calls, against the oracle's OpenMP colour
+    # sweep (equal to the serial replay, tests/test_oracle.py); the oracle
+    # takes about 45 s on 16 host threads
+    s = systems.linear_melt(512, 16384, 512, seed=0x5EED000000000002)
+    n = len(s["xyz"])
+    seed = 0x5EED000000000002
+    print(f"case: large melt n={n} seed={seed:#x} 10 x 100 MCS", flush=True)
+    u = updater_from_system(s, seed=seed, gpu=0)
+    for _ in range(10):
+        u.runSimulationOnGPU(100)
+    got = u.getMonomerPositions(n)
+    colors, _ = u.getColors(n)
+    stats = u.getStats()
+    u.cleanup()
+    print("running the oracle's OpenMP sweep for 1000 MCS", flush=True)
+    osys = orc.System(s, seed, colors)
+    want, acc, prop = orc.sweep_parallel(osys, s["xyz"], 0, 1000, 0)
+    diff = np.nonzero((got != want).any(axis=1))[0]
+    assert diff.size == 0, f"{diff.size} monomers differ, first {diff[:5]}"
+    assert stats["accepted"] == acc and stats["proposals"] == prop
+    rc, msg = orc.check_system(osys, got)
+    assert rc == 0, msg
+
+
 def test_network_three_colours(gpu_ready):
     s = systems.diamond_network(64, (2, 2, 8), 32, seed=41)
     st = assert_parity(s, seed=42, chunks=[150, 150])
