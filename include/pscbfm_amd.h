@@ -334,6 +334,14 @@ int bfm_batch_observe(bfm_batch *b, uint32_t member, bfm_observables *out);
  * and its host arrays; its id is not reused. */
 int bfm_batch_remove(bfm_batch *b, uint32_t member);
 int bfm_batch_set_graphs(bfm_batch *b, int on);
+/* Working-set budget of bfm_batch_run in bytes (no counterpart in the
+ * reference, which runs one system): where the members' hot arrays exceed it,
+ * bfm_batch_run splits them into groups of about that size and runs each
+ * group's n_mcs in turn, so that a group stays in the memory-side cache; the
+ * trajectories are unchanged.  0 never splits; the default is 128 MiB. */
+int bfm_batch_set_working_set_bytes(bfm_batch *b, uint64_t bytes);
+/* Number of groups bfm_batch_run would use now. */
+int bfm_batch_get_working_set_groups(bfm_batch *b, uint32_t *groups);
 /* As bfm_set_xcd_ranges, over the batch's whole grid (each member's blocks
  * stay on one or two XCDs). */
 int bfm_batch_set_xcd_ranges(bfm_batch *b, int on);

@@ -170,6 +170,8 @@ SIGNATURES = {
     "bfm_batch_remove": (c_int, [_H, c_uint32]),
     "bfm_batch_set_graphs": (c_int, [_H, c_int]),
     "bfm_batch_set_xcd_ranges": (c_int, [_H, c_int]),
+    "bfm_batch_set_working_set_bytes": (c_int, [_H, c_uint64]),
+    "bfm_batch_get_working_set_groups": (c_int, [_H, POINTER(c_uint32)]),
     "bfm_batch_synchronize": (c_int, [_H]),
     "bfm_server_create": (c_int, [POINTER(c_void_p), c_int, c_uint64, c_uint32]),
     "bfm_server_destroy": (c_int, [_H]),

@@ -91,6 +91,16 @@ class Batch:
     def set_graphs(self, on: bool) -> None:
         _capi.check(self._lib.bfm_batch_set_graphs(self._h, int(bool(on))))
 
+    def set_working_set_bytes(self, nbytes: int) -> None:
+        """Working-set budget of run() (0: never split the members into groups)."""
+        _capi.check(self._lib.bfm_batch_set_working_set_bytes(self._h, int(nbytes)))
+
+    def working_set_groups(self) -> int:
+        """Number of member groups run() would use now."""
+        g = c_uint32()
+        _capi.check(self._lib.bfm_batch_get_working_set_groups(self._h, byref(g)))
+        return g.value
+
     def set_xcd_ranges(self, on: bool) -> None:
         _capi.check(self._lib.bfm_batch_set_xcd_ranges(self._h, int(bool(on))))
 

@@ -241,6 +241,8 @@ uint32_t BatchEngine::add(HostSystem &&sys, uint64_t seed, int64_t age, bool kee
     const SlabLayout L = slab_layout(m->sys);
     m->slab = takeSlab(L.total, m->slab_class);
     m->bytes = L.total;
+    m->hot_bytes = L.bytes[kPartPPos] + L.bytes[kPartIds] + (L.upload - L.offset[kPartNbr]) +
+                   L.bytes[kPartLattice] + L.bytes[kPartClaim0] + L.bytes[kPartClaim1];
     try {
         // staged through pinned memory, so that admitting a member never
         // waits for the chunks already queued
@@ -670,13 +672,47 @@ uint32_t BatchEngine::sortDue(uint32_t limit)
     return until;
 }
 
+std::vector<std::vector<uint32_t>> BatchEngine::workingSetGroups(const std::vector<uint32_t> &ids) const
+{
+    size_t hot = 0;
+    uint64_t monomers = 0;
+    for (uint32_t id : ids) {
+        hot += member(id).hot_bytes;
+        monomers += member(id).sys.n;
+    }
+    uint64_t groups = 1;
+    if (mWorkingSetBytes > 0 && hot > mWorkingSetBytes) {
+        groups = (hot + mWorkingSetBytes - 1) / mWorkingSetBytes;
+        groups = std::min<uint64_t>(groups, std::max<uint64_t>(1, monomers / kMinGroupMonomers));
+        groups = std::min<uint64_t>(groups, ids.size());
+    }
+    if (groups <= 1)
+        return {ids};
+    // largest members first, each into the group with the fewest hot bytes;
+    // every group keeps the members' order
+    std::vector<uint32_t> order(ids);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return member(a).hot_bytes > member(b).hot_bytes; });
+    std::vector<std::vector<uint32_t>> out(groups);
+    std::vector<size_t> load(groups, 0);
+    for (uint32_t id : order) {
+        const size_t g = size_t(std::min_element(load.begin(), load.end()) - load.begin());
+        out[g].push_back(id);
+        load[g] += member(id).hot_bytes;
+    }
+    for (auto &g : out)
+        std::sort(g.begin(), g.end());
+    return out;
+}
+
 void BatchEngine::run(uint32_t n_mcs, uint32_t chunk)
 {
     std::vector<uint32_t> ids;
     for (uint32_t i = 0; i < mMembers.size(); ++i)
         if (active(i))
             ids.push_back(i);
-    runMembers(ids, n_mcs, chunk);
+    for (const auto &g : workingSetGroups(ids))
+        runMembers(g, n_mcs, chunk);
 }
 
 void BatchEngine::copyBack(uint32_t id)

@@ -59,6 +59,15 @@ private:
     std::vector<Buf> mBufs;
 };
 
+// Working-set budget of BatchEngine::run: half of the MI355X's 256 MB
+// memory-side cache, measured best for the 512-system mixed batch (its 0.48 GB
+// of hot arrays in 4 groups: 7.8e10 against 6.9e10 attempts/s in one, 6.9e10
+// in 2 groups, 6.4e10 in 8, where the launches get small).
+constexpr size_t kDefaultWorkingSetBytes = size_t(128) << 20;
+// A group holds at least this many monomers, so that its launches fill the
+// device (2 M monomers: 4,096 check waves per colour of 1 M).
+constexpr uint64_t kMinGroupMonomers = 2u << 20;
+
 class BatchEngine {
 public:
     explicit BatchEngine(int gpu);
@@ -78,8 +87,20 @@ public:
     // Device bytes a planned system would take as a member.
     static size_t bytesFor(const HostSystem &sys);
 
-    // Runs n_mcs MCS on every active member, in chunks of `chunk` MCS.
+    // Runs n_mcs MCS on every active member, in chunks of `chunk` MCS.  Where
+    // the members' hot arrays (sites, ids, partner columns, lattices and
+    // claim lattices) exceed the working-set budget, the members are split
+    // into groups of about that many hot bytes (each of at least
+    // kMinGroupMonomers monomers, to fill the device) and every group runs its
+    // n_mcs before the next starts: a group's arrays then stay in the
+    // memory-side cache (MALL) across its MCS.  Members are independent, so
+    // every trajectory is the same as without groups.
     void run(uint32_t n_mcs, uint32_t chunk);
+    // Working-set budget of run() in bytes (0: never split).
+    void setWorkingSetBytes(size_t bytes) { mWorkingSetBytes = bytes; }
+    size_t workingSetBytes() const { return mWorkingSetBytes; }
+    // The groups run() would use for these members.
+    std::vector<std::vector<uint32_t>> workingSetGroups(const std::vector<uint32_t> &ids) const;
     // Runs n_mcs MCS on the members listed (others idle).
     void runMembers(const std::vector<uint32_t> &ids, uint32_t n_mcs, uint32_t chunk);
 
@@ -166,6 +187,7 @@ private:
         uint64_t sorts = 0;
         BatchSystem dev{};
         uint32_t check_blocks_max = 0;
+        size_t hot_bytes = 0;    // arrays every MCS streams: sites, ids, partner columns, lattice, claims
         std::unique_ptr<DeviceAnalyzer> analyzer;   // made by the first observe()
     };
 
@@ -190,6 +212,7 @@ private:
     void ensureDevice(T *&p, size_t &cap, size_t n, const char *what);
 
     int mGpu;
+    size_t mWorkingSetBytes = kDefaultWorkingSetBytes;
     hipStream_t mStream = nullptr;
     PinnedPool mPinned;   // staging of member uploads and downloads
     std::vector<std::unique_ptr<Member>> mMembers;

@@ -730,6 +730,25 @@ int bfm_batch_set_graphs(bfm_batch *b, int on)
     return guarded([&] { b->e->setUseGraphs(on != 0); });
 }
 
+int bfm_batch_set_working_set_bytes(bfm_batch *b, uint64_t bytes)
+{
+    NEED(b, "batch handle");
+    return guarded([&] { b->e->setWorkingSetBytes(size_t(bytes)); });
+}
+
+int bfm_batch_get_working_set_groups(bfm_batch *b, uint32_t *groups)
+{
+    NEED(b, "batch handle");
+    NEED(groups, "groups");
+    return guarded([&] {
+        std::vector<uint32_t> ids;
+        for (uint32_t i = 0; i < b->e->size(); ++i)
+            if (b->e->active(i))
+                ids.push_back(i);
+        *groups = uint32_t(b->e->workingSetGroups(ids).size());
+    });
+}
+
 int bfm_batch_set_xcd_ranges(bfm_batch *b, int on)
 {
     NEED(b, "batch handle");

@@ -304,8 +304,9 @@ def test_mixed_batch_full_size_matches_plain(gpu_ready):
 
 def test_mixed_batch_full_run_1000_mcs(gpu_ready):
     """The 512-system mixed workload at its full size and length (20.3 M monomers, 10^3 MCS
-    each, chunks of 100 MCS, 2.0e10 attempts): every member ends where it ends alone, and
-    the three smallest members where the oracle's replay ends."""
+    each, chunks of 100 MCS, 2.0e10 attempts; the members in working-set groups): every
+    member ends where it ends alone, and the three smallest members where the oracle's
+    replay ends."""
     from pscbfm_amd import workloads
     specs = workloads.mixed_batch(512, seed=0xBA7C4)
     built = [workloads.build(sp) for sp in specs]
@@ -314,6 +315,9 @@ def test_mixed_batch_full_run_1000_mcs(gpu_ready):
     b = Batch(0)
     ids = [b.add(updater_from_system(x, seed=sp["seed"], validate_only=True), len(x["xyz"]))
            for x, sp in zip(built, specs)]
+    # about 0.48 GB of hot arrays: run() sweeps 4 groups of members in turn,
+    # each for all 1000 MCS
+    assert b.working_set_groups() == 4
     b.run(1000, 100)
     b.synchronize()
     bad = []
