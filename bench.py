@@ -169,7 +169,7 @@ def batched_leg(args) -> dict:
     check = f"{exact}/{len(built)} states bit-exact vs plain" if plain else "no plain comparison"
     log(f"batched: {args.batch_mcs} MCS in {t_batched:.2f} s -> {rate:.4g} attempts/s, {check}")
     return {"systems": len(built), "composition": composition, "monomers": n_tot, "mcs": args.batch_mcs,
-            "chunk": args.chunk, "graphs": not args.no_graphs,
+            "chunk": args.chunk, "graphs": not args.no_graphs, "working_set_groups": b.working_set_groups(),
             "value": rate, "unit": "attempts/s", "plain_value": plain_rate, "plain_mcs": args.plain_mcs,
             "speedup_vs_plain": rate / plain_rate if plain_rate else None, "bit_exact_systems": exact}
 
