@@ -147,6 +147,8 @@ def batched_leg(args) -> dict:
 
     b = Batch(0)
     b.set_graphs(not args.no_graphs)
+    if args.working_set_mib >= 0:
+        b.set_working_set_bytes(args.working_set_mib << 20)
     ids = [b.add(updater_from_system(x, seed=sp["seed"], validate_only=True), len(x["xyz"]))
            for x, sp in zip(built, specs)]
     b.synchronize()
@@ -169,7 +171,9 @@ def batched_leg(args) -> dict:
     check = f"{exact}/{len(built)} states bit-exact vs plain" if plain else "no plain comparison"
     log(f"batched: {args.batch_mcs} MCS in {t_batched:.2f} s -> {rate:.4g} attempts/s, {check}")
     return {"systems": len(built), "composition": composition, "monomers": n_tot, "mcs": args.batch_mcs,
-            "chunk": args.chunk, "graphs": not args.no_graphs, "working_set_groups": b.working_set_groups(),
+            "chunk": args.chunk, "graphs": not args.no_graphs,
+            "working_set_mib": 128 if args.working_set_mib < 0 else args.working_set_mib,
+            "working_set_groups": b.working_set_groups(),
             "value": rate, "unit": "attempts/s", "plain_value": plain_rate, "plain_mcs": args.plain_mcs,
             "speedup_vs_plain": rate / plain_rate if plain_rate else None, "bit_exact_systems": exact}
 
@@ -301,6 +305,11 @@ def main() -> int:
     ap.add_argument("--sort-every", type=int, default=0,
                     help="setStepsBetweenSortings of the plain leg (0 = the updater's default, 5000 as the "
                          "reference's); trajectories do not depend on it, the generator being keyed on the id")
+    ap.add_argument("--served-capacity", type=float, default=0.0,
+                    help="attempts/s that the served load factors scale (0 = the batched leg's rate)")
+    ap.add_argument("--working-set-mib", type=int, default=-1,
+                    help="working-set budget of the batched leg in MiB (-1 = the engine's "
+                         "default of 128, 0 = never split the systems into groups)")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the plain copy kernel that calibrates HBM counters (0 = skip)")
     args = ap.parse_args()
@@ -452,7 +461,7 @@ def main() -> int:
         if args.batch_uniform:
             log("served leg skipped: its jobs follow the mixed distribution, not --batch-uniform")
         elif not args.no_served:
-            out["served"] = served_leg(args, out["batched"]["value"])
+            out["served"] = served_leg(args, args.served_capacity or out["batched"]["value"])
     if not args.no_cpu:
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(system, w["seed"], args.cpu_seconds)

@@ -80,7 +80,6 @@ BatchEngine::BatchEngine(int gpu) : mGpu(gpu)
         throw std::invalid_argument("batch: GPU id out of range");
     hip_check(hipSetDevice(gpu), "hipSetDevice");
     hip_check(hipStreamCreate(&mStream), "hipStreamCreate");
-    hip_check(hipEventCreateWithFlags(&mTablesCopied, hipEventDisableTiming), "hipEventCreate");
     // members come and go: allocate their slabs in stream order from the
     // device's pool and keep freed memory in the pool for the next ones
     hipMemPool_t pool = nullptr;
@@ -123,8 +122,6 @@ BatchEngine::~BatchEngine()
         if (mDevPerformBlk[c])
             hipFree(mDevPerformBlk[c]);
     }
-    if (mTablesCopied)
-        hipEventDestroy(mTablesCopied);
     if (mStream)
         hipStreamDestroy(mStream);
 }
@@ -437,10 +434,9 @@ static void lay_out_blocks(const std::vector<BlockEntry> &real, uint32_t cap, bo
 
 void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
 {
-    // Launches already queued still read the device tables, but the copies
-    // below are queued behind them on the same stream; only the host staging
-    // vectors of the previous rebuild must have been copied before reuse.
-    hip_check(hipEventSynchronize(mTablesCopied), "hipEventSynchronize (tables)");
+    // Launches already queued still read the device tables; the copies below
+    // are queued behind them on the same stream, from pinned staging, so the
+    // host never waits here for the device.
     std::vector<BatchSystem> &sys = mHostSys;
     std::vector<BlockEntry> real_c[kClasses], real_p[kClasses];
     sys.clear();
@@ -485,8 +481,7 @@ void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
     idle.n_colours = 0;
     sys.resize(mSysCapN, idle);
     ensureDevice(mDevSys, mSysCap, sys.size(), "allocation of the batch systems");
-    hip_check(hipMemcpyAsync(mDevSys, sys.data(), sizeof(BatchSystem) * sys.size(), hipMemcpyHostToDevice, mStream),
-              "upload batch systems");
+    size_t bytes = sizeof(BatchSystem) * sys.size();
     for (uint32_t c = 0; c < kClasses; ++c) {
         if (!mCheckCapN[c])
             continue;
@@ -494,14 +489,25 @@ void BatchEngine::rebuildTables(const std::vector<uint32_t> &ids)
         lay_out_blocks(real_p[c], mPerformCapN[c], mXcdRanges, mHostPerformBlk[c]);
         ensureDevice(mDevCheckBlk[c], mCheckCap[c], mCheckCapN[c], "allocation of the batch blocks");
         ensureDevice(mDevPerformBlk[c], mPerformCap[c], mPerformCapN[c], "allocation of the batch blocks");
-        hip_check(hipMemcpyAsync(mDevCheckBlk[c], mHostCheckBlk[c].data(), sizeof(BlockEntry) * mCheckCapN[c],
-                                 hipMemcpyHostToDevice, mStream),
-                  "upload batch blocks");
-        hip_check(hipMemcpyAsync(mDevPerformBlk[c], mHostPerformBlk[c].data(), sizeof(BlockEntry) * mPerformCapN[c],
-                                 hipMemcpyHostToDevice, mStream),
-                  "upload batch blocks");
+        bytes += sizeof(BlockEntry) * (size_t(mCheckCapN[c]) + mPerformCapN[c]);
     }
-    hip_check(hipEventRecord(mTablesCopied, mStream), "hipEventRecord");
+    const size_t k = mPinned.take(bytes);
+    char *at = static_cast<char *>(mPinned.data(k));
+    auto stage = [&](void *dst, const void *src, size_t n, const char *what) {
+        std::memcpy(at, src, n);
+        hip_check(hipMemcpyAsync(dst, at, n, hipMemcpyHostToDevice, mStream), what);
+        at += n;
+    };
+    stage(mDevSys, sys.data(), sizeof(BatchSystem) * sys.size(), "upload batch systems");
+    for (uint32_t c = 0; c < kClasses; ++c) {
+        if (!mCheckCapN[c])
+            continue;
+        stage(mDevCheckBlk[c], mHostCheckBlk[c].data(), sizeof(BlockEntry) * mCheckCapN[c], "upload batch blocks");
+        stage(mDevPerformBlk[c], mHostPerformBlk[c].data(), sizeof(BlockEntry) * mPerformCapN[c],
+              "upload batch blocks");
+    }
+    mPinned.record(k, mStream);
+    mPinned.give(k);
     mCompIds = ids;
 }
 

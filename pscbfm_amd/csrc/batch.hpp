@@ -221,10 +221,10 @@ private:
     // tables of the current composition
     std::vector<uint32_t> mCompIds;          // member ids in launch order
     bool mPooled = false;                    // stream-ordered allocation available
-    // host copies of the tables, kept until their upload has run
+    // host copies of the tables, built here and staged through pinned memory
+    // (a copy from pageable memory would wait for the stream to drain)
     std::vector<BatchSystem> mHostSys;
     std::vector<BlockEntry> mHostCheckBlk[kClasses], mHostPerformBlk[kClasses];
-    hipEvent_t mTablesCopied = nullptr;
     BatchSystem *mDevSys = nullptr;
     size_t mSysCap = 0;          // allocated entries
     uint32_t mMaxColours = 0;

@@ -122,9 +122,34 @@ struct WaveBondMask {
     // key: bond_key() of the bond vector (a 9-bit index)
     __device__ __forceinline__ bool forbidden(uint32_t key) const
     {
-        return (uint32_t(__shfl(int(word), int(key >> 5))) >> (key & 31u)) & 1u;
+        // lane key >> 5 (below 16) holds the word; ds_bpermute takes the
+        // source lane as a byte address
+        return (uint32_t(__builtin_amdgcn_ds_bpermute(int((key >> 5) << 2), int(word))) >> (key & 31u)) & 1u;
     }
 };
+
+// A raw buffer resource over `words` 32-bit words at p.  dword 3 of the
+// descriptor selects the 32-bit data format and nothing else.  A load through
+// it whose byte offset lies at or past the end reads zero and makes no memory
+// access (the resource's range check), so a lane can drop out of a load
+// without a branch.
+constexpr int kRawBufferWord3 = 0x00020000;
+constexpr uint32_t kPastEnd = 0xFFFFFFFCu;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t word_rsrc(const uint32_t *p, uint32_t words)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(p), short(0), int(words * 4u), kRawBufferWord3);
+}
+// Word `word` of the resource (an offset below 2^32 bytes: 32-bit address
+// arithmetic per lane).
+__device__ __forceinline__ uint32_t load_word(__amdgpu_buffer_rsrc_t r, uint32_t word)
+{
+    return uint32_t(__builtin_amdgcn_raw_buffer_load_b32(r, int(word * 4u), 0, 0));
+}
+// Word `word` of the resource where `use`, else zero without an access.
+__device__ __forceinline__ uint32_t load_or_zero(__amdgpu_buffer_rsrc_t r, bool use, uint32_t word)
+{
+    return uint32_t(__builtin_amdgcn_raw_buffer_load_b32(r, int(use ? word * 4u : kPastEnd), 0, 0));
+}
 
 // Phase 1 for check block `blk` of `nblk` covering the colour segment
 // [begin, begin+count).  Every wavefront compacts its own survivors into surv.
@@ -135,10 +160,11 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
     const Geometry &g = buf.geom;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t gwave = wave_uniform(blk * (kThreads / kWave) + (threadIdx.x >> 6));
-    const gptr<const uint32_t> ppos = as_global<const uint32_t>(buf.ppos);
-    const gptr<const uint32_t> ids = as_global(buf.ids);
-    const gptr<const uint32_t> nbr = as_global(buf.nbr);
-    const gptr<const uint32_t> lat = as_global<const uint32_t>(buf.lattice);
+    // the arrays read per lane, as buffer resources: 32-bit offsets
+    const __amdgpu_buffer_rsrc_t ppos = word_rsrc(buf.ppos, buf.n_pad);
+    const __amdgpu_buffer_rsrc_t ids = word_rsrc(buf.ids, buf.n_pad);
+    const __amdgpu_buffer_rsrc_t lat_rsrc = word_rsrc(buf.lattice, buf.lattice_words);
+    auto nbr_column = [&](int j) { return word_rsrc(buf.nbr + size_t(j) * buf.n_pad, buf.n_pad); };
     const gptr<uint32_t> claim_cur = as_global(pick(parity, buf.claim0, buf.claim1));
     const gptr<uint32_t> claim_prev = as_global(pick(parity, buf.claim1, buf.claim0));
 
@@ -160,12 +186,12 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
         // makes it wait for all of them).  The partner table has
         // kMaxConnectivity columns, zero beyond a slot's own partners, so
         // these reads stay in bounds; a lane uses only its first nn.
-        const uint32_t idn = ids[i];
-        const uint32_t s0 = ppos[i];
+        const uint32_t idn = load_word(ids, i);
+        const uint32_t s0 = load_word(ppos, i);
         uint32_t partner[kMaxConnectivity];
 #pragma unroll
         for (int j = 0; j < kEarlyPartners; ++j)
-            partner[j] = nbr[size_t(j) * buf.n_pad + i];
+            partner[j] = load_word(nbr_column(j), i);
         const uint32_t id = idn & kIdMask, nn = idn >> kIdBits;
         // a lane of the wave with partners beyond the early columns (a
         // network's crosslinker)
@@ -179,39 +205,23 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
             rn = int_hash(int_hash(id) ^ sub_seed);
         const int dir = int(rn % 6u);
 
-        // excluded volume: no corner in the 9-site front.  All four words are
-        // read; a lane whose word q holds no front site (mask 0: the front
-        // stays within one word along b or c) reads word 0 instead, so that
-        // such lanes share one cache line rather than touching lines of their
-        // own (the vector L1's line accesses bound this kernel)
+        // excluded volume: no corner in the 9-site front.  Only the words
+        // that hold front sites are read (mask non-zero); the other lanes give
+        // their load an offset past the lattice's end, which reads zero
+        // without an access, so the loads stay unconditional
         const Front f = front_of<kCubic>(g, s0, dir);
         uint32_t l[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            l[q] = lat[f.m[q] != 0u ? f.w[q] : 0u];
+            l[q] = load_or_zero(lat_rsrc, f.m[q] != 0u, f.w[q]);
 
         // bond partners belong to other colours and do not move this substep;
         // every site load is issued before the first is used (a lane with
         // fewer partners reads its own slot instead)
-        uint32_t qpos[kMaxConnectivity];
+        uint32_t qpos[kEarlyPartners];
 #pragma unroll
         for (int j = 0; j < kEarlyPartners; ++j)
-            qpos[j] = ppos[uint32_t(j) < nn ? partner[j] : i];
-        uint32_t nn_wave = uint32_t(kEarlyPartners);
-        if (more) {
-            // the rest of the partners, after the counts
-#pragma unroll
-            for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
-                nn_wave = __ballot(nn > uint32_t(j)) ? uint32_t(j + 1) : nn_wave;
-#pragma unroll
-            for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
-                if (uint32_t(j) < nn_wave)
-                    partner[j] = nbr[size_t(j) * buf.n_pad + i];
-#pragma unroll
-            for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
-                if (uint32_t(j) < nn_wave)
-                    qpos[j] = ppos[uint32_t(j) < nn ? partner[j] : i];
-        }
+            qpos[j] = load_word(ppos, uint32_t(j) < nn ? partner[j] : i);
 
         bool ok = true;
         // walls: a move along a walled axis may not leave [0, L - 2]; the
@@ -229,13 +239,30 @@ __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t beg
             if (uint32_t(j) < nn && bad)
                 ok = false;
         }
+        if (more) {
+            // the rest of the partners, after the counts (their sites live
+            // only in this branch, so waves without such lanes carry none)
+            uint32_t nn_wave = uint32_t(kEarlyPartners);
 #pragma unroll
-        for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
-            if (uint32_t(j) < nn_wave) {
-                const bool bad = bonds.forbidden(bond_key(qpos[j], r1x, r1y, r1z));
-                if (uint32_t(j) < nn && bad)
-                    ok = false;
-            }
+            for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
+                nn_wave = __ballot(nn > uint32_t(j)) ? uint32_t(j + 1) : nn_wave;
+            uint32_t q_more[kMaxConnectivity - kEarlyPartners];
+#pragma unroll
+            for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
+                if (uint32_t(j) < nn_wave)
+                    partner[j] = load_word(nbr_column(j), i);
+#pragma unroll
+            for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
+                if (uint32_t(j) < nn_wave)
+                    q_more[j - kEarlyPartners] = load_word(ppos, uint32_t(j) < nn ? partner[j] : i);
+#pragma unroll
+            for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
+                if (uint32_t(j) < nn_wave) {
+                    const bool bad = bonds.forbidden(bond_key(q_more[j - kEarlyPartners], r1x, r1y, r1z));
+                    if (uint32_t(j) < nn && bad)
+                        ok = false;
+                }
+        }
         if ((l[0] & f.m[0]) | (l[1] & f.m[1]) | (l[2] & f.m[2]) | (l[3] & f.m[3]))
             ok = false;
 
@@ -286,7 +313,7 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t pwave = wave_uniform(blk * kWavesPerBlock + (threadIdx.x >> 6));
     const uint32_t cw0 = pwave * kGroup;
-    const gptr<const uint32_t> claim = as_global<const uint32_t>(pick(parity, buf.claim0, buf.claim1));
+    const __amdgpu_buffer_rsrc_t claim = word_rsrc(pick(parity, buf.claim0, buf.claim1), buf.lattice_words);
     const gptr<const uint2> surv = as_global<const uint2>(buf.surv);
     const gptr<const uint32_t> scount = as_global<const uint32_t>(buf.scount);
     const gptr<uint32_t> lattice = as_global(buf.lattice);
@@ -381,7 +408,7 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
         f[r] = front_of_entry(e[r]);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            k4[r][q] = claim[f[r].m[q] != 0u ? f[r].w[q] : 0u];   // shared line where unused
+            k4[r][q] = load_or_zero(claim, live[r] && f[r].m[q] != 0u, f[r].w[q]);   // no access where unused
     }
 #pragma unroll
     for (uint32_t r = 0; r < kRounds; ++r) {
@@ -397,7 +424,7 @@ __device__ __forceinline__ void perform_body(const SweepBuffers &buf, uint32_t c
         uint32_t c4[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            c4[q] = claim[f1.m[q] != 0u ? f1.w[q] : 0u];
+            c4[q] = load_or_zero(claim, f1.m[q] != 0u, f1.w[q]);
         const bool go = !clashes(f1, c4);
         if (go)
             apply(e1, f1);
