@@ -370,7 +370,8 @@ def main() -> int:
     else:
         cfg = [u.launchConfig(c) for c in range(n_colors)]
         tuned = {"perform_group": cfg[0]["perform_group"], "xcd_ranges": cfg[0]["xcd_ranges"],
-                 "check_block": [c["check_block"] for c in cfg], "perform_groups": [c["perform_group"] for c in cfg],
+                 "check_block": [c["check_block"] for c in cfg], "check_items": [c["check_items"] for c in cfg],
+                 "perform_groups": [c["perform_group"] for c in cfg],
                  "morton_order": cfg[0]["morton_order"]}
     log(f"launch configuration: {tuned}")
     u.setKernelTiming(every)
@@ -436,7 +437,8 @@ def main() -> int:
                    "steps_between_sortings": u.getStepsBetweenSortings(),
                    "p_acc": round(p_acc, 4), "p_proposal": round(p_prop, 4),
                    "rng": "int_hash (the reference's default generator)",
-                   "launch": {"check_workgroup": tuned["check_block"], "perform_group": tuned["perform_groups"],
+                   "launch": {"check_workgroup": tuned["check_block"], "check_monomers_per_lane": tuned["check_items"],
+                              "perform_group": tuned["perform_groups"],
                               "xcd_ranges": tuned["xcd_ranges"],
                               "order_within_colour": "morton" if tuned["morton_order"] else "id",
                               "autotune_probe_mcs": probe_mcs}},

@@ -212,25 +212,35 @@ int bfm_set_xcd_ranges(bfm_updater *h, int on);
  * configuration in force before the call stays.  After initialize. */
 int bfm_set_perform_group(bfm_updater *h, uint32_t group);
 int bfm_autotune(bfm_updater *h, uint32_t probe_mcs, uint32_t *group, int *xcd_ranges);
-/* Check workgroup size (64, 128, 256 = default, 512 or 1024 threads) for every colour
- * segment; the reference tunes threads per block per species
- * (UpdaterGPUScBFM_AB_Type.cu:2940-2990).  Any time (after initialize it
- * applies to every segment). */
-int bfm_set_check_block(bfm_updater *h, uint32_t threads);
+/* Check workgroup size in monomers (64, 128, 256 = default, 512 or 1024) for
+ * every colour segment; the reference tunes threads per block per species
+ * (UpdaterGPUScBFM_AB_Type.cu:2940-2990).  With one monomer per lane (below)
+ * that is the number of threads.  Any time (after initialize it applies to
+ * every segment). */
+int bfm_set_check_block(bfm_updater *h, uint32_t block);
+/* Monomers per lane of the check kernel, 1 or 2 (default), for every colour
+ * segment (no counterpart in the reference, one thread per monomer): with 2 a
+ * workgroup of B monomers has B / 2 threads and each wavefront takes two runs
+ * of 64 monomers, with the memory loads of both in flight together.  A
+ * workgroup of 64 monomers takes one per lane.  Any time. */
+int bfm_set_check_items(bfm_updater *h, uint32_t items);
 /* Launch configuration of one colour segment (after initialize): its check
- * workgroup size and perform group; get also reports the block-to-XCD
- * placement and whether the monomers of each colour lie in Morton order of
- * position (1) or in id order (0).  bfm_autotune chooses all of them: the
+ * workgroup size, monomers per lane and perform group; get also reports the
+ * block-to-XCD placement and whether the monomers of each colour lie in
+ * Morton order of position (1) or in id order (0), and bfm_get_check_items
+ * the monomers per lane in force.  bfm_autotune chooses all of them: the
  * order within colours by timing both (re-sorting on the device), then per
- * segment the fastest check workgroup size and perform group from the
- * segment's own kernel durations, under the faster placement, and keeps that
- * pick only if it also beats the defaults by wall time; each choice needs a
- * 2 % lead over the default.  The system advances by 15 * probe_mcs MCS when
- * the pick is the default, else 19 * probe_mcs (four fewer without spatial
- * sorting). */
-int bfm_set_segment_config(bfm_updater *h, uint32_t colour, uint32_t threads, uint32_t group);
-int bfm_get_launch_config(const bfm_updater *h, uint32_t colour, uint32_t *threads, uint32_t *group, int *xcd_ranges,
+ * segment the fastest check shape (workgroup size with one and with two
+ * monomers per lane) and perform group from the segment's own kernel
+ * durations, under the faster placement, and keeps that pick only if it also
+ * beats the defaults by wall time; each choice needs a 2 % lead over the
+ * default.  The system advances by 23 * probe_mcs MCS when the pick is the
+ * default, else 27 * probe_mcs (four fewer without spatial sorting). */
+int bfm_set_segment_config(bfm_updater *h, uint32_t colour, uint32_t block, uint32_t group);
+int bfm_set_segment_items(bfm_updater *h, uint32_t colour, uint32_t items);
+int bfm_get_launch_config(const bfm_updater *h, uint32_t colour, uint32_t *block, uint32_t *group, int *xcd_ranges,
                           int *morton_order);
+int bfm_get_check_items(const bfm_updater *h, uint32_t colour, uint32_t *items);
 
 /* Runs n_mcs steps without the copy-back of runSimulationOnGPU (positions
  * stay on the device; the age still advances).  For timing loops. */

@@ -141,6 +141,9 @@ SIGNATURES = {
     "bfm_set_perform_group": (c_int, [_H, c_uint32]),
     "bfm_set_check_block": (c_int, [_H, c_uint32]),
     "bfm_set_segment_config": (c_int, [_H, c_uint32, c_uint32, c_uint32]),
+    "bfm_set_check_items": (c_int, [_H, c_uint32]),
+    "bfm_set_segment_items": (c_int, [_H, c_uint32, c_uint32]),
+    "bfm_get_check_items": (c_int, [_H, c_uint32, POINTER(c_uint32)]),
     "bfm_get_launch_config": (c_int, [_H, c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_int), POINTER(c_int)]),
     "bfm_autotune": (c_int, [_H, c_uint32, POINTER(c_uint32), POINTER(c_int)]),
     "bfm_batch_set_perform_group": (c_int, [_H, c_uint32]),

@@ -151,156 +151,291 @@ __device__ __forceinline__ uint32_t load_or_zero(__amdgpu_buffer_rsrc_t r, bool 
     return uint32_t(__builtin_amdgcn_raw_buffer_load_b32(r, int(use ? word * 4u : kPastEnd), 0, 0));
 }
 
-// Phase 1 for check block `blk` of `nblk` covering the colour segment
-// [begin, begin+count).  Every wavefront compacts its own survivors into surv.
-template <bool kAllPeriodic, bool kCubic, uint32_t kThreads = kBlock>
-__device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
-                                           uint32_t sub_seed, uint32_t parity, uint32_t blk, uint32_t nblk)
-{
-    const Geometry &g = buf.geom;
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t gwave = wave_uniform(blk * (kThreads / kWave) + (threadIdx.x >> 6));
-    // the arrays read per lane, as buffer resources: 32-bit offsets
-    const __amdgpu_buffer_rsrc_t ppos = word_rsrc(buf.ppos, buf.n_pad);
-    const __amdgpu_buffer_rsrc_t ids = word_rsrc(buf.ids, buf.n_pad);
-    const __amdgpu_buffer_rsrc_t lat_rsrc = word_rsrc(buf.lattice, buf.lattice_words);
-    auto nbr_column = [&](int j) { return word_rsrc(buf.nbr + size_t(j) * buf.n_pad, buf.n_pad); };
-    const gptr<uint32_t> claim_cur = as_global(pick(parity, buf.claim0, buf.claim1));
-    const gptr<uint32_t> claim_prev = as_global(pick(parity, buf.claim1, buf.claim0));
+// ---- phase 1: the check ------------------------------------------------------
+// Pieces shared by the two forms of the check body below.
 
+// The arrays a check reads per lane, as buffer resources (32-bit offsets),
+// and the claim lattices of this substep and of the previous one.
+struct CheckArrays {
+    __amdgpu_buffer_rsrc_t ppos, ids, lattice;
+    gptr<uint32_t> claim_cur, claim_prev;
+    __device__ __forceinline__ CheckArrays(const SweepBuffers &buf, uint32_t parity)
+        : ppos(word_rsrc(buf.ppos, buf.n_pad)), ids(word_rsrc(buf.ids, buf.n_pad)),
+          lattice(word_rsrc(buf.lattice, buf.lattice_words)), claim_cur(as_global(pick(parity, buf.claim0, buf.claim1))),
+          claim_prev(as_global(pick(parity, buf.claim1, buf.claim0)))
+    {
+    }
+};
+
+// Column j of the partner table.  It has kMaxConnectivity columns; past a
+// slot's own partners, up to the largest partner count of the system, it
+// holds the slot's own index, so every read below stays in bounds (a lane
+// with fewer partners reads its own slot and ignores it), and a lane uses
+// only its first nn.  The site load's address is then the partner load alone,
+// which the compiler cannot turn into a branch around the load (that would
+// make it wait for every load in flight).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t nbr_column(const SweepBuffers &buf, int j)
+{
+    return word_rsrc(buf.nbr + size_t(j) * buf.n_pad, buf.n_pad);
+}
+
+// direction = rn % 6 (.cu:964-967); the generator is uniform per launch
+__device__ __forceinline__ int move_direction(const SweepBuffers &buf, uint32_t id, uint64_t age, uint32_t sub_seed)
+{
+    uint32_t rn;
+    if (buf.rng == kRngPhilox)
+        rn = philox_word0(buf.seed, id, age, kTagMove);
+    else
+        rn = int_hash(int_hash(id) ^ sub_seed);
+    return int(rn % 6u);
+}
+
+// Excluded volume: no corner in the 9-site front.  Only the words that hold
+// front sites are read (mask non-zero); the other lanes give their load an
+// offset past the lattice's end, which reads zero without an access, so the
+// loads stay unconditional.
+__device__ __forceinline__ void load_front(const CheckArrays &a, const Front &f, uint32_t (&l)[4])
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        l[q] = load_or_zero(a.lattice, f.m[q] != 0u, f.w[q]);
+}
+__device__ __forceinline__ bool front_free(const Front &f, const uint32_t (&l)[4])
+{
+    return ((l[0] & f.m[0]) | (l[1] & f.m[1]) | (l[2] & f.m[2]) | (l[3] & f.m[3])) == 0u;
+}
+
+// Walls and the bonds to the first kEarlyPartners partners (sites q) from
+// the moved corner.  Walls: a move along a walled axis may not leave
+// [0, L - 2]; the moved corner's coordinate is then L - 1 (from L - 2 upward,
+// or from 0 downward modulo L), whose dilated form is the axis's mask.  The
+// bond lookups run on every lane; a lane with fewer partners tests its own
+// slot and ignores the result.
+template <bool kAllPeriodic>
+__device__ __forceinline__ bool walls_and_early_bonds_ok(const Geometry &g, const WaveBondMask &bonds, const Front &f,
+                                                         int dir, uint32_t nn, const uint32_t (&q)[kEarlyPartners])
+{
+    bool ok = true;
+    if (!kAllPeriodic && !((g.periodic >> (dir >> 1)) & 1u) && (f.new_site & f.ma) == f.ma)
+        ok = false;
+    const uint32_t r1x = f.new_site & kLow9[0], r1y = f.new_site & kLow9[1], r1z = f.new_site & kLow9[2];
+#pragma unroll
+    for (int j = 0; j < kEarlyPartners; ++j) {
+        const bool bad = bonds.forbidden(bond_key(q[j], r1x, r1y, r1z));
+        if (uint32_t(j) < nn && bad)
+            ok = false;
+    }
+    return ok;
+}
+
+// The bonds to the partners beyond the early columns, in a wavefront with a
+// lane that has some (a network's crosslinker); every lane of the wave calls
+// it.  Their indices and sites are read here, after the counts, and live only
+// in this call, so waves without such lanes carry none.
+__device__ __forceinline__ bool late_bonds_ok(const SweepBuffers &buf, const CheckArrays &a,
+                                              const WaveBondMask &bonds, const Front &f, uint32_t i, uint32_t nn)
+{
+    bool ok = true;
+    uint32_t nn_wave = uint32_t(kEarlyPartners);
+#pragma unroll
+    for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
+        nn_wave = __ballot(nn > uint32_t(j)) ? uint32_t(j + 1) : nn_wave;
+    uint32_t partner[kMaxConnectivity - kEarlyPartners], q[kMaxConnectivity - kEarlyPartners];
+#pragma unroll
+    for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
+        if (uint32_t(j) < nn_wave)
+            partner[j - kEarlyPartners] = load_word(nbr_column(buf, j), i);
+#pragma unroll
+    for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
+        if (uint32_t(j) < nn_wave)
+            q[j - kEarlyPartners] = load_word(a.ppos, partner[j - kEarlyPartners]);
+    const uint32_t r1x = f.new_site & kLow9[0], r1y = f.new_site & kLow9[1], r1z = f.new_site & kLow9[2];
+#pragma unroll
+    for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
+        if (uint32_t(j) < nn_wave) {
+            const bool bad = bonds.forbidden(bond_key(q[j - kEarlyPartners], r1x, r1y, r1z));
+            if (uint32_t(j) < nn && bad)
+                ok = false;
+        }
+    return ok;
+}
+
+// A survivor marks its moved corner in the reference's tentative lattice
+// (dpLatticeTmp, :1012-1023), one bit per moved corner, and keeps its entry
+// (sorted index | direction, old corner) for the wave's compaction.
+__device__ __forceinline__ uint2 claim(const CheckArrays &a, const Front &f, uint32_t i, int dir, uint32_t s0)
+{
+    atomicOr((uint32_t *)&a.claim_cur[f.new_site >> 5], 1u << (f.new_site & 31u));
+    return make_uint2(i | (uint32_t(dir) << kSortedIndexBits), s0);
+}
+
+// Wave compaction of one run's survivors into its 64 slots of surv.
+__device__ __forceinline__ void compact(const SweepBuffers &buf, uint32_t run, bool pass, uint2 entry)
+{
+    const uint64_t mball = __ballot(pass);
+    if (pass)
+        element(as_global(buf.surv) + size_t(run) * kWave + lanes_below(mball)) = entry;
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        as_global(buf.scount)[run] = uint32_t(__popcll(mball));
+}
+
+// The previous substep's claims (the reference zeroes its tentative lattice in
+// a third kernel, :1240-1262): nobody reads claim[parity^1] in this kernel, so
+// the grid's `threads` threads (this one is number t) clear all of it with
+// coalesced 16-byte stores; at the densities of a melt a survivor's mark
+// shares every 64-byte line with others, so clearing marks one by one writes
+// more.
+__device__ __forceinline__ void clear_previous_claims(const SweepBuffers &buf, const CheckArrays &a, uint32_t t,
+                                                      uint32_t threads)
+{
+    const gptr<uint4> cp = reinterpret_cast<gptr<uint4>>(a.claim_prev);
+    for (uint32_t k = t; k < buf.lattice_words / 4u; k += threads)
+        element(cp + k) = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Phase 1 for check block `blk` of `nblk` covering the colour segment
+// [begin, begin+count), one monomer per lane; every wavefront compacts its
+// own survivors into surv.  Two levels of dependent loads per wave: the
+// slot's own words and its first kEarlyPartners partner indices, then the
+// front words and the partner sites.  Every load of the two stages is
+// unconditional, so the compiler knows how many are in flight and waits for
+// exactly the ones each stage needs (a load under a branch makes it wait for
+// all of them).  Every lane of the wave runs the body (lanes past the segment
+// on its last monomer), so that the cross-lane bond lookups find every source
+// lane active; only lanes inside the segment may pass.
+template <bool kAllPeriodic, bool kCubic, uint32_t kThreads>
+__device__ __forceinline__ void check_body_one(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
+                                               uint32_t sub_seed, uint32_t parity, uint32_t blk, uint32_t nblk)
+{
+    const CheckArrays a(buf, parity);
+    const uint32_t gwave = wave_uniform(blk * (kThreads / kWave) + (threadIdx.x >> 6));
     const uint32_t t = blk * kThreads + threadIdx.x;
     const WaveBondMask bonds(buf.tables);
     bool pass = false;
     uint2 entry = make_uint2(0u, 0u);
     if (count != 0) {
-        // Every lane of the wave runs the body (lanes past the segment on its
-        // last monomer), so that the cross-lane bond lookups find every
-        // source lane active; only lanes inside the segment may pass.
         const bool inside = t < count;
         const uint32_t i = begin + (inside ? t : count - 1u);
-        // Two levels of dependent loads per wave: the slot's own words and
-        // its first kEarlyPartners partner indices, then the front words and
-        // the partner sites.  Every load of the first two stages is
-        // unconditional, so the compiler knows how many are in flight and
-        // waits for exactly the ones each stage needs (a load under a branch
-        // makes it wait for all of them).  The partner table has
-        // kMaxConnectivity columns, zero beyond a slot's own partners, so
-        // these reads stay in bounds; a lane uses only its first nn.
-        const uint32_t idn = load_word(ids, i);
-        const uint32_t s0 = load_word(ppos, i);
-        uint32_t partner[kMaxConnectivity];
+        const uint32_t idn = load_word(a.ids, i);
+        const uint32_t s0 = load_word(a.ppos, i);
+        uint32_t partner[kEarlyPartners];
 #pragma unroll
         for (int j = 0; j < kEarlyPartners; ++j)
-            partner[j] = load_word(nbr_column(j), i);
-        const uint32_t id = idn & kIdMask, nn = idn >> kIdBits;
-        // a lane of the wave with partners beyond the early columns (a
-        // network's crosslinker)
+            partner[j] = load_word(nbr_column(buf, j), i);
+        const uint32_t nn = idn >> kIdBits;
         const bool more = __ballot(nn > uint32_t(kEarlyPartners)) != 0ull;
-
-        // direction = rn % 6 (.cu:964-967); the generator is uniform per launch
-        uint32_t rn;
-        if (buf.rng == kRngPhilox)
-            rn = philox_word0(buf.seed, id, age, kTagMove);
-        else
-            rn = int_hash(int_hash(id) ^ sub_seed);
-        const int dir = int(rn % 6u);
-
-        // excluded volume: no corner in the 9-site front.  Only the words
-        // that hold front sites are read (mask non-zero); the other lanes give
-        // their load an offset past the lattice's end, which reads zero
-        // without an access, so the loads stay unconditional
-        const Front f = front_of<kCubic>(g, s0, dir);
+        const int dir = move_direction(buf, idn & kIdMask, age, sub_seed);
+        const Front f = front_of<kCubic>(buf.geom, s0, dir);
         uint32_t l[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            l[q] = load_or_zero(lat_rsrc, f.m[q] != 0u, f.w[q]);
-
+        load_front(a, f, l);
         // bond partners belong to other colours and do not move this substep;
-        // every site load is issued before the first is used (a lane with
-        // fewer partners reads its own slot instead)
-        uint32_t qpos[kEarlyPartners];
+        // every site load is issued before the first is used
+        uint32_t q[kEarlyPartners];
 #pragma unroll
         for (int j = 0; j < kEarlyPartners; ++j)
-            qpos[j] = load_word(ppos, uint32_t(j) < nn ? partner[j] : i);
-
-        bool ok = true;
-        // walls: a move along a walled axis may not leave [0, L - 2]; the
-        // moved corner's coordinate is then L - 1 (from L - 2 upward, or from
-        // 0 downward modulo L), whose dilated form is the axis's mask
-        if (!kAllPeriodic && !((g.periodic >> (dir >> 1)) & 1u) && (f.new_site & f.ma) == f.ma)
+            q[j] = load_word(a.ppos, partner[j]);
+        bool ok = walls_and_early_bonds_ok<kAllPeriodic>(buf.geom, bonds, f, dir, nn, q);
+        if (more && !late_bonds_ok(buf, a, bonds, f, i, nn))
             ok = false;
-
-        // bonds from the moved corner (the lookups run on every lane; a lane
-        // with fewer partners tests its own slot and ignores the result)
-        const uint32_t r1x = f.new_site & kLow9[0], r1y = f.new_site & kLow9[1], r1z = f.new_site & kLow9[2];
-#pragma unroll
-        for (int j = 0; j < kEarlyPartners; ++j) {
-            const bool bad = bonds.forbidden(bond_key(qpos[j], r1x, r1y, r1z));
-            if (uint32_t(j) < nn && bad)
-                ok = false;
-        }
-        if (more) {
-            // the rest of the partners, after the counts (their sites live
-            // only in this branch, so waves without such lanes carry none)
-            uint32_t nn_wave = uint32_t(kEarlyPartners);
-#pragma unroll
-            for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
-                nn_wave = __ballot(nn > uint32_t(j)) ? uint32_t(j + 1) : nn_wave;
-            uint32_t q_more[kMaxConnectivity - kEarlyPartners];
-#pragma unroll
-            for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
-                if (uint32_t(j) < nn_wave)
-                    partner[j] = load_word(nbr_column(j), i);
-#pragma unroll
-            for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
-                if (uint32_t(j) < nn_wave)
-                    q_more[j - kEarlyPartners] = load_word(ppos, uint32_t(j) < nn ? partner[j] : i);
-#pragma unroll
-            for (int j = kEarlyPartners; j < kMaxConnectivity; ++j)
-                if (uint32_t(j) < nn_wave) {
-                    const bool bad = bonds.forbidden(bond_key(q_more[j - kEarlyPartners], r1x, r1y, r1z));
-                    if (uint32_t(j) < nn && bad)
-                        ok = false;
-                }
-        }
-        if ((l[0] & f.m[0]) | (l[1] & f.m[1]) | (l[2] & f.m[2]) | (l[3] & f.m[3]))
-            ok = false;
-
-        if (ok && inside) {
-            // the reference's tentative lattice (dpLatticeTmp, :1012-1023),
-            // one bit per moved corner
-            atomicOr((uint32_t *)&claim_cur[f.new_site >> 5], 1u << (f.new_site & 31u));
+        if (ok && front_free(f, l) && inside) {
+            entry = claim(a, f, i, dir, s0);
             pass = true;
-            entry = make_uint2(i | (uint32_t(dir) << kSortedIndexBits), s0);
         }
     }
-
-    // wave compaction of the survivors
-    const uint64_t mball = __ballot(pass);
-    if (pass)
-        element(as_global(buf.surv) + size_t(gwave) * kWave + lanes_below(mball)) = entry;
-    if (lane == 0)
-        as_global(buf.scount)[gwave] = uint32_t(__popcll(mball));
-
-    // the previous substep's claims (the reference zeroes its tentative
-    // lattice in a third kernel, :1240-1262): nobody reads claim[parity^1] in
-    // this kernel, so the grid clears all of it with coalesced 16-byte
-    // stores; at the densities of a melt a survivor's mark shares every
-    // 64-byte line with others, so clearing marks one by one writes more
-    const gptr<uint4> cp = reinterpret_cast<gptr<uint4>>(claim_prev);
-    for (uint32_t k = t; k < buf.lattice_words / 4u; k += nblk * kThreads)
-        element(cp + k) = make_uint4(0u, 0u, 0u, 0u);
+    compact(buf, gwave, pass, entry);
+    clear_previous_claims(buf, a, t, nblk * kThreads);
 }
 
-template <bool kAllPeriodic, bool kCubic, uint32_t kThreads>
+// The same with kItems >= 2 monomers per lane: a wavefront takes kItems runs
+// of 64 consecutive monomers (one per lane in each), issues the loads of all
+// of them before using any, and compacts each run's survivors into its own 64
+// slots of surv.  Run r of the segment always covers monomers
+// [64 r, 64 r + 64), so the survivors (and the perform kernel's work) do not
+// depend on the workgroup size or on kItems.  (With one run per lane this
+// form measured slower than check_body_one, whose order of loads and
+// arithmetic the compiler keeps; with two, 6 % faster on the relaxed 512^3
+// melt.)
+template <bool kAllPeriodic, bool kCubic, uint32_t kThreads, uint32_t kItems>
+__device__ __forceinline__ void check_body_runs(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
+                                                uint32_t sub_seed, uint32_t parity, uint32_t blk, uint32_t nblk)
+{
+    const CheckArrays a(buf, parity);
+    const uint32_t gwave = wave_uniform(blk * (kThreads / kWave) + (threadIdx.x >> 6));
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const WaveBondMask bonds(buf.tables);
+    bool pass[kItems];
+    uint2 entry[kItems];
+#pragma unroll
+    for (uint32_t it = 0; it < kItems; ++it) {
+        pass[it] = false;
+        entry[it] = make_uint2(0u, 0u);
+    }
+    if (count != 0) {
+        bool inside[kItems];
+        uint32_t i[kItems], idn[kItems], s0[kItems], partner[kItems][kEarlyPartners];
+#pragma unroll
+        for (uint32_t it = 0; it < kItems; ++it) {
+            const uint32_t ti = (gwave * kItems + it) * kWave + lane;
+            inside[it] = ti < count;
+            i[it] = begin + (inside[it] ? ti : count - 1u);
+            idn[it] = load_word(a.ids, i[it]);
+            s0[it] = load_word(a.ppos, i[it]);
+#pragma unroll
+            for (int j = 0; j < kEarlyPartners; ++j)
+                partner[it][j] = load_word(nbr_column(buf, j), i[it]);
+        }
+        int dir[kItems];
+        bool more[kItems];
+        Front f[kItems];
+        uint32_t l[kItems][4], q[kItems][kEarlyPartners];
+#pragma unroll
+        for (uint32_t it = 0; it < kItems; ++it) {
+            more[it] = __ballot((idn[it] >> kIdBits) > uint32_t(kEarlyPartners)) != 0ull;
+            dir[it] = move_direction(buf, idn[it] & kIdMask, age, sub_seed);
+            f[it] = front_of<kCubic>(buf.geom, s0[it], dir[it]);
+            load_front(a, f[it], l[it]);
+#pragma unroll
+            for (int j = 0; j < kEarlyPartners; ++j)
+                q[it][j] = load_word(a.ppos, partner[it][j]);
+        }
+#pragma unroll
+        for (uint32_t it = 0; it < kItems; ++it) {
+            const uint32_t nn = idn[it] >> kIdBits;
+            bool ok = walls_and_early_bonds_ok<kAllPeriodic>(buf.geom, bonds, f[it], dir[it], nn, q[it]);
+            if (more[it] && !late_bonds_ok(buf, a, bonds, f[it], i[it], nn))
+                ok = false;
+            if (ok && front_free(f[it], l[it]) && inside[it]) {
+                entry[it] = claim(a, f[it], i[it], dir[it], s0[it]);
+                pass[it] = true;
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t it = 0; it < kItems; ++it)
+        compact(buf, gwave * kItems + it, pass[it], entry[it]);
+    clear_previous_claims(buf, a, blk * kThreads + threadIdx.x, nblk * kThreads);
+}
+
+// Phase 1 with kItems monomers per lane.
+template <bool kAllPeriodic, bool kCubic, uint32_t kThreads = kBlock, uint32_t kItems = 1>
+__device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
+                                           uint32_t sub_seed, uint32_t parity, uint32_t blk, uint32_t nblk)
+{
+    if constexpr (kItems == 1)
+        check_body_one<kAllPeriodic, kCubic, kThreads>(buf, begin, count, age, sub_seed, parity, blk, nblk);
+    else
+        check_body_runs<kAllPeriodic, kCubic, kThreads, kItems>(buf, begin, count, age, sub_seed, parity, blk, nblk);
+}
+
+template <bool kAllPeriodic, bool kCubic, uint32_t kThreads, uint32_t kItems>
 __global__ __launch_bounds__(kThreads) void k_check(SweepBuffers buf, uint32_t begin, uint32_t count, uint64_t age,
                                                     uint32_t sub_seed, uint32_t parity)
 {
     // the grid size from the arguments (launch_check sizes the grid so), not
     // from the launch's implicit arguments, which are read after a branch
-    const uint32_t nblk = check_blocks(count, kThreads);
+    const uint32_t nblk = check_blocks(count, kThreads * kItems);
     const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, nblk) : blockIdx.x;
-    check_body<kAllPeriodic, kCubic, kThreads>(buf, begin, count, age, sub_seed, parity, blk, nblk);
+    check_body<kAllPeriodic, kCubic, kThreads, kItems>(buf, begin, count, age, sub_seed, parity, blk, nblk);
 }
 
 // Phase 2: every wave of perform block `blk` takes the survivors of kGroup
@@ -486,6 +621,8 @@ __global__ __launch_bounds__(kBlock) void k_schedule(const BatchSystem *__restri
 // load; the slot entry and the buffers depend on the system only, so both are
 // read before either exit test.  n_blocks is the grid size, passed with the
 // arguments rather than read from the launch's implicit arguments later.
+// A batched block takes one monomer per lane (two measured the same on the
+// 512-system mix, 173.5 against 173.2 ms of check kernels per 1000 MCS).
 template <bool kAllPeriodic, bool kCubic>
 __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__restrict__ sys,
                                                         const BlockEntry *__restrict__ blk_tab,
@@ -587,26 +724,44 @@ void dispatch(void (*kernel)(P...), dim3 grid, uint32_t threads, hipStream_t str
         hipLaunchKernelGGL(kernel, grid, dim3(threads), 0, stream, P(args)...);
 }
 
-template <uint32_t T>
+// B monomers per workgroup: B / I threads with I runs of 64 per wavefront
+template <uint32_t B, uint32_t I>
 void launch_check_threads(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t sub_seed,
                           uint32_t parity, hipStream_t stream, hipEvent_t start, hipEvent_t stop)
 {
-    const dim3 grid(check_blocks(count, T));
+    constexpr uint32_t T = B / I;
+    const dim3 grid(check_blocks(count, B));
     const bool periodic = buf.geom.periodic == 7u, cubic = buf.geom.cubic != 0u;
-    auto kernel = periodic ? (cubic ? k_check<true, true, T> : k_check<true, false, T>)
-                           : (cubic ? k_check<false, true, T> : k_check<false, false, T>);
+    auto kernel = periodic ? (cubic ? k_check<true, true, T, I> : k_check<true, false, T, I>)
+                           : (cubic ? k_check<false, true, T, I> : k_check<false, false, T, I>);
     dispatch(kernel, grid, T, stream, start, stop, buf, begin, count, age, sub_seed, parity);
 }
 
 hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t sub_seed,
-                        uint32_t parity, hipStream_t stream, hipEvent_t start, hipEvent_t stop, uint32_t block)
+                        uint32_t parity, hipStream_t stream, hipEvent_t start, hipEvent_t stop, uint32_t block,
+                        uint32_t items)
 {
+    if (!valid_check_items(items))
+        return hipErrorInvalidValue;
+    const bool two = check_items_for(block, items) == 2u;
     switch (block) {
-    case 64: launch_check_threads<64>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
-    case 128: launch_check_threads<128>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
-    case 256: launch_check_threads<256>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
-    case 512: launch_check_threads<512>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
-    case 1024: launch_check_threads<1024>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
+    case 64: launch_check_threads<64, 1>(buf, begin, count, age, sub_seed, parity, stream, start, stop); break;
+    case 128:
+        two ? launch_check_threads<128, 2>(buf, begin, count, age, sub_seed, parity, stream, start, stop)
+            : launch_check_threads<128, 1>(buf, begin, count, age, sub_seed, parity, stream, start, stop);
+        break;
+    case 256:
+        two ? launch_check_threads<256, 2>(buf, begin, count, age, sub_seed, parity, stream, start, stop)
+            : launch_check_threads<256, 1>(buf, begin, count, age, sub_seed, parity, stream, start, stop);
+        break;
+    case 512:
+        two ? launch_check_threads<512, 2>(buf, begin, count, age, sub_seed, parity, stream, start, stop)
+            : launch_check_threads<512, 1>(buf, begin, count, age, sub_seed, parity, stream, start, stop);
+        break;
+    case 1024:
+        two ? launch_check_threads<1024, 2>(buf, begin, count, age, sub_seed, parity, stream, start, stop)
+            : launch_check_threads<1024, 1>(buf, begin, count, age, sub_seed, parity, stream, start, stop);
+        break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -74,12 +74,22 @@ __host__ __device__ inline uint32_t perform_blocks(uint32_t cb, uint32_t group =
 {
     return (cb + group - 1) / group;
 }
-// Workgroup sizes of the plain check kernel with a compiled variant (a launch
-// parameter the autotuner chooses among per colour segment); kBlock is the
-// default, and surv / scount are sized for the largest.
+// Workgroup sizes of the plain check kernel with a compiled variant, in
+// monomers (a launch parameter the autotuner chooses among per colour
+// segment); kBlock is the default, and surv / scount are sized for the
+// largest.
 constexpr uint32_t kCheckBlockSizes[] = {64, 128, 256, 512, 1024};
 constexpr uint32_t kMaxCheckBlock = 1024;
 inline bool valid_check_block(uint32_t b) { return b == 64 || b == 128 || b == 256 || b == 512 || b == 1024; }
+// Monomers per lane of the plain check kernel, 1 or 2 (a launch parameter the
+// autotuner chooses among per colour segment): with 2, a workgroup of B
+// monomers has B / 2 threads and every wavefront takes two runs of 64
+// monomers, issuing the loads of both before using any.  A workgroup of 64
+// monomers takes one per lane whatever the setting.  2 is the default:
+// 26.9 against 28.7 us per check launch on the relaxed 512^3 melt.
+constexpr uint32_t kCheckItems = 2;
+inline bool valid_check_items(uint32_t items) { return items == 1 || items == 2; }
+inline uint32_t check_items_for(uint32_t block, uint32_t items) { return block >= 128 ? items : 1u; }
 // Perform group sizes with a compiled kernel (a launch parameter the
 // autotuners choose among); kPerformGroup is the default.
 constexpr uint32_t kPerformGroups[] = {2, 4, 8};
@@ -103,10 +113,11 @@ inline uint32_t default_perform_group(uint32_t check_waves, uint32_t resident_wa
 // sub_seed: substep_seed() of this substep (used by the integer-hash RNG).
 // With a start and a stop event, the events take the kernel's own dispatch
 // timestamps (its execution time, as a kernel trace reports it).
-// block: the check workgroup size (one of kCheckBlockSizes).
+// block: the check workgroup size in monomers (one of kCheckBlockSizes);
+// items: monomers per lane (check_items_for(block, items) are used).
 hipError_t launch_check(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age, uint32_t sub_seed,
                         uint32_t parity, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
-                        uint32_t block = kBlock);
+                        uint32_t block = kBlock, uint32_t items = kCheckItems);
 
 // Phase 2: every survivor with no other moved corner in its front moves.
 // count is the colour segment size of the check launch, check_block its

@@ -545,27 +545,47 @@ int bfm_autotune(bfm_updater *h, uint32_t probe_mcs, uint32_t *group, int *xcd_r
     });
 }
 
-int bfm_set_check_block(bfm_updater *h, uint32_t threads)
+int bfm_set_check_block(bfm_updater *h, uint32_t block)
 {
     NEED_HANDLE(h);
-    return guarded([&] { h->u.setCheckBlock(threads); });
+    return guarded([&] { h->u.setCheckBlock(block); });
 }
 
-int bfm_set_segment_config(bfm_updater *h, uint32_t colour, uint32_t threads, uint32_t group)
+int bfm_set_check_items(bfm_updater *h, uint32_t items)
 {
     NEED_HANDLE(h);
-    return guarded([&] { h->u.setSegmentConfig(colour, threads, group); });
+    return guarded([&] { h->u.setCheckItems(items); });
 }
 
-int bfm_get_launch_config(const bfm_updater *h, uint32_t colour, uint32_t *threads, uint32_t *group, int *xcd_ranges,
+int bfm_set_segment_config(bfm_updater *h, uint32_t colour, uint32_t block, uint32_t group)
+{
+    NEED_HANDLE(h);
+    return guarded([&] { h->u.setSegmentConfig(colour, block, group); });
+}
+
+int bfm_set_segment_items(bfm_updater *h, uint32_t colour, uint32_t items)
+{
+    NEED_HANDLE(h);
+    return guarded([&] { h->u.setSegmentItems(colour, items); });
+}
+
+int bfm_get_check_items(const bfm_updater *h, uint32_t colour, uint32_t *items)
+{
+    NEED_HANDLE(h);
+    if (!items)
+        return fail(BFM_E_INVALID_ARGUMENT, "null items");
+    return guarded([&] { *items = h->u.segmentItems(colour); });
+}
+
+int bfm_get_launch_config(const bfm_updater *h, uint32_t colour, uint32_t *block, uint32_t *group, int *xcd_ranges,
                           int *morton_order)
 {
     NEED_HANDLE(h);
     return guarded([&] {
         uint32_t t = 0, g = 0;
         h->u.segmentConfig(colour, t, g);
-        if (threads)
-            *threads = t;
+        if (block)
+            *block = t;
         if (group)
             *group = g;
         if (xcd_ranges)

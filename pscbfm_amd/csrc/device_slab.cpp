@@ -9,7 +9,8 @@
 namespace pscbfm {
 
 // Partner-table columns in use (the largest partner count); the columns
-// above are never read and stay zero.
+// above are never read (zero, or the slot's own index after a re-sort on the
+// device).
 static uint32_t used_partner_columns(const HostSystem &s)
 {
     uint32_t cols = 0;

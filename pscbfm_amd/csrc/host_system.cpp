@@ -413,7 +413,9 @@ void plan(HostSystem &s)
     s.ids_sorted.assign(s.n_pad, 0xFFFFFFFFu);
     s.image_sorted.assign(3 * size_t(s.n_pad), 0);
     // slot-major, only the columns in use (the device table has all
-    // kMaxConnectivity; the upload copies these and leaves the rest zero)
+    // kMaxConnectivity; the upload copies these and leaves the rest zero);
+    // past its own partners a slot holds its own index, which the check
+    // kernel reads unconditionally in place of a missing partner
     uint32_t cols = 0;
     for (const uint8_t c : s.nnbr)
         cols = std::max<uint32_t>(cols, c);
@@ -432,6 +434,8 @@ void plan(HostSystem &s)
         s.ids_sorted[t] = i | (uint32_t(s.nnbr[i]) << kIdBits);
         for (uint32_t j = 0; j < s.nnbr[i]; ++j)
             s.nbr_sorted[size_t(j) * s.n_pad + t] = s.to_sorted[s.nbr[size_t(i) * kMaxConnectivity + j]];
+        for (uint32_t j = s.nnbr[i]; j < cols; ++j)
+            s.nbr_sorted[size_t(j) * s.n_pad + t] = t;
     }
 }
 

@@ -45,7 +45,8 @@ __global__ __launch_bounds__(kResortBlock) void k_inverse(const uint32_t *__rest
         inv[perm[t]] = t;
 }
 
-// New slot t takes old slot perm[t]; partner indices move with their owners.
+// New slot t takes old slot perm[t]; partner indices move with their owners,
+// and past a slot's own partners the table holds the slot's own index.
 __global__ __launch_bounds__(kResortBlock) void k_gather(const uint32_t *__restrict__ ppos,
                                                          const uint32_t *__restrict__ ids,
                                                          const int32_t *__restrict__ image,
@@ -68,7 +69,7 @@ __global__ __launch_bounds__(kResortBlock) void k_gather(const uint32_t *__restr
     const uint32_t nn = idn == 0xFFFFFFFFu ? 0u : idn >> kIdBits;
 #pragma unroll
     for (int j = 0; j < kMaxConnectivity; ++j)
-        nbr2[size_t(j) * n_pad + t] = uint32_t(j) < nn ? inv[nbr[size_t(j) * n_pad + o]] : 0u;
+        nbr2[size_t(j) * n_pad + t] = uint32_t(j) < nn ? inv[nbr[size_t(j) * n_pad + o]] : t;
 }
 
 uint32_t site_bits(const HostSystem &s)

@@ -186,6 +186,34 @@ void UpdaterGPUScBFM_AB_Type::setCheckBlock(uint32_t block)
     std::fill(mSegBlock.begin(), mSegBlock.end(), block);
 }
 
+void UpdaterGPUScBFM_AB_Type::setCheckItems(uint32_t items)
+{
+    if (!valid_check_items(items))
+        throw std::invalid_argument("check monomers per lane must be 1 or 2");
+    mCheckItems = items;
+    std::fill(mSegItems.begin(), mSegItems.end(), items);
+}
+
+void UpdaterGPUScBFM_AB_Type::setSegmentItems(uint32_t colour, uint32_t items)
+{
+    if (!mInitialized)
+        throw StateError("segment configuration before initialize");
+    if (colour >= mSegItems.size())
+        throw std::invalid_argument("no such colour");
+    if (!valid_check_items(items))
+        throw std::invalid_argument("check monomers per lane must be 1 or 2");
+    mSegItems[colour] = items;
+}
+
+uint32_t UpdaterGPUScBFM_AB_Type::segmentItems(uint32_t colour) const
+{
+    if (!mInitialized)
+        throw StateError("segment configuration before initialize");
+    if (colour >= mSegItems.size())
+        throw std::invalid_argument("no such colour");
+    return check_items_for(mSegBlock[colour], mSegItems[colour]);
+}
+
 void UpdaterGPUScBFM_AB_Type::setSegmentConfig(uint32_t colour, uint32_t check_block, uint32_t group)
 {
     if (!mInitialized)
@@ -267,15 +295,18 @@ double UpdaterGPUScBFM_AB_Type::wallProbe(uint32_t n_mcs)
 // than 2 %: short kernels time within a few per cent of each other from run
 // to run, and a choice made on noise can be slower than the default.
 static constexpr double kTuneMargin = 0.98;
-// The probes of the per-segment search: every check workgroup size once, each
-// paired with a perform group so that every group is timed; probe kDefaultProbe
-// has the default workgroup size (each segment's default group is
-// default_perform_group of its size).
-constexpr int kProbes = 5;
-constexpr uint32_t kProbeBlock[kProbes] = {64, 128, 256, 512, 1024};
-constexpr uint32_t kProbeGroup[kProbes] = {4, 2, 4, 8, 4};
-constexpr int kDefaultProbe = 2;
-static_assert(kProbeBlock[kDefaultProbe] == kBlock, "the default probe has the default workgroup size");
+// The probes of the per-segment search: every check workgroup size with one
+// and with two monomers per lane (64 with one only), each paired with a
+// perform group so that every group is timed; probe kDefaultProbe has the
+// default shape (each segment's default group is default_perform_group of its
+// size).
+constexpr int kProbes = 9;
+constexpr uint32_t kProbeBlock[kProbes] = {64, 128, 256, 512, 1024, 128, 256, 512, 1024};
+constexpr uint32_t kProbeItems[kProbes] = {1, 1, 1, 1, 1, 2, 2, 2, 2};
+constexpr uint32_t kProbeGroup[kProbes] = {4, 2, 8, 8, 4, 4, 4, 2, 8};
+constexpr int kDefaultProbe = 6;
+static_assert(kProbeBlock[kDefaultProbe] == kBlock && kProbeItems[kDefaultProbe] == kCheckItems,
+              "the default probe has the default check shape");
 
 void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
 {
@@ -285,15 +316,18 @@ void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
         throw std::invalid_argument("autotune: probe_mcs must be positive");
     hip_check(hipSetDevice(mGpu), "hipSetDevice");
     const uint32_t nc = mSys.n_colours;
-    const std::vector<uint32_t> block0 = mSegBlock, group0 = mSegGroup;
+    const std::vector<uint32_t> block0 = mSegBlock, items0 = mSegItems, group0 = mSegGroup;
     const bool xcd0 = mXcdRanges, morton0 = mMorton;
     auto restore = [&] {
         mSegBlock = block0;
+        mSegItems = items0;
         mSegGroup = group0;
         setXcdRanges(xcd0);
     };
-    auto set_config = [&](const std::vector<uint32_t> &block, const std::vector<uint32_t> &group, bool xcd) {
+    auto set_config = [&](const std::vector<uint32_t> &block, const std::vector<uint32_t> &items,
+                          const std::vector<uint32_t> &group, bool xcd) {
         mSegBlock = block;
+        mSegItems = items;
         mSegGroup = group;
         setXcdRanges(xcd);
     };
@@ -317,21 +351,24 @@ void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
             if (t_other < kTuneMargin * t_now)
                 resortLayout(!mMorton);
         }
-        // 2. per segment: check workgroup size and perform group, per placement,
-        // from each colour's own kernel durations (the perform kernel's work
-        // does not depend on the check workgroup size: wave w of a check launch
-        // covers monomers [64 w, 64 w + 64) whatever the size)
+        // 2. per segment: check shape (workgroup size, monomers per lane) and
+        // perform group, per placement, from each colour's own kernel
+        // durations (the perform kernel's work does not depend on the check
+        // shape: run r of a check launch covers monomers [64 r, 64 r + 64)
+        // whatever the shape)
         double best_total[2] = {0.0, 0.0};
-        std::vector<uint32_t> best_block[2], best_group[2];
+        std::vector<uint32_t> best_block[2], best_items[2], best_group[2];
         for (int x = 0; x < 2; ++x) {
             setXcdRanges(x == 0);
             std::vector<double> check_t[kProbes], perform_t[kProbes];
             for (int k = 0; k < kProbes; ++k) {
                 std::fill(mSegBlock.begin(), mSegBlock.end(), kProbeBlock[k]);
+                std::fill(mSegItems.begin(), mSegItems.end(), kProbeItems[k]);
                 std::fill(mSegGroup.begin(), mSegGroup.end(), kProbeGroup[k]);
                 timedProbe(probe_mcs, check_t[k], perform_t[k]);
             }
             best_block[x].assign(nc, kBlock);
+            best_items[x].assign(nc, kCheckItems);
             best_group[x].resize(nc);
             for (uint32_t c = 0; c < nc; ++c) {
                 // the default unless another is faster by more than the margin;
@@ -355,6 +392,7 @@ void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
                     if (group_t[gi] >= 0.0 && group_t[gi] < kTuneMargin * group_t[bp])
                         bp = gi;
                 best_block[x][c] = kProbeBlock[bc];
+                best_items[x][c] = kProbeItems[bc];
                 best_group[x][c] = kPerformGroups[bp];
                 best_total[x] += check_t[bc][c] + group_t[bp];
             }
@@ -363,18 +401,19 @@ void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
         // 3. the pick against the defaults by wall time, which also sees what
         // per-kernel durations do not (the boundaries between dependent
         // kernels): two alternating probes each, the faster of each pair
-        const std::vector<uint32_t> def_block(nc, kBlock);
+        const std::vector<uint32_t> def_block(nc, kBlock), def_items(nc, kCheckItems);
         std::vector<uint32_t> def_group(nc);
         for (uint32_t c = 0; c < nc; ++c)
             def_group[c] = defaultGroup(c);
-        const bool tuned_is_default = best_block[x] == def_block && best_group[x] == def_group && x == 0;
+        const bool tuned_is_default =
+            best_block[x] == def_block && best_items[x] == def_items && best_group[x] == def_group && x == 0;
         bool keep_tuned = false;
         if (!tuned_is_default) {
             double t_def = 0.0, t_tuned = 0.0;
             for (int rep = 0; rep < 2; ++rep) {
-                set_config(def_block, def_group, true);
+                set_config(def_block, def_items, def_group, true);
                 const double a = wallProbe(probe_mcs);
-                set_config(best_block[x], best_group[x], x == 0);
+                set_config(best_block[x], best_items[x], best_group[x], x == 0);
                 const double b = wallProbe(probe_mcs);
                 t_def = rep == 0 ? a : std::min(t_def, a);
                 t_tuned = rep == 0 ? b : std::min(t_tuned, b);
@@ -382,9 +421,9 @@ void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
             keep_tuned = t_tuned < kTuneMargin * t_def;
         }
         if (keep_tuned)
-            set_config(best_block[x], best_group[x], x == 0);
+            set_config(best_block[x], best_items[x], best_group[x], x == 0);
         else
-            set_config(def_block, def_group, true);
+            set_config(def_block, def_items, def_group, true);
     } catch (...) {
         restore();
         if (mMorton != morton0) {
@@ -397,6 +436,7 @@ void UpdaterGPUScBFM_AB_Type::autotune(uint32_t probe_mcs)
     }
     mPerformGroup = mSegGroup.empty() ? kPerformGroup : mSegGroup[0];
     mCheckBlock = mSegBlock.empty() ? kBlock : mSegBlock[0];
+    mCheckItems = mSegItems.empty() ? kCheckItems : mSegItems[0];
 }
 
 void UpdaterGPUScBFM_AB_Type::setSpatialSorting(bool on)
@@ -472,6 +512,7 @@ void UpdaterGPUScBFM_AB_Type::initialize()
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, mGpu) == hipSuccess && cus > 0)
         mResidentWaves = uint32_t(cus) * 4u * 8u;
     mSegBlock.assign(s.n_colours, mCheckBlock);
+    mSegItems.assign(s.n_colours, mCheckItems);
     mSegGroup.resize(s.n_colours);
     for (uint32_t c = 0; c < s.n_colours; ++c)
         mSegGroup[c] = mPerformGroupSet ? mPerformGroup : defaultGroup(c);
@@ -509,7 +550,7 @@ void UpdaterGPUScBFM_AB_Type::runResident(uint32_t n_mcs, bool timing)
                 for (auto &x : ev)
                     x = takeEvent();
             hip_check(launch_check(mBuf, s.seg_begin[c], s.seg_count[c], age, substep_seed(mSeed, age, k), parity,
-                                   mStream, ev[0], ev[1], mSegBlock[c]),
+                                   mStream, ev[0], ev[1], mSegBlock[c], mSegItems[c]),
                       "check kernel");
             hip_check(launch_perform(mBuf, s.seg_count[c], parity, mStream, mSegGroup[c], ev[2], ev[3], mSegBlock[c]),
                       "perform kernel");

@@ -91,6 +91,11 @@ public:
     uint32_t performGroup() const { return mPerformGroup; }
     // check workgroup size (64, 128, 256, 512 or 1024 threads), for every colour segment
     void setCheckBlock(uint32_t block);
+    // Monomers per lane of the check kernel (1 or 2) for every colour segment
+    // (any time), or for one (after initialize).
+    void setCheckItems(uint32_t items);
+    void setSegmentItems(uint32_t colour, uint32_t items);
+    uint32_t segmentItems(uint32_t colour) const;
     // the launch configuration of colour segment c (after initialize)
     void setSegmentConfig(uint32_t colour, uint32_t check_block, uint32_t group);
     void segmentConfig(uint32_t colour, uint32_t &check_block, uint32_t &group) const;
@@ -173,7 +178,8 @@ private:
     bool mPerformGroupSet = false;    // setPerformGroup called: every segment uses it
     uint32_t mResidentWaves = 8192;   // 256 CUs x 4 SIMDs x 8 waves; queried at initialize
     uint32_t mCheckBlock = kBlock;
-    std::vector<uint32_t> mSegBlock, mSegGroup;   // per colour segment
+    uint32_t mCheckItems = kCheckItems;
+    std::vector<uint32_t> mSegBlock, mSegGroup, mSegItems;   // per colour segment
     bool mMorton = true;          // current order within colours (with spatial sorting)
     int64_t mStepsBetweenSortings = 5000;   // mnStepsBetweenSortings, UpdaterGPUScBFM_AB_Type.cu:1379
     std::vector<hipEvent_t> mEventPool;          // reusable events

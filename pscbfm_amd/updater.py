@@ -17,6 +17,14 @@ Rng_IntHash = 1   # the reference's default (.cu:1407)
 Rng_Philox = 3
 
 
+
+def _mcs(n_mcs: int) -> int:
+    """An MCS count for the C ABI's uint32_t (a negative count would wrap)."""
+    n = int(n_mcs)
+    if n < 0 or n > 0xFFFFFFFF:
+        raise ValueError(f"MCS count {n_mcs} out of range [0, 2^32)")
+    return n
+
 class UpdaterGPUScBFM_AB_Type:
     """Same setters, call order and errors as the reference updater.
 
@@ -92,13 +100,13 @@ class UpdaterGPUScBFM_AB_Type:
         _capi.check(self._lib.bfm_initialize(self._h))
 
     def runSimulationOnGPU(self, n_mcs: int) -> None:
-        _capi.check(self._lib.bfm_run_simulation_on_gpu(self._h, n_mcs))
+        _capi.check(self._lib.bfm_run_simulation_on_gpu(self._h, _mcs(n_mcs)))
 
     def runSimulationOnGPUNoAlloc(self, n_mcs: int) -> None:
         """runSimulationOnGPU without any allocation: the copy-back is staged in
         the library's static device scratch (at most bfm_noalloc_max_monomers
         monomers; calls that overlap on one GPU race)."""
-        _capi.check(self._lib.bfm_run_simulation_on_gpu_noalloc(self._h, n_mcs))
+        _capi.check(self._lib.bfm_run_simulation_on_gpu_noalloc(self._h, _mcs(n_mcs)))
 
     def getMonomerPositionInX(self, i: int) -> int:
         v = c_int32()
@@ -174,8 +182,8 @@ class UpdaterGPUScBFM_AB_Type:
         """Chooses the order within colours (Morton or id), then per colour
         segment the check workgroup size and perform group, and the XCD
         placement, from short timed probes, keeping the pick only if it also
-        beats the defaults by wall time (the system advances by 15 * probe_mcs
-        MCS when the pick is the default, else 19 * probe_mcs; four fewer
+        beats the defaults by wall time (the system advances by 23 * probe_mcs
+        MCS when the pick is the default, else 27 * probe_mcs; four fewer
         without spatial sorting).  Returns the chosen configuration (per
         colour with n_colors)."""
         g, x = c_uint32(), ctypes.c_int()
@@ -184,22 +192,30 @@ class UpdaterGPUScBFM_AB_Type:
         if n_colors:
             cfg = [self.launchConfig(c) for c in range(n_colors)]
             out.update({"check_block": [c["check_block"] for c in cfg],
+                        "check_items": [c["check_items"] for c in cfg],
                         "perform_groups": [c["perform_group"] for c in cfg],
                         "morton_order": cfg[0]["morton_order"]})
         return out
 
-    def setCheckBlock(self, threads: int) -> None:
-        """Check workgroup size (64, 128, 256, 512 or 1024 threads) for every colour segment."""
-        _capi.check(self._lib.bfm_set_check_block(self._h, int(threads)))
+    def setCheckBlock(self, block: int) -> None:
+        """Check workgroup size (64, 128, 256, 512 or 1024 monomers) for every colour segment."""
+        _capi.check(self._lib.bfm_set_check_block(self._h, int(block)))
 
-    def setSegmentConfig(self, colour: int, threads: int, group: int) -> None:
-        _capi.check(self._lib.bfm_set_segment_config(self._h, int(colour), int(threads), int(group)))
+    def setCheckItems(self, items: int) -> None:
+        """Monomers per lane of the check kernel (1 or 2) for every colour segment."""
+        _capi.check(self._lib.bfm_set_check_items(self._h, int(items)))
+
+    def setSegmentConfig(self, colour: int, block: int, group: int, items: int | None = None) -> None:
+        _capi.check(self._lib.bfm_set_segment_config(self._h, int(colour), int(block), int(group)))
+        if items is not None:
+            _capi.check(self._lib.bfm_set_segment_items(self._h, int(colour), int(items)))
 
     def launchConfig(self, colour: int) -> dict:
-        t, g, x, m = c_uint32(), c_uint32(), ctypes.c_int(), ctypes.c_int()
+        t, g, x, m, it = c_uint32(), c_uint32(), ctypes.c_int(), ctypes.c_int(), c_uint32()
         _capi.check(self._lib.bfm_get_launch_config(self._h, int(colour), byref(t), byref(g), byref(x), byref(m)))
-        return {"check_block": t.value, "perform_group": g.value, "xcd_ranges": bool(x.value),
-                "morton_order": bool(m.value)}
+        _capi.check(self._lib.bfm_get_check_items(self._h, int(colour), byref(it)))
+        return {"check_block": t.value, "check_items": it.value, "perform_group": g.value,
+                "xcd_ranges": bool(x.value), "morton_order": bool(m.value)}
 
     def setXcdRanges(self, on: bool) -> None:
         """Contiguous block range per XCD (default) or round-robin placement;
@@ -207,7 +223,7 @@ class UpdaterGPUScBFM_AB_Type:
         _capi.check(self._lib.bfm_set_xcd_ranges(self._h, int(bool(on))))
 
     def runResident(self, n_mcs: int) -> None:
-        _capi.check(self._lib.bfm_run_resident(self._h, n_mcs))
+        _capi.check(self._lib.bfm_run_resident(self._h, _mcs(n_mcs)))
 
     def copyBack(self) -> None:
         _capi.check(self._lib.bfm_copy_back(self._h))

@@ -221,20 +221,21 @@ def test_autotune_keeps_the_trajectory(gpu_ready):
     seed = 0x5EED0000000000F3
     print(f"case: autotune n={n}", flush=True)
     u = updater_from_system(s, seed=seed, gpu=0)
-    # one untimed probe, 4 layout probes, 10 configurations, and 4 more when
+    # one untimed probe, 4 layout probes, 18 configurations, and 4 more when
     # the pick is not the default (compared by wall time): 5 MCS each
     cfg = u.autotune(5, n_colors=2)
     assert cfg["perform_group"] in (2, 4, 8)
     assert all(b in (64, 128, 256, 512, 1024) for b in cfg["check_block"])
+    assert all(i in (1, 2) for i in cfg["check_items"])
     assert all(g in (2, 4, 8) for g in cfg["perform_groups"])
     age = u.getAge()
-    assert age in (75, 95)
-    u.runSimulationOnGPU(100 - age)
+    assert age in (115, 135)
+    u.runSimulationOnGPU(150 - age)
     got = u.getMonomerPositions(n)
     colors, _ = u.getColors(n)
     st = u.getStats()
     u.cleanup()
-    want, acc, prop = orc.sweep(orc.System(s, seed, colors), s["xyz"], 0, 100, 0)
+    want, acc, prop = orc.sweep(orc.System(s, seed, colors), s["xyz"], 0, 150, 0)
     assert np.array_equal(got, want)
     assert (st["accepted"], st["proposals"]) == (acc, prop)
 
@@ -242,9 +243,10 @@ def test_autotune_keeps_the_trajectory(gpu_ready):
 @pytest.mark.parametrize("system", ["melt", "network", "walls"])
 def test_every_launch_candidate_keeps_the_trajectory(gpu_ready, system):
     """Every candidate the autotuner chooses among -- check workgroup size 64,
-    128, 256, 512 or 1024, perform group 2, 4 or 8, XCD placement on or off, Morton or id
-    order within colours -- and different configurations per colour segment
-    give the oracle's trajectory bit-exactly."""
+    128, 256, 512 or 1024 monomers with one or two monomers per lane, perform
+    group 2, 4 or 8, XCD placement on or off, Morton or id order within
+    colours -- and different configurations per colour segment give the
+    oracle's trajectory bit-exactly."""
     if system == "melt":
         s = systems.linear_melt(32, 128, 16, seed=91)
     elif system == "network":
@@ -255,26 +257,29 @@ def test_every_launch_candidate_keeps_the_trajectory(gpu_ready, system):
     seed = 0x5EED0000000000E9
     mcs = 12
     want, acc, prop = orc.sweep(orc.System(s, seed), s["xyz"], 0, mcs, 0)
-    configs = [(b, g, x, m) for b in (64, 128, 256, 512, 1024) for g in (2, 4, 8) for x in (True, False)
-               for m in (True, False)]
+    configs = [(b, i, g, x, m) for b in (64, 128, 256, 512, 1024) for i in (1, 2) for g in (2, 4, 8)
+               for x in (True, False) for m in (True, False) if b > 64 or i == 1]
     print(f"case: {system}, n={n}, {len(configs)} launch configurations + mixed per-segment ones", flush=True)
-    for b, g, x, m in configs:
+    for b, i, g, x, m in configs:
         u = updater_from_system(s, seed=seed, gpu=0, spatial_sorting=m)
         u.setCheckBlock(b)
+        u.setCheckItems(i)
         u.setPerformGroup(g)
         u.setXcdRanges(x)
         u.runSimulationOnGPU(mcs)
         st = u.getStats()
         got = u.getMonomerPositions(n)
+        assert u.launchConfig(0)["check_items"] == i
         u.cleanup()
-        assert np.array_equal(got, want), (b, g, x, m)
-        assert (st["accepted"], st["proposals"]) == (acc, prop), (b, g, x, m)
+        assert np.array_equal(got, want), (b, i, g, x, m)
+        assert (st["accepted"], st["proposals"]) == (acc, prop), (b, i, g, x, m)
     u = updater_from_system(s, seed=seed, gpu=0)
     _, nc = u.getColors(n)
-    blocks, groups = (1024, 64, 512, 128), (8, 2, 4, 2)
+    blocks, groups, items = (1024, 64, 512, 128), (8, 2, 4, 2), (2, 1, 1, 2)
     for c in range(nc):
-        u.setSegmentConfig(c, blocks[c % 4], groups[c % 4])
+        u.setSegmentConfig(c, blocks[c % 4], groups[c % 4], items[c % 4])
     assert [u.launchConfig(c)["check_block"] for c in range(nc)] == [blocks[c % 4] for c in range(nc)]
+    assert [u.launchConfig(c)["check_items"] for c in range(nc)] == [items[c % 4] for c in range(nc)]
     assert [u.launchConfig(c)["perform_group"] for c in range(nc)] == [groups[c % 4] for c in range(nc)]
     u.runSimulationOnGPU(mcs)
     assert np.array_equal(u.getMonomerPositions(n), want)

@@ -296,7 +296,8 @@ def test_perform_group_must_be_compiled():
 
 
 def test_check_block_must_be_compiled():
-    # the check kernel is built for 64 ... 1024 threads per workgroup
+    # the check kernel is built for 64 ... 1024 monomers per workgroup, one or
+    # two per lane
     from pscbfm_amd import _capi, systems, updater_from_system
     u = updater_from_system(systems.linear_melt(16, 8, 8, seed=1), seed=1, gpu=None, validate_only=True)
     for b in (64, 128, 256, 512, 1024):
@@ -304,3 +305,8 @@ def test_check_block_must_be_compiled():
     for b in (0, 32, 96, 192, 2048):
         with pytest.raises(_capi.InvalidArgument):
             u.setCheckBlock(b)
+    for i in (1, 2):
+        u.setCheckItems(i)
+    for i in (0, 3, 4):
+        with pytest.raises(_capi.InvalidArgument):
+            u.setCheckItems(i)
