@@ -218,11 +218,13 @@ int bfm_autotune(bfm_updater *h, uint32_t probe_mcs, uint32_t *group, int *xcd_r
  * that is the number of threads.  Any time (after initialize it applies to
  * every segment). */
 int bfm_set_check_block(bfm_updater *h, uint32_t block);
-/* Monomers per lane of the check kernel, 1 or 2 (default), for every colour
- * segment (no counterpart in the reference, one thread per monomer): with 2 a
- * workgroup of B monomers has B / 2 threads and each wavefront takes two runs
- * of 64 monomers, with the memory loads of both in flight together.  A
- * workgroup of 64 monomers takes one per lane.  Any time. */
+/* Monomers per lane of the check kernel, 1 or 2, for every colour segment (no
+ * counterpart in the reference, one thread per monomer): with 2 a workgroup of
+ * B monomers has B / 2 threads and each wavefront takes two runs of 64
+ * monomers, with the memory loads of both in flight together.  A workgroup of
+ * 64 monomers takes one per lane.  Without this call a segment takes 2 where
+ * it has at least 4 resident generations of 64-monomer runs (about 2 M
+ * monomers on an MI355X), else 1.  Any time. */
 int bfm_set_check_items(bfm_updater *h, uint32_t items);
 /* Launch configuration of one colour segment (after initialize): its check
  * workgroup size, monomers per lane and perform group; get also reports the

@@ -85,11 +85,21 @@ inline bool valid_check_block(uint32_t b) { return b == 64 || b == 128 || b == 2
 // autotuner chooses among per colour segment): with 2, a workgroup of B
 // monomers has B / 2 threads and every wavefront takes two runs of 64
 // monomers, issuing the loads of both before using any.  A workgroup of 64
-// monomers takes one per lane whatever the setting.  2 is the default:
-// 26.9 against 28.7 us per check launch on the relaxed 512^3 melt.
+// monomers takes one per lane whatever the setting.
 constexpr uint32_t kCheckItems = 2;
 inline bool valid_check_items(uint32_t items) { return items == 1 || items == 2; }
 inline uint32_t check_items_for(uint32_t block, uint32_t items) { return block >= 128 ? items : 1u; }
+// The monomers per lane a colour segment of `check_waves` runs of 64 starts
+// with: 2 where the launch still has at least 2 generations of resident
+// wavefronts with them (fewer, longer waves hide more of each wave's
+// dependent loads: 27.3 against 28.7 us per check launch on the relaxed 512^3
+// melt, 65,536 runs, 8 generations of one run), else 1 (a launch of about one
+// generation, such as a 532,480-monomer colour, would leave half the wave
+// slots empty: 9.0 against 8.2 us there).
+inline uint32_t default_check_items(uint32_t check_waves, uint32_t resident_waves)
+{
+    return check_waves >= 4ull * resident_waves ? 2u : 1u;
+}
 // Perform group sizes with a compiled kernel (a launch parameter the
 // autotuners choose among); kPerformGroup is the default.
 constexpr uint32_t kPerformGroups[] = {2, 4, 8};

@@ -15,25 +15,38 @@ namespace {
 constexpr int kResortBlock = 256;
 
 // Sort key of every slot (padding slots included; they lie outside the
-// sorted ranges): the site of its lower corner, or its id; and the identity
-// permutation.
+// sorted ranges): the site of its lower corner, its id, or the slot saved for
+// its id; and the identity permutation.
 __global__ __launch_bounds__(kResortBlock) void k_keys(const uint32_t *__restrict__ ppos,
                                                        const uint32_t *__restrict__ ids, uint32_t n_pad,
-                                                       uint32_t by_id, uint32_t *__restrict__ keys,
-                                                       uint32_t *__restrict__ ident, uint32_t *__restrict__ perm)
+                                                       uint32_t by_id, const uint32_t *__restrict__ slot_of_id,
+                                                       uint32_t *__restrict__ keys, uint32_t *__restrict__ ident,
+                                                       uint32_t *__restrict__ perm)
 {
     const uint32_t t = blockIdx.x * kResortBlock + threadIdx.x;
     if (t >= n_pad)
         return;
-    if (by_id) {
-        keys[t] = ids[t] & kIdMask;
-        ident[t] = t;
-        perm[t] = t;
-        return;
-    }
-    keys[t] = ppos[t];   // the corner's site: Morton order
+    const uint32_t idn = ids[t];
+    if (slot_of_id)
+        keys[t] = idn == 0xFFFFFFFFu ? t : slot_of_id[idn & kIdMask];
+    else if (by_id)
+        keys[t] = idn & kIdMask;
+    else
+        keys[t] = ppos[t];   // the corner's site: Morton order
     ident[t] = t;
     perm[t] = t;
+}
+
+// slot_of_id[id] = t for every slot t that holds a monomer.
+__global__ __launch_bounds__(kResortBlock) void k_save_slots(const uint32_t *__restrict__ ids, uint32_t n_pad,
+                                                             uint32_t *__restrict__ slot_of_id)
+{
+    const uint32_t t = blockIdx.x * kResortBlock + threadIdx.x;
+    if (t >= n_pad)
+        return;
+    const uint32_t idn = ids[t];
+    if (idn != 0xFFFFFFFFu)
+        slot_of_id[idn & kIdMask] = t;
 }
 
 // inv[perm[t]] = t: where every old slot went.
@@ -147,7 +160,7 @@ void Resorter::reserve(const HostSystem &s, const SlabLayout &L)
 }
 
 void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const SweepBuffers &buf, hipStream_t stream,
-                   bool by_id)
+                   bool by_id, const uint32_t *slot_of_id)
 {
     const uint32_t n_pad = s.n_pad;
     const ResortSpace sp = resort_space(s, L);
@@ -159,11 +172,11 @@ void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const S
     uint32_t *ident = reinterpret_cast<uint32_t *>(w + sp.keys + 2 * sp.words);
     uint32_t *perm = reinterpret_cast<uint32_t *>(w + sp.keys + 3 * sp.words);
     uint32_t *inv = reinterpret_cast<uint32_t *>(w + sp.keys + 4 * sp.words);
-    const uint32_t bits = by_id ? kIdBits : site_bits(s);
+    const uint32_t bits = slot_of_id ? kSortedIndexBits : (by_id ? kIdBits : site_bits(s));
 
     const dim3 grid((n_pad + kResortBlock - 1) / kResortBlock), block(kResortBlock);
-    hipLaunchKernelGGL(k_keys, grid, block, 0, stream, buf.ppos, buf.ids, n_pad, by_id ? 1u : 0u, keys, ident,
-                       perm);
+    hipLaunchKernelGGL(k_keys, grid, block, 0, stream, buf.ppos, buf.ids, n_pad, by_id ? 1u : 0u, slot_of_id, keys,
+                       ident, perm);
     hip_check(hipGetLastError(), "re-sort keys");
     for (uint32_t c = 0; c < s.n_colours; ++c) {
         const uint32_t b = s.seg_begin[c], n = s.seg_count[c];
@@ -186,6 +199,13 @@ void Resorter::run(const HostSystem &s, void *slab, const SlabLayout &L, const S
         hip_check(hipMemcpyAsync(slab_part(slab, L, pb.first), w + pb.second, L.bytes[pb.first],
                                  hipMemcpyDeviceToDevice, stream),
                   "re-sort copy");
+}
+
+void Resorter::saveSlots(const HostSystem &s, const SweepBuffers &buf, uint32_t *slot_of_id, hipStream_t stream)
+{
+    const dim3 grid((s.n_pad + kResortBlock - 1) / kResortBlock), block(kResortBlock);
+    hipLaunchKernelGGL(k_save_slots, grid, block, 0, stream, buf.ids, s.n_pad, slot_of_id);
+    hip_check(hipGetLastError(), "save slots");
 }
 
 }  // namespace pscbfm

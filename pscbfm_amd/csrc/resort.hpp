@@ -34,9 +34,12 @@ public:
     // Re-sorts the system held in `slab` (layout L, kernels' view buf) on
     // `stream`, in the workspace reserve() made (radix sort of rocPRIM): by
     // the lattice site of each monomer's corner, or with by_id by original id
-    // (the layout without spatial order).
+    // (the layout without spatial order), or with slot_of_id into the slots
+    // it gives by original id (a layout saved by saveSlots, restored).
     void run(const HostSystem &s, void *slab, const SlabLayout &L, const SweepBuffers &buf, hipStream_t stream,
-             bool by_id = false);
+             bool by_id = false, const uint32_t *slot_of_id = nullptr);
+    // slot_of_id[id] = the slot of monomer id now (s.n entries), on `stream`.
+    static void saveSlots(const HostSystem &s, const SweepBuffers &buf, uint32_t *slot_of_id, hipStream_t stream);
     // Frees the workspace (call with the device current).
     void release();
     size_t workspaceBytes() const { return mWorkBytes + mSortBytes; }

@@ -88,6 +88,7 @@ public:
     // without it each segment starts with default_perform_group of its size
     void setPerformGroup(uint32_t group);
     uint32_t defaultGroup(uint32_t colour) const;
+    uint32_t defaultItems(uint32_t colour) const;
     uint32_t performGroup() const { return mPerformGroup; }
     // check workgroup size (64, 128, 256, 512 or 1024 threads), for every colour segment
     void setCheckBlock(uint32_t block);
@@ -176,6 +177,7 @@ private:
     bool mXcdRanges = true;
     uint32_t mPerformGroup = kPerformGroup;
     bool mPerformGroupSet = false;    // setPerformGroup called: every segment uses it
+    bool mCheckItemsSet = false;      // setCheckItems called: every segment uses it
     uint32_t mResidentWaves = 8192;   // 256 CUs x 4 SIMDs x 8 waves; queried at initialize
     uint32_t mCheckBlock = kBlock;
     uint32_t mCheckItems = kCheckItems;

@@ -202,7 +202,8 @@ class UpdaterGPUScBFM_AB_Type:
         _capi.check(self._lib.bfm_set_check_block(self._h, int(block)))
 
     def setCheckItems(self, items: int) -> None:
-        """Monomers per lane of the check kernel (1 or 2) for every colour segment."""
+        """Monomers per lane of the check kernel (1 or 2) for every colour
+        segment (default: 2 for segments of about 2 M monomers or more, else 1)."""
         _capi.check(self._lib.bfm_set_check_items(self._h, int(items)))
 
     def setSegmentConfig(self, colour: int, block: int, group: int, items: int | None = None) -> None:
