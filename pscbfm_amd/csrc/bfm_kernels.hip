@@ -287,12 +287,42 @@ __device__ __forceinline__ void compact(const SweepBuffers &buf, uint32_t run, b
 // more.  A wave issues its part first, before its own loads: the stores then
 // drain while it waits on them, instead of at the end of the last waves
 // (27.2 -> 26.5 us per check launch on the relaxed 512^3 melt).
-__device__ __forceinline__ void clear_previous_claims(const SweepBuffers &buf, const CheckArrays &a, uint32_t t,
-                                                      uint32_t threads)
+struct ClearPart {
+    uint32_t t, threads;   // this thread's number among the part's threads
+    uint32_t lo, hi;       // the part: 16-byte words [lo, hi) of the claim lattice
+};
+__device__ __forceinline__ void clear_previous_claims(const CheckArrays &a, const ClearPart &p)
 {
     const gptr<uint4> cp = reinterpret_cast<gptr<uint4>>(a.claim_prev);
-    for (uint32_t k = t; k < buf.lattice_words / 4u; k += threads)
+    for (uint32_t k = p.lo + p.t; k < p.hi; k += p.threads)
         element(cp + k) = make_uint4(0u, 0u, 0u, 0u);
+}
+// The whole lattice over the launch's nblk blocks, block blk of them.
+__device__ __forceinline__ ClearPart whole_clear(const SweepBuffers &buf, uint32_t blk, uint32_t nblk,
+                                                 uint32_t threads_per_block)
+{
+    return ClearPart{blk * threads_per_block + threadIdx.x, nblk * threads_per_block, 0u, buf.lattice_words / 4u};
+}
+// With XCD ranges: the blocks on XCD x (physical blocks x, x + 8, ...) clear
+// the x-th eighth of the lattice, the region its own range of Morton-ordered
+// monomers lies in, so that every XCD stores an equal share.  Numbered by
+// logical block over the whole lattice, the first blocks, which alone store
+// where the lattice has fewer 16-byte words than the launch has threads, are
+// the ranges of the first XCDs.  For the launches of two runs per lane (4
+// and more generations of waves) the equal shares measured faster (26.5 ->
+// 25.7 us on the relaxed 512^3 melt); for one run per lane (about one
+// generation: the network's 532,480-monomer colours) slower (7.7 -> 8.3 us),
+// and so did every other spread tried there, so those keep the whole-lattice
+// numbering.
+__device__ __forceinline__ ClearPart xcd_clear(const SweepBuffers &buf, uint32_t nblk, uint32_t threads_per_block)
+{
+    if (!buf.xcd_ranges || nblk < kXcds)
+        return whole_clear(buf, blockIdx.x, nblk, threads_per_block);
+    const uint32_t x = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
+    const uint32_t n_x = nblk / kXcds + (x < nblk % kXcds ? 1u : 0u);
+    const uint32_t w = buf.lattice_words / 4u;
+    return ClearPart{i * threads_per_block + threadIdx.x, n_x * threads_per_block, uint32_t(uint64_t(x) * w / kXcds),
+                     uint32_t(uint64_t(x + 1) * w / kXcds)};
 }
 
 // Phase 1 for check block `blk` of `nblk` covering the colour segment
@@ -307,13 +337,14 @@ __device__ __forceinline__ void clear_previous_claims(const SweepBuffers &buf, c
 // lane active; only lanes inside the segment may pass.
 template <bool kAllPeriodic, bool kCubic, uint32_t kThreads>
 __device__ __forceinline__ void check_body_one(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
-                                               uint32_t sub_seed, uint32_t parity, uint32_t blk, uint32_t nblk)
+                                               uint32_t sub_seed, uint32_t parity, uint32_t blk, uint32_t nblk,
+                                               const ClearPart &clear)
 {
     const CheckArrays a(buf, parity);
     const uint32_t gwave = wave_uniform(blk * (kThreads / kWave) + (threadIdx.x >> 6));
     const uint32_t t = blk * kThreads + threadIdx.x;
     const WaveBondMask bonds(buf.tables);
-    clear_previous_claims(buf, a, t, nblk * kThreads);
+    clear_previous_claims(a, clear);
     bool pass = false;
     uint2 entry = make_uint2(0u, 0u);
     if (count != 0) {
@@ -359,13 +390,14 @@ __device__ __forceinline__ void check_body_one(const SweepBuffers &buf, uint32_t
 // melt.)
 template <bool kAllPeriodic, bool kCubic, uint32_t kThreads, uint32_t kItems>
 __device__ __forceinline__ void check_body_runs(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
-                                                uint32_t sub_seed, uint32_t parity, uint32_t blk, uint32_t nblk)
+                                                uint32_t sub_seed, uint32_t parity, uint32_t blk, uint32_t nblk,
+                                                const ClearPart &clear)
 {
     const CheckArrays a(buf, parity);
     const uint32_t gwave = wave_uniform(blk * (kThreads / kWave) + (threadIdx.x >> 6));
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const WaveBondMask bonds(buf.tables);
-    clear_previous_claims(buf, a, blk * kThreads + threadIdx.x, nblk * kThreads);
+    clear_previous_claims(a, clear);
     bool pass[kItems];
     uint2 entry[kItems];
 #pragma unroll
@@ -421,12 +453,14 @@ __device__ __forceinline__ void check_body_runs(const SweepBuffers &buf, uint32_
 // Phase 1 with kItems monomers per lane.
 template <bool kAllPeriodic, bool kCubic, uint32_t kThreads = kBlock, uint32_t kItems = 1>
 __device__ __forceinline__ void check_body(const SweepBuffers &buf, uint32_t begin, uint32_t count, uint64_t age,
-                                           uint32_t sub_seed, uint32_t parity, uint32_t blk, uint32_t nblk)
+                                           uint32_t sub_seed, uint32_t parity, uint32_t blk, uint32_t nblk,
+                                           const ClearPart &clear)
 {
     if constexpr (kItems == 1)
-        check_body_one<kAllPeriodic, kCubic, kThreads>(buf, begin, count, age, sub_seed, parity, blk, nblk);
+        check_body_one<kAllPeriodic, kCubic, kThreads>(buf, begin, count, age, sub_seed, parity, blk, nblk, clear);
     else
-        check_body_runs<kAllPeriodic, kCubic, kThreads, kItems>(buf, begin, count, age, sub_seed, parity, blk, nblk);
+        check_body_runs<kAllPeriodic, kCubic, kThreads, kItems>(buf, begin, count, age, sub_seed, parity, blk, nblk,
+                                                                clear);
 }
 
 template <bool kAllPeriodic, bool kCubic, uint32_t kThreads, uint32_t kItems>
@@ -437,7 +471,9 @@ __global__ __launch_bounds__(kThreads) void k_check(SweepBuffers buf, uint32_t b
     // from the launch's implicit arguments, which are read after a branch
     const uint32_t nblk = check_blocks(count, kThreads * kItems);
     const uint32_t blk = buf.xcd_ranges ? xcd_logical(blockIdx.x, nblk) : blockIdx.x;
-    check_body<kAllPeriodic, kCubic, kThreads, kItems>(buf, begin, count, age, sub_seed, parity, blk, nblk);
+    check_body<kAllPeriodic, kCubic, kThreads, kItems>(
+        buf, begin, count, age, sub_seed, parity, blk, nblk,
+        kItems == 1 ? whole_clear(buf, blk, nblk, kThreads) : xcd_clear(buf, nblk, kThreads));
 }
 
 // Phase 2: every wave of perform block `blk` takes the survivors of kGroup
@@ -648,7 +684,8 @@ __global__ __launch_bounds__(kBlock) void k_check_batch(const BatchSystem *__res
     if (e.colour == kNoColour || be.blk >= nb)
         return;
     const uint32_t blk = be.blk;
-    check_body<kAllPeriodic, kCubic>(buf, e.begin, e.count, e.age, e.sub_seed, e.parity, blk, nb);
+    check_body<kAllPeriodic, kCubic>(buf, e.begin, e.count, e.age, e.sub_seed, e.parity, blk, nb,
+                                     whole_clear(buf, blk, nb, kBlock));
 }
 
 template <bool kCubic, uint32_t kGroup>
