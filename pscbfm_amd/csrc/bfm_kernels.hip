@@ -281,12 +281,13 @@ __device__ __forceinline__ void compact(const SweepBuffers &buf, uint32_t run, b
 
 // The previous substep's claims (the reference zeroes its tentative lattice in
 // a third kernel, :1240-1262): nobody reads claim[parity^1] in this kernel, so
-// the grid's `threads` threads (this one is number t) clear all of it with
-// coalesced 16-byte stores; at the densities of a melt a survivor's mark
-// shares every 64-byte line with others, so clearing marks one by one writes
-// more.  A wave issues its part first, before its own loads: the stores then
-// drain while it waits on them, instead of at the end of the last waves
-// (27.2 -> 26.5 us per check launch on the relaxed 512^3 melt).
+// the grid's threads clear all of it, each its share of one part of the
+// lattice (a ClearPart), with coalesced 16-byte stores; at the densities of a
+// melt a survivor's mark shares every 64-byte line with others, so clearing
+// marks one by one writes more.  A wave issues its share first, before its own
+// loads: the stores then drain while it waits on them, instead of at the end
+// of the last waves (27.2 -> 26.5 us per check launch on the relaxed 512^3
+// melt).
 struct ClearPart {
     uint32_t t, threads;   // this thread's number among the part's threads
     uint32_t lo, hi;       // the part: 16-byte words [lo, hi) of the claim lattice
